@@ -1,0 +1,583 @@
+// bh_api.cpp — C ABI implementation: contexts, device textures, scene
+// validation and upload, kernel launches, camera-path graphs, timing.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bh_api.h"
+#include "bh_internal.h"
+
+using bh::DevOut;
+using bh::DevQueue;
+using bh::DevScene;
+using bh::DevTex;
+
+struct DeviceTexture {
+  uint32_t* d = nullptr;
+  int rows = 0, cols = 0;
+};
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct bh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  DeviceTexture tex[BH_MAX_TEXTURES];
+  DevBuf scene, rgb, hit, phi, hitpt, steps, qitems, qcount;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  char name[256] = {0};
+  int n_cu = 0, clock_khz = 0;
+};
+
+struct bh_path {
+  bh_ctx* ctx = nullptr;
+  int n_frames = 0, W = 0, H = 0;
+  DevScene* d_scenes = nullptr;
+  uint8_t* d_rgb = nullptr;
+  uint8_t* d_hit = nullptr;
+  uint32_t* d_qitems = nullptr;
+  uint32_t* d_qcount = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+const char* g_static_err = "";
+thread_local std::string g_tls_err;
+
+int fail(bh_ctx* ctx, int code, const std::string& msg) {
+  if (ctx)
+    ctx->err = msg;
+  else
+    g_tls_err = msg;
+  return code;
+}
+
+int hip_fail(bh_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(ctx, e == hipErrorOutOfMemory ? BH_ERR_OUT_OF_MEMORY : BH_ERR_HIP, m);
+}
+
+#define HIP_TRY(ctx, expr)                          \
+  do {                                              \
+    hipError_t e_ = (expr);                         \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+int ensure(bh_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes) return BH_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+  b.bytes = bytes;
+  return BH_OK;
+}
+
+bool finite3(const double* v) {
+  return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+
+DevTex devtex(const bh_ctx* ctx, int slot) {
+  DevTex t{nullptr, 0, 0};
+  if (slot >= 0 && slot < BH_MAX_TEXTURES && ctx->tex[slot].d) {
+    t.data = ctx->tex[slot].d;
+    t.rows = ctx->tex[slot].rows;
+    t.cols = ctx->tex[slot].cols;
+  }
+  return t;
+}
+
+void copy3(double* dst, const double* src) { std::memcpy(dst, src, 3 * sizeof(double)); }
+
+// Options with zeros replaced by the reference defaults.
+bh_options normalized(const bh_options* o) {
+  bh_options r;
+  std::memset(&r, 0, sizeof r);
+  if (o) r = *o;
+  if (r.nstep == 0) r.nstep = 20;
+  if (r.escape && r.step_cap == 0) r.step_cap = 10000;
+  return r;
+}
+
+// Validate and resolve one scene into the device layout.
+int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevScene* d) {
+  if (!sc) return fail(ctx, BH_ERR_INVALID_ARG, "scene is NULL");
+  const bh_options o = normalized(opt_in);
+  const bh_camera& c = sc->camera;
+  if (c.width <= 0 || c.height <= 0 || c.width > 32768 || c.height > 32768)
+    return fail(ctx, BH_ERR_INVALID_ARG, "camera width/height must be in 1..32768");
+  if ((int64_t)c.width * c.height > (int64_t)1 << 30)
+    return fail(ctx, BH_ERR_INVALID_ARG, "frame larger than 2^30 pixels");
+  if (!finite3(c.position) || !finite3(c.axis_x) || !finite3(c.axis_y) || !finite3(c.axis_z) ||
+      !std::isfinite(c.fov))
+    return fail(ctx, BH_ERR_INVALID_ARG, "camera pose/fov must be finite");
+  if (!(sc->hole.mass > 0.0) || !std::isfinite(sc->hole.mass) || !finite3(sc->hole.position))
+    return fail(ctx, BH_ERR_INVALID_ARG, "hole mass must be positive and finite");
+  if (o.precision != BH_PREC_FP64 && o.precision != BH_PREC_FP32)
+    return fail(ctx, BH_ERR_INVALID_ARG, "unknown precision");
+  if (o.nstep < 2 || o.nstep > 100000) return fail(ctx, BH_ERR_INVALID_ARG, "nstep must be in 2..100000");
+  if (o.filter != BH_FILTER_NEAREST && o.filter != BH_FILTER_BILINEAR)
+    return fail(ctx, BH_ERR_INVALID_ARG, "unknown filter");
+  if (o.root != BH_ROOT_BISECT20 && o.root != BH_ROOT_EXACT)
+    return fail(ctx, BH_ERR_INVALID_ARG, "unknown root mode");
+  if (o.escape && (!(o.r_escape > 0.0) || o.step_cap <= 0 || o.step_cap > 1000000))
+    return fail(ctx, BH_ERR_INVALID_ARG, "escape needs r_escape > 0 and step_cap in 1..1e6");
+  if (!(o.band >= 0.0)) return fail(ctx, BH_ERR_INVALID_ARG, "band must be >= 0");
+  if (sc->n_spheres < 0 || sc->n_spheres > BH_MAX_SPHERES)
+    return fail(ctx, BH_ERR_INVALID_ARG, "n_spheres out of range");
+
+  std::memset(d, 0, sizeof *d);
+  copy3(d->C, c.position);
+  copy3(d->vy, c.axis_y);
+  copy3(d->vz, c.axis_z);
+  const double f = bh_focal_length(c.width, c.fov);
+  for (int i = 0; i < 3; ++i) d->fv[i] = c.axis_x[i] * f;  // Camera::focus_vector
+  d->W = c.width;
+  d->H = c.height;
+  copy3(d->B, sc->hole.position);
+  for (int i = 0; i < 3; ++i) d->F[i] = c.position[i] - sc->hole.position[i];
+  d->M = sc->hole.mass;
+  d->bc = 3.0 * std::sqrt(3.0) * d->M;  // b_c (blackhole_solution.h:25)
+  d->Rh = 2 * d->M;                     // radius() (blackhole_solution.h:57)
+  d->l0 = std::cbrt(DBL_EPSILON);       // epsilon<double>() (utility.h:17-21)
+  const double r0 = std::sqrt(d->F[0] * d->F[0] + d->F[1] * d->F[1] + d->F[2] * d->F[2]);
+  if (!(r0 > 0.0)) return fail(ctx, BH_ERR_INVALID_ARG, "camera coincides with the hole");
+  d->u0 = 1.0 / r0;
+
+  d->has_disc = sc->has_disc ? 1 : 0;
+  if (d->has_disc) {
+    const bh_annulus& a = sc->disc;
+    for (int i = 0; i < 5; ++i)
+      if (!finite3(a.vertex[i])) return fail(ctx, BH_ERR_INVALID_ARG, "disc vertices must be finite");
+    copy3(d->dv0, a.vertex[0]);
+    copy3(d->dv1, a.vertex[1]);
+    copy3(d->dv2, a.vertex[2]);
+    copy3(d->dv4, a.vertex[4]);
+    copy3(d->dn, a.normal);
+    d->r_in = a.r_inner;
+    d->r_out = a.r_outer;
+    d->disc_tex = devtex(ctx, a.texture);
+    if (!d->disc_tex.data)
+      return fail(ctx, BH_ERR_EMPTY_TEXTURE, "disc texture slot is empty (Material::SetTexture)");
+  }
+  d->n_spheres = sc->n_spheres;
+  for (int i = 0; i < sc->n_spheres; ++i) {
+    const bh_sphere& s = sc->spheres[i];
+    if (!finite3(s.center) || !(s.radius > 0.0))
+      return fail(ctx, BH_ERR_INVALID_ARG, "sphere centre/radius invalid");
+    copy3(d->sc[i], s.center);
+    d->sr[i] = s.radius;
+    d->stex[i] = devtex(ctx, s.texture);
+    if (!d->stex[i].data)
+      return fail(ctx, BH_ERR_EMPTY_TEXTURE, "sphere texture slot is empty");
+  }
+  d->has_bg = 0;
+  if (sc->background >= 0) {
+    d->bg = devtex(ctx, sc->background);
+    if (!d->bg.data) return fail(ctx, BH_ERR_EMPTY_TEXTURE, "background texture slot is empty");
+    d->has_bg = 1;
+  }
+  d->nstep = o.nstep;
+  d->escape = o.escape ? 1 : 0;
+  d->step_cap = o.step_cap;
+  d->filter = o.filter;
+  d->root = o.root;
+  d->precision = o.precision;
+  d->r_esc = o.r_escape;
+  d->band = o.band;
+  d->edge = o.edge_width;
+  return BH_OK;
+}
+
+// The FP32 event march assumes the disc is centred on the hole (its plane
+// passes through B, so every crossing is an angle in the orbit plane) and
+// renders no spheres yet; other scenes take the FP64 march.
+bool fp32_eligible(const DevScene& d) {
+  if (d.precision != BH_PREC_FP32) return false;
+  if (d.n_spheres > 0) return false;
+  if (d.filter != BH_FILTER_NEAREST) return false;
+  if (d.has_disc && (d.dv0[0] != d.B[0] || d.dv0[1] != d.B[1] || d.dv0[2] != d.B[2])) return false;
+  return true;
+}
+
+int launch_frame(bh_ctx* ctx, const DevScene& host_scene, DevScene* d_scene, const DevOut& out,
+                 uint32_t* qitems, uint32_t* qcount, uint32_t qcap, hipStream_t st) {
+  const int W = host_scene.W, H = host_scene.H;
+  if (fp32_eligible(host_scene)) {
+    HIP_TRY(ctx, bh::launch_zero_u32(qcount, 1, st));
+    DevQueue q{qitems, qcount, qcap};
+    HIP_TRY(ctx, bh::launch_march_fp32(d_scene, out, W, H, q, st, host_scene.escape != 0));
+    HIP_TRY(ctx, bh::launch_band_fp64(d_scene, out, q, st));
+  } else {
+    HIP_TRY(ctx, bh::launch_march_fp64(d_scene, out, W, H, st));
+  }
+  return BH_OK;
+}
+
+int prepare_queue(bh_ctx* ctx, size_t npix) {
+  int rc = ensure(ctx, ctx->qitems, npix * sizeof(uint32_t));
+  if (rc) return rc;
+  return ensure(ctx, ctx->qcount, 256);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bh_ctx_create(int device, bh_ctx** out) {
+  if (!out) return fail(nullptr, BH_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return fail(nullptr, BH_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(nullptr, BH_ERR_NO_DEVICE, "device ordinal out of range");
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(nullptr, BH_ERR_HIP, "hipSetDevice failed");
+  bh_ctx* c = new bh_ctx();
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(nullptr, BH_ERR_HIP, "hipStreamCreate failed");
+  }
+  c->own_stream = true;
+  (void)hipEventCreate(&c->ev0);
+  (void)hipEventCreate(&c->ev1);
+  (void)hipEventCreate(&c->ev2);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    std::snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
+    c->n_cu = prop.multiProcessorCount;
+    c->clock_khz = prop.clockRate;
+  }
+  *out = c;
+  return BH_OK;
+}
+
+void bh_ctx_destroy(bh_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& t : ctx->tex)
+    if (t.d) (void)hipFree(t.d);
+  for (DevBuf* b : {&ctx->scene, &ctx->rgb, &ctx->hit, &ctx->phi, &ctx->hitpt, &ctx->steps,
+                    &ctx->qitems, &ctx->qcount})
+    if (b->p) (void)hipFree(b->p);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* bh_last_error(const bh_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_tls_err.empty() ? g_static_err : g_tls_err.c_str();
+}
+
+int bh_device_info(bh_ctx* ctx, char* name, int name_len, int* n_cu, int* clock_khz) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", ctx->name);
+  if (n_cu) *n_cu = ctx->n_cu;
+  if (clock_khz) *clock_khz = ctx->clock_khz;
+  return BH_OK;
+}
+
+int bh_ctx_set_stream(bh_ctx* ctx, void* stream) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  (void)hipSetDevice(ctx->device);
+  if (ctx->own_stream && ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  if (stream) {
+    ctx->stream = (hipStream_t)stream;
+    ctx->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamCreate");
+    ctx->own_stream = true;
+  }
+  return BH_OK;
+}
+
+int bh_synchronize(bh_ctx* ctx) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return BH_OK;
+}
+
+int bh_texture_set(bh_ctx* ctx, int slot, const bh_image* img) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (slot < 0 || slot >= BH_MAX_TEXTURES) return fail(ctx, BH_ERR_INVALID_ARG, "texture slot out of range");
+  if (!img || !img->data || img->rows <= 0 || img->cols <= 0)
+    return fail(ctx, BH_ERR_EMPTY_TEXTURE, "empty texture (Material::SetTexture)");
+  if (img->channels != 3 && img->channels != 4)
+    return fail(ctx, BH_ERR_INVALID_ARG, "texture channels must be 3 or 4");
+  if ((int64_t)img->rows * img->cols > ((int64_t)1 << 28))
+    return fail(ctx, BH_ERR_INVALID_ARG, "texture too large");
+  (void)hipSetDevice(ctx->device);
+  const size_t n = (size_t)img->rows * img->cols;
+  std::vector<uint32_t> packed(n);
+  const int C = img->channels;
+  for (size_t i = 0; i < n; ++i) {
+    const uint8_t* p = img->data + i * C;
+    packed[i] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+                ((C == 4 ? (uint32_t)p[3] : 255u) << 24);
+  }
+  DeviceTexture& t = ctx->tex[slot];
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (t.d && (size_t)t.rows * t.cols != n) {
+    (void)hipFree(t.d);
+    t.d = nullptr;
+  }
+  if (!t.d) HIP_TRY(ctx, hipMalloc(&t.d, n * sizeof(uint32_t)));
+  HIP_TRY(ctx, hipMemcpy(t.d, packed.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  t.rows = img->rows;
+  t.cols = img->cols;
+  return BH_OK;
+}
+
+int bh_texture_clear(bh_ctx* ctx, int slot) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (slot < 0 || slot >= BH_MAX_TEXTURES) return fail(ctx, BH_ERR_INVALID_ARG, "texture slot out of range");
+  DeviceTexture& t = ctx->tex[slot];
+  if (t.d) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(t.d);
+  }
+  t = DeviceTexture{};
+  return BH_OK;
+}
+
+int bh_scene_check(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  DevScene d;
+  return build_scene(ctx, scene, opt, &d);
+}
+
+int bh_render_device(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                     const bh_outputs* out_device) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (!out_device) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
+  DevScene d;
+  int rc = build_scene(ctx, scene, opt, &d);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
+  const size_t npix = (size_t)d.W * d.H;
+  if ((rc = prepare_queue(ctx, npix))) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
+  DevOut o{out_device->rgb, out_device->hit, out_device->phi_end, out_device->hit_point,
+           out_device->steps};
+  return launch_frame(ctx, d, (DevScene*)ctx->scene.p, o, (uint32_t*)ctx->qitems.p,
+                      (uint32_t*)ctx->qcount.p, (uint32_t)npix, ctx->stream);
+}
+
+int bh_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, const bh_outputs* out) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
+  DevScene d;
+  int rc = build_scene(ctx, scene, opt, &d);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const size_t npix = (size_t)d.W * d.H;
+  DevOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (out->rgb) { if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc; o.rgb = (uint8_t*)ctx->rgb.p; }
+  if (out->hit) { if ((rc = ensure(ctx, ctx->hit, npix))) return rc; o.hit = (uint8_t*)ctx->hit.p; }
+  if (out->phi_end) { if ((rc = ensure(ctx, ctx->phi, npix * 8))) return rc; o.phi = (double*)ctx->phi.p; }
+  if (out->hit_point) { if ((rc = ensure(ctx, ctx->hitpt, npix * 24))) return rc; o.hitpt = (double*)ctx->hitpt.p; }
+  if (out->steps) { if ((rc = ensure(ctx, ctx->steps, npix * 4))) return rc; o.steps = (int32_t*)ctx->steps.p; }
+  bh_outputs dev{o.rgb, o.hit, o.phi, o.hitpt, o.steps};
+  if ((rc = bh_render_device(ctx, scene, opt, &dev))) return rc;
+  hipStream_t st = ctx->stream;
+  if (out->rgb) HIP_TRY(ctx, hipMemcpyAsync(out->rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, st));
+  if (out->hit) HIP_TRY(ctx, hipMemcpyAsync(out->hit, o.hit, npix, hipMemcpyDeviceToHost, st));
+  if (out->phi_end) HIP_TRY(ctx, hipMemcpyAsync(out->phi_end, o.phi, npix * 8, hipMemcpyDeviceToHost, st));
+  if (out->hit_point) HIP_TRY(ctx, hipMemcpyAsync(out->hit_point, o.hitpt, npix * 24, hipMemcpyDeviceToHost, st));
+  if (out->steps) HIP_TRY(ctx, hipMemcpyAsync(out->steps, o.steps, npix * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return BH_OK;
+}
+
+int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, int iters,
+                   double* avg_ms_main, double* avg_ms_total, int64_t* band_rays) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
+  DevScene d;
+  int rc = build_scene(ctx, scene, opt, &d);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const size_t npix = (size_t)d.W * d.H;
+  if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc;
+  if ((rc = ensure(ctx, ctx->hit, npix))) return rc;
+  if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
+  if ((rc = prepare_queue(ctx, npix))) return rc;
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, st));
+  DevOut o{(uint8_t*)ctx->rgb.p, (uint8_t*)ctx->hit.p, nullptr, nullptr, nullptr};
+  DevScene* ds = (DevScene*)ctx->scene.p;
+  uint32_t* qi = (uint32_t*)ctx->qitems.p;
+  uint32_t* qc = (uint32_t*)ctx->qcount.p;
+  const bool fast = fp32_eligible(d);
+  // warm-up
+  if ((rc = launch_frame(ctx, d, ds, o, qi, qc, (uint32_t)npix, st))) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  double main_ms = 0, total_ms = 0;
+  for (int it = 0; it < iters; ++it) {
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    if (fast) {
+      HIP_TRY(ctx, bh::launch_zero_u32(qc, 1, st));
+      DevQueue q{qi, qc, (uint32_t)npix};
+      HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+      HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0));
+      HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+      HIP_TRY(ctx, bh::launch_band_fp64(ds, o, q, st));
+    } else {
+      HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+      HIP_TRY(ctx, bh::launch_march_fp64(ds, o, d.W, d.H, st));
+      HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+    }
+    hipEvent_t end;
+    HIP_TRY(ctx, hipEventCreate(&end));
+    HIP_TRY(ctx, hipEventRecord(end, st));
+    HIP_TRY(ctx, hipEventSynchronize(end));
+    float a = 0, b = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->ev1, ctx->ev2));
+    HIP_TRY(ctx, hipEventElapsedTime(&b, ctx->ev0, end));
+    (void)hipEventDestroy(end);
+    main_ms += a;
+    total_ms += b;
+  }
+  if (avg_ms_main) *avg_ms_main = main_ms / iters;
+  if (avg_ms_total) *avg_ms_total = total_ms / iters;
+  if (band_rays) {
+    uint32_t n = 0;
+    if (fast) HIP_TRY(ctx, hipMemcpy(&n, qc, sizeof n, hipMemcpyDeviceToHost));
+    *band_rays = n;
+  }
+  return BH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// camera paths as HIP graphs
+// ---------------------------------------------------------------------------
+int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                   const bh_camera* cameras, int n_frames, bh_path** out) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (!out || !scene || !cameras || n_frames <= 0)
+    return fail(ctx, BH_ERR_INVALID_ARG, "path needs scene, cameras and n_frames > 0");
+  *out = nullptr;
+  (void)hipSetDevice(ctx->device);
+  std::vector<DevScene> frames((size_t)n_frames);
+  for (int i = 0; i < n_frames; ++i) {
+    bh_scene s = *scene;
+    s.camera = cameras[i];
+    if (cameras[i].width != cameras[0].width || cameras[i].height != cameras[0].height)
+      return fail(ctx, BH_ERR_INVALID_ARG, "path cameras must share width/height");
+    int rc = build_scene(ctx, &s, opt, &frames[(size_t)i]);
+    if (rc) return rc;
+  }
+  bh_path* p = new bh_path();
+  p->ctx = ctx;
+  p->n_frames = n_frames;
+  p->W = frames[0].W;
+  p->H = frames[0].H;
+  const size_t npix = (size_t)p->W * p->H;
+  auto cleanup = [&](hipError_t e, const char* what) {
+    bh_path_destroy(p);
+    return hip_fail(ctx, e, what);
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&p->d_scenes, sizeof(DevScene) * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc scenes");
+  if ((e = hipMalloc(&p->d_rgb, npix * 3 * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc rgb");
+  if ((e = hipMalloc(&p->d_hit, npix * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc hit");
+  if ((e = hipMalloc(&p->d_qitems, npix * sizeof(uint32_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
+  if ((e = hipMalloc(&p->d_qcount, 256)) != hipSuccess) return cleanup(e, "hipMalloc count");
+  if ((e = hipMemcpy(p->d_scenes, frames.data(), sizeof(DevScene) * n_frames, hipMemcpyHostToDevice)) != hipSuccess)
+    return cleanup(e, "hipMemcpy scenes");
+  hipStream_t st = ctx->stream;
+  if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return cleanup(e, "capture");
+  for (int i = 0; i < n_frames; ++i) {
+    DevOut o{p->d_rgb + npix * 3 * i, p->d_hit + npix * i, nullptr, nullptr, nullptr};
+    int rc = launch_frame(ctx, frames[(size_t)i], p->d_scenes + i, o, p->d_qitems, p->d_qcount,
+                          (uint32_t)npix, st);
+    if (rc) {
+      hipGraph_t g;
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      std::string m = ctx->err;
+      bh_path_destroy(p);
+      return fail(ctx, rc, m);
+    }
+  }
+  if ((e = hipStreamEndCapture(st, &p->graph)) != hipSuccess) return cleanup(e, "end capture");
+  if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess)
+    return cleanup(e, "hipGraphInstantiate");
+  *out = p;
+  return BH_OK;
+}
+
+int bh_path_launch(bh_path* p, float* ms) {
+  if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
+  bh_ctx* ctx = p->ctx;
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  HIP_TRY(ctx, hipGraphLaunch(p->exec, st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  return BH_OK;
+}
+
+int bh_path_read(bh_path* p, int frame, uint8_t* rgb, uint8_t* hit) {
+  if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
+  bh_ctx* ctx = p->ctx;
+  if (frame < 0 || frame >= p->n_frames) return fail(ctx, BH_ERR_INVALID_ARG, "frame out of range");
+  (void)hipSetDevice(ctx->device);
+  const size_t npix = (size_t)p->W * p->H;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (rgb) HIP_TRY(ctx, hipMemcpy(rgb, p->d_rgb + npix * 3 * frame, npix * 3, hipMemcpyDeviceToHost));
+  if (hit) HIP_TRY(ctx, hipMemcpy(hit, p->d_hit + npix * frame, npix, hipMemcpyDeviceToHost));
+  return BH_OK;
+}
+
+void bh_path_destroy(bh_path* p) {
+  if (!p) return;
+  if (p->ctx) {
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+  }
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->d_scenes) (void)hipFree(p->d_scenes);
+  if (p->d_rgb) (void)hipFree(p->d_rgb);
+  if (p->d_hit) (void)hipFree(p->d_hit);
+  if (p->d_qitems) (void)hipFree(p->d_qitems);
+  if (p->d_qcount) (void)hipFree(p->d_qcount);
+  delete p;
+}
+
+int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* scene, const bh_outputs* out) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  (void)scene;
+  (void)out;
+  return fail(ctx, BH_ERR_UNSUPPORTED, "flat march not built yet");
+}
+
+}  // extern "C"

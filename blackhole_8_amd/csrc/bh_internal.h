@@ -1,0 +1,77 @@
+// bh_internal.h — device-side scene layout and kernel launchers shared by the
+// C ABI implementation (bh_api.cpp) and the HIP kernels (bh_march.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bh_api.h"
+
+namespace bh {
+
+// Texture in HBM: one packed little-endian RGBA word per texel (channel 0 in
+// the low byte), so every lookup is one aligned 4-byte load.
+struct DevTex {
+  const uint32_t* data;
+  int32_t rows, cols;
+};
+
+// Everything a ray needs, resolved on the host once per frame.  Lives in
+// device memory; every field is wave-uniform and is read with scalar loads.
+struct DevScene {
+  // camera (camera.h:30-63)
+  double C[3];
+  double vy[3], vz[3];
+  double fv[3];  // axis_x * focal length (Camera::focus_vector)
+  int32_t W, H;
+  // hole (blackhole_solution.h:24-25,57)
+  double B[3];
+  double F[3];     // C - B
+  double M, bc, Rh;
+  double l0;       // SolveG lower bracket cbrt(DBL_EPSILON) (utility.h:17-21)
+  double u0;       // 1 / |C - B|
+  // disc (Annulus)
+  int32_t has_disc, n_spheres;
+  double dv0[3], dv1[3], dv2[3], dv4[3];
+  double dn[3];
+  double r_in, r_out;
+  DevTex disc_tex;
+  // spheres
+  double sc[BH_MAX_SPHERES][3];
+  double sr[BH_MAX_SPHERES];
+  DevTex stex[BH_MAX_SPHERES];
+  // background
+  int32_t has_bg, pad0;
+  DevTex bg;
+  // options
+  int32_t nstep, escape, step_cap, filter, root, precision;
+  double r_esc, band, edge;
+};
+
+// Output planes (device pointers, any may be null).  Pixel (x, y) of the
+// frame lives at index y * W + x of each plane.
+struct DevOut {
+  uint8_t* rgb;
+  uint8_t* hit;
+  double* phi;
+  double* hitpt;
+  int32_t* steps;
+};
+
+// FP64 band queue written by the FP32 march: pixel indices of the rays whose
+// impact distance lies within the photon-sphere band.
+struct DevQueue {
+  uint32_t* items;
+  uint32_t* count;
+  uint32_t capacity;
+};
+
+hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, int H,
+                             hipStream_t stream);
+hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, int H,
+                             const DevQueue& q, hipStream_t stream, bool escape);
+hipError_t launch_band_fp64(const DevScene* scene, const DevOut& out, const DevQueue& q,
+                            hipStream_t stream);
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
+
+}  // namespace bh

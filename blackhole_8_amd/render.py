@@ -1,0 +1,126 @@
+"""Host-side API over the C ABI: a Renderer owns one bh_ctx (one HIP device).
+
+Mirrors the reference's render entry point (the inline march of
+blackhole_solution_test.cc:161-298) as `Renderer.render(scene, options)`.
+There is no CPU fallback: without the built HIP library or a GPU every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class BHError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bh error {code}: {msg}")
+        self.code = code
+
+
+OUTPUT_NAMES = ("rgb", "hit", "phi", "hitpt", "steps")
+
+
+def alloc_outputs(width: int, height: int, want=("rgb", "hit")):
+    shapes = {"rgb": ((height, width, 3), np.uint8), "hit": ((height, width), np.uint8),
+              "phi": ((height, width), np.float64), "hitpt": ((height, width, 3), np.float64),
+              "steps": ((height, width), np.int32)}
+    arrs = {k: np.zeros(*shapes[k]) for k in want}
+    o = abi.Outputs(abi.ptr(arrs.get("rgb")), abi.ptr(arrs.get("hit")), abi.ptr(arrs.get("phi")),
+                    abi.ptr(arrs.get("hitpt")), abi.ptr(arrs.get("steps")))
+    return arrs, o
+
+
+class Renderer:
+    def __init__(self, device: int = 0):
+        self.lib = abi.load()
+        self.ctx = C.c_void_p()
+        rc = self.lib.bh_ctx_create(device, C.byref(self.ctx))
+        if rc != abi.BH_OK:
+            raise BHError(rc, self.lib.bh_last_error(None).decode())
+        self._tex_keep = {}
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if self.ctx:
+            self.lib.bh_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != abi.BH_OK:
+            raise BHError(rc, self.lib.bh_last_error(self.ctx).decode())
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        cu, clk = C.c_int(), C.c_int()
+        self._check(self.lib.bh_device_info(self.ctx, name, 256, C.byref(cu), C.byref(clk)))
+        return {"name": name.value.decode(), "cu": cu.value, "clock_khz": clk.value}
+
+    # -- textures (Material::SetTexture) ----------------------------------
+    def set_texture(self, slot: int, img: np.ndarray):
+        img = np.ascontiguousarray(img, np.uint8)
+        self._check(self.lib.bh_texture_set(self.ctx, slot, C.byref(abi.image(img))))
+
+    def set_textures(self, textures: dict):
+        for slot, img in textures.items():
+            self.set_texture(slot, img)
+
+    # -- rendering ---------------------------------------------------------
+    def render(self, scene: abi.Scene, opts: abi.Options, want=("rgb", "hit")) -> dict:
+        w, h = scene.camera.width, scene.camera.height
+        arrs, o = alloc_outputs(w, h, want)
+        self._check(self.lib.bh_render(self.ctx, C.byref(scene), C.byref(opts), C.byref(o)))
+        return arrs
+
+    def render_preset(self, preset, want=("rgb", "hit")) -> dict:
+        self.set_textures(preset.textures)
+        return self.render(preset.scene, preset.options, want)
+
+    def check_scene(self, scene: abi.Scene, opts: abi.Options) -> int:
+        return self.lib.bh_scene_check(self.ctx, C.byref(scene), C.byref(opts))
+
+    def time_render(self, scene: abi.Scene, opts: abi.Options, iters: int = 10):
+        m, t, n = C.c_double(), C.c_double(), C.c_int64()
+        self._check(self.lib.bh_time_render(self.ctx, C.byref(scene), C.byref(opts), iters,
+                                            C.byref(m), C.byref(t), C.byref(n)))
+        return m.value, t.value, n.value
+
+    def path(self, scene: abi.Scene, opts: abi.Options, cameras) -> "Path":
+        return Path(self, scene, opts, cameras)
+
+
+class Path:
+    """A camera path captured as one HIP graph (bh_path_*)."""
+
+    def __init__(self, r: Renderer, scene: abi.Scene, opts: abi.Options, cameras):
+        self.r = r
+        self.n = len(cameras)
+        self.w, self.h = cameras[0].width, cameras[0].height
+        cams = (abi.Camera * self.n)(*cameras)
+        self.handle = C.c_void_p()
+        r._check(r.lib.bh_path_create(r.ctx, C.byref(scene), C.byref(opts), cams, self.n,
+                                      C.byref(self.handle)))
+
+    def launch(self) -> float:
+        ms = C.c_float()
+        self.r._check(self.r.lib.bh_path_launch(self.handle, C.byref(ms)))
+        return ms.value
+
+    def read(self, frame: int):
+        rgb = np.zeros((self.h, self.w, 3), np.uint8)
+        hit = np.zeros((self.h, self.w), np.uint8)
+        self.r._check(self.r.lib.bh_path_read(self.handle, frame, abi.ptr(rgb), abi.ptr(hit)))
+        return rgb, hit
+
+    def close(self):
+        if self.handle:
+            self.r.lib.bh_path_destroy(self.handle)
+            self.handle = C.c_void_p()

@@ -1,0 +1,276 @@
+/*
+ * bh_api.h — C ABI of the MI355X photon-geodesic renderer.
+ *
+ * This is the drop-in boundary for the per-pixel Schwarzschild march that the
+ * reference runs inline in main():
+ *   include/blackhole/blackhole_solution_test.cc:161-298   (the march)
+ *   include/blackhole/blackhole_solution.h:16-93           (StaticBlackhole)
+ *   include/blackhole/camera.h:21-71                       (Camera)
+ *   include/blackhole/object/object_manager.h:69-85        (FindCollision)
+ *   include/blackhole/object/vector_object.h:159-179,306-356 (Annulus + texel)
+ *   include/blackhole/ray_tracer.h:52-96                   (RayTracer::Prograde)
+ * The reference has no FFI of its own; its seams are C++ templates.  Every
+ * entry point below cites the reference interface it replaces.
+ *
+ * Conventions: plain C types only, caller-owned buffers, int status codes
+ * (BH_OK == 0, negative on error, message via bh_last_error), no exceptions
+ * across the boundary.  One context per host thread.
+ */
+#ifndef BH_API_H_
+#define BH_API_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_API_VERSION 1
+#define BH_MAX_SPHERES 32
+#define BH_MAX_TEXTURES 64
+#define BH_MAX_RECTS 8
+#define BH_MAX_PLANES 4
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  BH_OK = 0,
+  BH_ERR_INVALID_ARG = -1,
+  BH_ERR_NO_DEVICE = -2,
+  BH_ERR_HIP = -3,
+  BH_ERR_OUT_OF_MEMORY = -4,
+  BH_ERR_EMPTY_TEXTURE = -5, /* Material::SetTexture terminates (material.h:29-35) */
+  BH_ERR_UNSUPPORTED = -6,
+  BH_ERR_BAD_HANDLE = -7,
+  BH_ERR_NOT_BUILT = -8
+};
+
+/* ---- hit classes (one byte per pixel) ---------------------------------- */
+enum {
+  BH_HIT_NONE = 0,    /* nothing hit: black (reference) or background (escape) */
+  BH_HIT_HORIZON = 1, /* StaticBlackhole::Collide (blackhole_solution.h:65-88) */
+  BH_HIT_DISC = 2,    /* Annulus::Collide (vector_object.h:328-347) */
+  BH_HIT_OBJECT = 3   /* any other object: textured sphere, rectangle, plane */
+};
+
+/* ---- option enums ------------------------------------------------------ */
+enum {
+  BH_PREC_FP64 = 0, /* every step in FP64, reference operation order */
+  BH_PREC_FP32 = 1  /* FP32 march, FP64 set-up, FP64 photon-sphere band and hits */
+};
+enum { BH_FILTER_NEAREST = 0, BH_FILTER_BILINEAR = 1 };
+enum {
+  BH_ROOT_BISECT20 = 0, /* StaticBlackhole::SolveG: 20 bisections, lower end */
+  BH_ROOT_EXACT = 1     /* turning point to full FP64 precision (smooth in b, M) */
+};
+
+/* ---- images ------------------------------------------------------------ */
+/* Row-major rows x cols x channels bytes, channels 3 or 4.  Only the first
+ * three channels are used and are passed through in order, exactly like the
+ * reference's cv::imread(IMREAD_COLOR) texture (config.h:35-37) whose bytes
+ * go straight to the framebuffer (vector_object.h:176-178). */
+typedef struct bh_image {
+  const uint8_t* data;
+  int32_t rows, cols, channels;
+} bh_image;
+
+/* ---- scene ------------------------------------------------------------- */
+/* Camera<T> (camera.h:30-63): position and the three Object axes
+ * (object.h:112-114, camera default (1,0,0),(0,-1,0),(0,0,-1)). */
+typedef struct bh_camera {
+  double position[3];
+  double axis_x[3], axis_y[3], axis_z[3];
+  int32_t width, height;
+  double fov; /* radians; clamped to [0, nextafter(pi,0)] like Camera::fov */
+} bh_camera;
+
+/* StaticBlackhole<T>(position, mass): r_s = 2M (blackhole_solution.h:24,57) */
+typedef struct bh_hole {
+  double position[3];
+  double mass;
+} bh_hole;
+
+/* Annulus<T>(p1,p2,p3,p4,r_outer,r_inner) (vector_object.h:320-326).
+ * vertex[0] is the centre/pivot and vertex[1..4] are p1..p4 (the Object
+ * initializer-list constructor appends to the default vertex, object.h:38,109).
+ * normal is normalize((p3-p1) x (p2-p1)) taken at construction. */
+typedef struct bh_annulus {
+  double vertex[5][3];
+  double normal[3];
+  double r_outer, r_inner;
+  int32_t texture; /* texture slot */
+  int32_t pad_;
+} bh_annulus;
+
+/* Textured sphere (not in the reference; its Sphere::Collide is unfinished,
+ * vector_object.h:281-300).  Hit test = the horizon's segment/sphere test,
+ * texture = equirectangular map of the outward normal. */
+typedef struct bh_sphere {
+  double center[3];
+  double radius;
+  int32_t texture;
+  int32_t pad_;
+} bh_sphere;
+
+typedef struct bh_scene {
+  bh_camera camera;
+  bh_hole hole;
+  int32_t has_disc;
+  int32_t n_spheres;
+  bh_annulus disc;
+  bh_sphere spheres[BH_MAX_SPHERES];
+  int32_t background; /* texture slot of an equirect background, -1 = none */
+  int32_t pad_;
+} bh_scene;
+
+/* ---- options ----------------------------------------------------------- */
+typedef struct bh_options {
+  int32_t precision; /* BH_PREC_*                                           */
+  int32_t nstep;     /* reference: 20 (blackhole_solution_test.cc:202)      */
+  int32_t escape;    /* 0: march ends near the camera radius (reference);
+                        1: continue outbound until r >= r_escape or step_cap */
+  int32_t step_cap;  /* total steps per ray in escape mode (10000)          */
+  double r_escape;   /* absolute radius of the escape sphere                */
+  int32_t filter;    /* BH_FILTER_*                                         */
+  int32_t root;      /* BH_ROOT_*                                           */
+  double band;       /* FP32 mode: rays with |b/b_c - 1| < band march in FP64 */
+  double edge_width; /* bilinear mode only: soft disc-edge width (0 = hard) */
+} bh_options;
+
+/* ---- per-pixel outputs (NULL = not wanted) ------------------------------ */
+typedef struct bh_outputs {
+  uint8_t* rgb;       /* H*W*3, texture channel order (reference: BGR)      */
+  uint8_t* hit;       /* H*W hit class                                      */
+  double* phi_end;    /* H*W orbit-plane angle at the last step taken       */
+  double* hit_point;  /* H*W*3 world position of the hit (NaN when none)    */
+  int32_t* steps;     /* H*W integration steps taken                         */
+} bh_outputs;
+
+/* ---- flat-space scene (ray_tracer_test path, ray_tracer.h:52-96) -------- */
+typedef struct bh_rect {
+  double vertex[5][3]; /* [0] pivot, [1..4] = p1..p4 (vector_object.h:94-98) */
+  int32_t texture;
+  int32_t pad_;
+} bh_rect;
+
+typedef struct bh_plane {
+  double position[3];
+  double axis_x[3], axis_y[3], axis_z[3];
+  double pattern_size; /* ChessPattern2D(size) (pattern.h:14-50); 0 = black */
+} bh_plane;
+
+typedef struct bh_flat_scene {
+  bh_camera camera;
+  int32_t n_rects;
+  int32_t n_planes;
+  bh_rect rects[BH_MAX_RECTS];
+  bh_plane planes[BH_MAX_PLANES];
+  int32_t steps; /* Prograde step_size (ray_tracer_test.cc:145 uses 10) */
+  int32_t pad_;
+} bh_flat_scene;
+
+/* ======================================================================== */
+/* Context                                                                   */
+/* ======================================================================== */
+typedef struct bh_ctx bh_ctx;
+
+/* Replaces the reference's process-wide state (ObjectManager singleton,
+ * object_manager.h:30-33).  device = HIP ordinal. */
+int bh_ctx_create(int device, bh_ctx** out);
+void bh_ctx_destroy(bh_ctx* ctx);
+/* Message of the last failing call on ctx (static string when ctx is NULL). */
+const char* bh_last_error(const bh_ctx* ctx);
+/* Library version and the device name / CU count the context runs on. */
+int bh_version(void);
+int bh_device_info(bh_ctx* ctx, char* name, int name_len, int* n_cu, int* clock_khz);
+/* Run the context's work on a caller stream (hipStream_t), NULL = own stream. */
+int bh_ctx_set_stream(bh_ctx* ctx, void* stream);
+int bh_synchronize(bh_ctx* ctx);
+
+/* ======================================================================== */
+/* Textures — Material::SetTexture (material.h:29-35)                        */
+/* ======================================================================== */
+/* Upload img into slot (0..BH_MAX_TEXTURES-1); replaces the slot's content.
+ * Empty image -> BH_ERR_EMPTY_TEXTURE (the reference calls std::terminate). */
+int bh_texture_set(bh_ctx* ctx, int slot, const bh_image* img);
+int bh_texture_clear(bh_ctx* ctx, int slot);
+
+/* ======================================================================== */
+/* Rendering — the march of blackhole_solution_test.cc:161-298               */
+/* ======================================================================== */
+/* Validate a scene against the textures bound in ctx (no rendering). */
+int bh_scene_check(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt);
+
+/* Render one frame into host buffers (synchronous). */
+int bh_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+              const bh_outputs* out);
+
+/* Render one frame into device buffers on the context stream (asynchronous).
+ * Pointers in out are device addresses of the same shapes as bh_render's. */
+int bh_render_device(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                     const bh_outputs* out_device);
+
+/* ======================================================================== */
+/* Camera paths — one frame per camera, replayed as one HIP graph            */
+/* ======================================================================== */
+typedef struct bh_path bh_path;
+/* Capture n_frames renders of scene with cameras[i] into a graph writing
+ * rgb/hit planes of frame i at offset i*H*W*{3,1} of device buffers owned by
+ * the path.  All cameras must share width/height. */
+int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                   const bh_camera* cameras, int n_frames, bh_path** out);
+/* Launch the captured graph once; ms (optional) = GPU time from HIP events. */
+int bh_path_launch(bh_path* path, float* ms);
+/* Copy frame i back to host buffers (rgb H*W*3, hit H*W; either may be NULL). */
+int bh_path_read(bh_path* path, int frame, uint8_t* rgb, uint8_t* hit);
+void bh_path_destroy(bh_path* path);
+
+/* ======================================================================== */
+/* Kernel timing for the measurement harness                                 */
+/* ======================================================================== */
+/* Average duration in ms of the main march kernel over iters eager launches
+ * on the context stream, bracketed by HIP events on that stream.  The frame
+ * is rendered into device buffers owned by ctx. */
+int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                   int iters, double* avg_ms_main, double* avg_ms_total,
+                   int64_t* band_rays);
+
+/* ======================================================================== */
+/* Flat-space march — RayTracer::Prograde (ray_tracer.h:52-96)              */
+/* ======================================================================== */
+int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* scene, const bh_outputs* out);
+
+/* ======================================================================== */
+/* Host-side helpers shared by every caller (no GPU needed)                  */
+/* ======================================================================== */
+/* Camera::fov + focal length W/(2 tan(fov/2)) (camera.h:37-41). */
+double bh_focal_length(int width, double fov);
+/* Object::Rotate{X,Y,Z} with RotationMatrixForAxis (object.h:58-80,
+ * matrix.h:342-356).  axes = {x[3], y[3], z[3]}, verts = n x 3 rotated about
+ * verts[0] (NULL/0 for a bare frame).  which: 0 = X, 1 = Y, 2 = Z. */
+void bh_object_rotate(double axes[9], double* verts, int n_verts, int which, double rad);
+/* Camera frame looking from position at target with world up vector: sets
+ * axis_x toward target, axis_z = -up (image top), axis_y = axis_z x axis_x,
+ * matching the default camera's handedness (camera.h:32). */
+void bh_camera_look_at(bh_camera* cam, const double target[3], const double up[3]);
+/* Annulus normal and vertex list from p1..p4 around centre (0,0,0)+offset. */
+void bh_annulus_init(bh_annulus* a, const double p1[3], const double p2[3],
+                     const double p3[3], const double p4[3], double r_outer,
+                     double r_inner, int texture);
+
+/* Seeded generators (SplitMix64) shared by every test and benchmark input. */
+uint64_t bh_splitmix64(uint64_t* state);
+double bh_uniform(uint64_t* state, double lo, double hi); /* [lo, hi) */
+/* 2048x1024-style equirect background: uniform u8 noise overlaid with a
+ * cells_x x cells_y checkerboard (dark cells scaled to 1/4, light cells
+ * lifted by 3/4 toward 255).  channels = 3. */
+void bh_gen_background(uint8_t* dst, int rows, int cols, int cells_x, int cells_y, uint64_t seed);
+/* Disc texture: radial bands (period band_px) over uniform noise, RGBA. */
+void bh_gen_disc_texture(uint8_t* dst, int rows, int cols, int band_px, uint64_t seed);
+/* Uniform u8 noise texture, any channel count. */
+void bh_gen_noise(uint8_t* dst, size_t n, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BH_API_H_ */

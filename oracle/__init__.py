@@ -1,0 +1,144 @@
+"""FP64 CPU oracle (test infrastructure only).
+
+Python binding of oracle/liboracle.so, the plain-C restatement of the
+reference march (include/blackhole/blackhole_solution_test.cc:161-298).  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this package, and only as the checker; the product (blackhole_8_amd) never
+does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from blackhole_8_amd import abi
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    args = ["make", "-C", str(HERE)] + (["-B"] if force else [])
+    subprocess.run(args, check=True, capture_output=True, text=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    P = C.c_void_p
+    D = C.c_double
+    dp = C.POINTER(C.c_double)
+    sigs = {
+        "or_G": (D, [D, D, D]),
+        "or_b_c": (D, [D]),
+        "or_solve_g": (D, [D, D, C.c_int]),
+        "or_focal_length": (D, [C.c_int, D]),
+        "or_pixel_vector": (None, [C.POINTER(abi.Camera), C.c_int, C.c_int, dp]),
+        "or_inv3": (None, [dp, dp, C.POINTER(C.c_int)]),
+        "or_seg_sphere": (C.c_int, [dp, dp, dp, D, dp]),
+        "or_annulus_collide": (C.c_int, [C.POINTER(abi.Annulus), dp, dp, dp]),
+        "or_rect_texel": (None, [dp, C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int)]),
+        "or_equirect_texel": (None, [dp, C.c_int, C.c_int, dp, dp, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
+        "or_trace_pixel": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.Options),
+                                     C.POINTER(abi.Image), C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(abi.Outputs)]),
+        "or_render": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.Options),
+                                C.POINTER(abi.Image), C.c_int, C.c_int,
+                                C.POINTER(abi.Outputs)]),
+        "or_render_rows": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.Options),
+                                     C.POINTER(abi.Image), C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(abi.Outputs)]),
+        "or_inbound_sweep": (D, [D, D, D, C.c_int, C.c_int]),
+        "or_render_flat": (C.c_int, [C.POINTER(abi.FlatScene), C.POINTER(abi.Image), C.c_int,
+                                     C.POINTER(abi.Outputs)]),
+    }
+    for n, (r, a) in sigs.items():
+        f = getattr(lib, n)
+        f.restype = r
+        f.argtypes = a
+    _lib = lib
+    return lib
+
+
+class Ray(C.Structure):
+    _fields_ = [("pv", C.c_double * 3), ("F", C.c_double * 3), ("R", C.c_double * 9),
+                ("Rinv", C.c_double * 9), ("Fp", C.c_double * 3), ("b", C.c_double),
+                ("r0", C.c_double), ("phi0", C.c_double), ("u0", C.c_double),
+                ("u_end", C.c_double), ("du", C.c_double), ("du_h", C.c_double),
+                ("captured", C.c_int), ("degenerate", C.c_int)]
+
+
+def ray_setup(scene, opts, x: int, y: int) -> Ray:
+    lib = load()
+    r = Ray()
+    lib.or_ray_setup.restype = C.c_int
+    lib.or_ray_setup.argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.Options), C.c_int, C.c_int,
+                                 C.POINTER(Ray)]
+    lib.or_ray_setup(C.byref(scene), C.byref(opts), x, y, C.byref(r))
+    return r
+
+
+def _textures(textures: dict):
+    n = (max(textures) + 1) if textures else 0
+    arr = (abi.Image * max(n, 1))()
+    keep = []
+    for slot, img in textures.items():
+        img = np.ascontiguousarray(img, np.uint8)
+        keep.append(img)
+        arr[slot] = abi.image(img)
+    return arr, n, keep
+
+
+def render(scene, opts, textures: dict, threads: int = 1, want=("rgb", "hit"), rows=None):
+    """Render with the oracle.  rows=(y0, y1) renders a band of rows only
+    (the rest of the outputs stay zero)."""
+    from blackhole_8_amd.render import alloc_outputs
+
+    lib = load()
+    w, h = scene.camera.width, scene.camera.height
+    arrs, o = alloc_outputs(w, h, want)
+    tex, n, keep = _textures(textures)
+    y0, y1 = rows if rows else (0, h)
+    rc = lib.or_render_rows(C.byref(scene), C.byref(opts), tex, n, threads, y0, y1, C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"oracle render failed: {rc}")
+    del keep
+    return arrs
+
+
+def render_preset(preset, threads: int = 1, want=("rgb", "hit"), rows=None):
+    return render(preset.scene, preset.options, preset.textures, threads, want, rows)
+
+
+def trace_pixel(scene, opts, textures: dict, x: int, y: int):
+    from blackhole_8_amd.render import alloc_outputs
+
+    lib = load()
+    w, h = scene.camera.width, scene.camera.height
+    arrs, o = alloc_outputs(w, h, ("rgb", "hit", "phi", "hitpt", "steps"))
+    tex, n, keep = _textures(textures)
+    lib.or_trace_pixel(C.byref(scene), C.byref(opts), tex, n, x, y, C.byref(o))
+    return {k: v[y, x] for k, v in arrs.items()}
+
+
+def render_flat(scene, textures: dict, want=("rgb", "hit")):
+    from blackhole_8_amd.render import alloc_outputs
+
+    lib = load()
+    w, h = scene.camera.width, scene.camera.height
+    arrs, o = alloc_outputs(w, h, want)
+    tex, n, keep = _textures(textures)
+    lib.or_render_flat(C.byref(scene), tex, n, C.byref(o))
+    return arrs
