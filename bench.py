@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the photon march on one MI355X: Mrays/s at 1080p hole+disc.
+
+Workload (one "step"): the 1920x1080 hole + thin-disc scene (disc 3..12 r_s,
+camera at 20 r_s, inclination 80 deg, fov 60 deg) rendered along its
+120-frame circular camera path (3 deg per frame), replayed as one HIP graph:
+248,832,000 rays per step, FP32 march with FP64 set-up and the FP64
+photon-sphere band.  Inputs are resident in HBM before the timed region;
+outputs stay in HBM.
+
+Prints one JSON line.  `roofline` covers the dominant kernel (the FP32
+march): algorithmic FLOPs per frame (blackhole_8_amd/flops.py, from the
+frame's own per-pixel step counts) over its HIP-event duration on the
+context stream, against the FP32 vector peak.  `cpu_baseline` is the FP64
+oracle (a C restatement of the reference march) on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FP32_PEAK_TFLOPS = 157.3        # MI355X FP32 vector (packed) peak
+FP32_NONPACKED_TFLOPS = 78.6    # one FMA per lane per clock
+TRAFFIC_FILE = ROOT / "profiles" / "march32_config1_traffic.json"
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=120)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=3)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build renders on a single MI355X "
+              "(rays shard by frame or tile across GPUs with no exchange; see DESIGN.md). "
+              f"Requested --gpus {args.gpus}, WORLD_SIZE {world}.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from blackhole_8_amd import Renderer, abi, flops, scenes
+
+    p = scenes.config1(frames=args.frames)
+    W, H, F = p.width, p.height, len(p.cameras)
+    rays_per_step = W * H * F
+
+    r = Renderer(0)
+    dev = r.device_info()
+    r.set_textures(p.textures)
+    path = r.path(p.scene, p.options, p.cameras)
+
+    # ---- timed region: K graph launches of the 120-frame path -------------
+    for _ in range(args.warmup):
+        path.launch()
+    r.lib.bh_synchronize(r.ctx)
+    t0 = time.perf_counter()
+    gpu_ms = 0.0
+    for _ in range(args.steps):
+        gpu_ms += path.launch()
+    r.lib.bh_synchronize(r.ctx)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = rays_per_step * args.steps / wall / 1e6
+
+    # ---- dominant kernel: per-frame duration and algorithmic work ----------
+    kern_ms, total_ms, work, band = 0.0, 0.0, 0.0, 0
+    for k in range(F):
+        sc = p.with_camera(p.cameras[k]).scene
+        m, t, n = r.time_render(sc, p.options, iters=2)
+        out = r.render(sc, p.options, ("hit", "steps"))
+        kern_ms += m
+        total_ms += t
+        band += n
+        work += flops.frame_flops(out["steps"], out["hit"], True)
+    avg_kernel_ms = kern_ms / F
+    achieved = work / (kern_ms * 1e-3) / 1e12
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        try:
+            traffic = json.loads(TRAFFIC_FILE.read_text()).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    roofline = {
+        "bound": "valu",
+        "kernel": "march32_kernel",
+        "achieved": round(achieved, 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+        "frac_nonpacked": round(achieved / FP32_NONPACKED_TFLOPS, 4),
+        "traffic": traffic,
+        "avg_kernel_ms": round(avg_kernel_ms, 5),
+        "avg_frame_ms": round(total_ms / F, 5),
+        "flop_per_frame": round(work / F),
+        "band_rays_per_frame": round(band / F, 1),
+    }
+
+    # ---- accuracy of frame 0 and the CPU baseline ----------------------------
+    cpu = None
+    accuracy = None
+    if not args.no_cpu_baseline:
+        import oracle
+        from tests.parity import compare
+
+        threads = host_threads()
+        want = ("rgb", "hit", "phi", "hitpt", "steps")
+        frames = [int(round(i * F / args.cpu_frames)) for i in range(args.cpu_frames)]
+        cpu_s = 0.0
+        ref0 = None
+        for k in frames:
+            pk = p.with_camera(p.cameras[k])
+            t = time.perf_counter()
+            ref = oracle.render_preset(pk, threads, want if k == 0 else ("rgb", "hit"))
+            cpu_s += time.perf_counter() - t
+            if k == 0:
+                ref0 = ref
+        cpu = {"value": round(W * H * len(frames) / cpu_s / 1e6, 3), "unit": "Mrays/s",
+               "cores": threads, "kind": "port",
+               "sample": f"frames {frames} of the 120-frame 1080p path, FP64 oracle "
+                         f"(C restatement of the reference march), {threads} threads, "
+                         f"{cpu_model()}"}
+        g0 = r.render(p.with_camera(p.cameras[0]).scene, p.options, want)
+        st = compare(g0, ref0)
+        accuracy = {k: st[k] for k in ("class_mismatch_frac", "rgb_mismatch_frac",
+                                       "phi_max_err", "hit_point_max_err")}
+    path.close()
+    r.close()
+
+    line = {
+        "metric": "Mrays/s at 1080p BH+disc",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "config1: hole + thin disc 3..12 r_s, 1920x1080, camera at "
+                               "20 r_s, inclination 80 deg, 120-frame orbit path as one HIP "
+                               "graph, FP32 march + FP64 set-up and photon-sphere band",
+                   "width": W, "height": H, "frames": F, "rays_per_step": rays_per_step,
+                   "gpu_event_ms_per_step": round(gpu_ms / args.steps, 4),
+                   "device": dev["name"], "cu": dev["cu"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "accuracy_frame0": accuracy,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
