@@ -31,6 +31,7 @@ struct DevBuf {
 struct bh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // band kernels of captured paths
   bool own_stream = false;
   std::string err;
   DeviceTexture tex[BH_MAX_TEXTURES];
@@ -48,6 +49,7 @@ struct bh_path {
   uint8_t* d_hit = nullptr;
   uint32_t* d_qitems = nullptr;
   uint32_t* d_qcount = nullptr;
+  std::vector<hipEvent_t> events;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -200,38 +202,76 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
   d->r_esc = o.r_escape;
   d->band = o.band;
   d->edge = o.edge_width;
+
+  // per-frame constants of the FP32 march
+  for (int i = 0; i < 3; ++i) {
+    d->fvB[i] = d->fv[i] - d->B[i];
+    d->e1f[i] = (float)(d->F[i] / r0);
+    d->nf[i] = (float)d->dn[i];
+    d->Bf[i] = (float)d->B[i];
+  }
+  d->bc2 = d->bc * d->bc;
+  const double blo = d->bc * (1.0 - o.band), bhi = d->bc * (1.0 + o.band);
+  d->lo2 = o.band > 0.0 ? blo * blo : d->bc2;
+  d->hi2 = o.band > 0.0 ? bhi * bhi : d->bc2;
+  d->Uc = 1.0 / (3.0 * d->M);
+  d->cell = (d->Uc - d->l0) * (1.0 / 1048576.0);
+  d->inv_cell = 1048576.0 / (d->Uc - d->l0);
+  d->alpha_f = d->nf[0] * d->e1f[0] + d->nf[1] * d->e1f[1] + d->nf[2] * d->e1f[2];
+  d->r_in2 = d->r_in > 0.0 ? (float)(d->r_in * d->r_in) : -1.0f;
+  d->r_out2 = d->r_out >= 0.0 ? (float)(d->r_out * d->r_out) : -1.0f;
+  if (d->has_disc) {
+    double s1[3], s2[3];
+    for (int i = 0; i < 3; ++i) {
+      s1[i] = d->dv2[i] - d->dv1[i];
+      s2[i] = d->dv4[i] - d->dv1[i];
+      d->tv1[i] = (float)d->dv1[i];
+      d->ts1[i] = (float)s1[i];
+    }
+    const double l1 = std::sqrt(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
+    const double l2 = std::sqrt(s2[0] * s2[0] + s2[1] * s2[1] + s2[2] * s2[2]);
+    d->tka = (float)(d->disc_tex.cols / (l1 * l1));
+    d->tkb = (float)(d->disc_tex.rows / (l1 * l2));
+  }
+  d->fast_ok = d->precision == BH_PREC_FP32 && d->n_spheres == 0 &&
+               d->filter == BH_FILTER_NEAREST && d->nstep <= 4096 &&
+               (!d->has_disc || (d->dv0[0] == d->B[0] && d->dv0[1] == d->B[1] &&
+                                 d->dv0[2] == d->B[2]));
   return BH_OK;
 }
 
 // The FP32 event march assumes the disc is centred on the hole (its plane
 // passes through B, so every crossing is an angle in the orbit plane) and
 // renders no spheres yet; other scenes take the FP64 march.
-bool fp32_eligible(const DevScene& d) {
-  if (d.precision != BH_PREC_FP32) return false;
-  if (d.n_spheres > 0) return false;
-  if (d.filter != BH_FILTER_NEAREST) return false;
-  if (d.has_disc && (d.dv0[0] != d.B[0] || d.dv0[1] != d.B[1] || d.dv0[2] != d.B[2])) return false;
-  return true;
-}
+bool fp32_eligible(const DevScene& d) { return d.fast_ok != 0; }
 
+// FP32 march then the FP64 band kernel, or the FP64 reference march.
 int launch_frame(bh_ctx* ctx, const DevScene& host_scene, DevScene* d_scene, const DevOut& out,
-                 uint32_t* qitems, uint32_t* qcount, uint32_t qcap, hipStream_t st) {
+                 const DevQueue& q, hipStream_t st) {
   const int W = host_scene.W, H = host_scene.H;
   if (fp32_eligible(host_scene)) {
-    HIP_TRY(ctx, bh::launch_zero_u32(qcount, 1, st));
-    DevQueue q{qitems, qcount, qcap};
-    HIP_TRY(ctx, bh::launch_march_fp32(d_scene, out, W, H, q, st, host_scene.escape != 0));
-    HIP_TRY(ctx, bh::launch_band_fp64(d_scene, out, q, st));
+    const bool esc = host_scene.escape != 0;
+    HIP_TRY(ctx, bh::launch_march_fp32(d_scene, out, W, H, q, st, esc));
+    HIP_TRY(ctx, bh::launch_band(d_scene, out, q, st, esc));
   } else {
     HIP_TRY(ctx, bh::launch_march_fp64(d_scene, out, W, H, st));
   }
   return BH_OK;
 }
 
-int prepare_queue(bh_ctx* ctx, size_t npix) {
+// Queue of npix items with a zeroed count pair (the band kernel keeps it
+// zeroed between frames).
+int queue_buffers(bh_ctx* ctx, uint32_t** items, uint32_t** count, size_t npix) {
+  const bool fresh = ctx->qcount.p == nullptr;
   int rc = ensure(ctx, ctx->qitems, npix * sizeof(uint32_t));
   if (rc) return rc;
-  return ensure(ctx, ctx->qcount, 256);
+  if ((rc = ensure(ctx, ctx->qcount, 256))) return rc;
+  if (fresh) {
+    HIP_TRY(ctx, bh::launch_zero_u32((uint32_t*)ctx->qcount.p, 64, ctx->stream));
+  }
+  *items = (uint32_t*)ctx->qitems.p;
+  *count = (uint32_t*)ctx->qcount.p;
+  return BH_OK;
 }
 
 }  // namespace
@@ -255,6 +295,12 @@ int bh_ctx_create(int device, bh_ctx** out) {
     return fail(nullptr, BH_ERR_HIP, "hipStreamCreate failed");
   }
   c->own_stream = true;
+  e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(nullptr, BH_ERR_HIP, "hipStreamCreate failed");
+  }
   (void)hipEventCreate(&c->ev0);
   (void)hipEventCreate(&c->ev1);
   (void)hipEventCreate(&c->ev2);
@@ -281,6 +327,7 @@ void bh_ctx_destroy(bh_ctx* ctx) {
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   delete ctx;
 }
 
@@ -379,13 +426,14 @@ int bh_render_device(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
   if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
+  uint32_t *qi, *qc;
   const size_t npix = (size_t)d.W * d.H;
-  if ((rc = prepare_queue(ctx, npix))) return rc;
+  if ((rc = queue_buffers(ctx, &qi, &qc, npix))) return rc;
   HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
   DevOut o{out_device->rgb, out_device->hit, out_device->phi_end, out_device->hit_point,
            out_device->steps};
-  return launch_frame(ctx, d, (DevScene*)ctx->scene.p, o, (uint32_t*)ctx->qitems.p,
-                      (uint32_t*)ctx->qcount.p, (uint32_t)npix, ctx->stream);
+  DevQueue q{qi, qc, (uint32_t)npix};
+  return launch_frame(ctx, d, (DevScene*)ctx->scene.p, o, q, ctx->stream);
 }
 
 int bh_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, const bh_outputs* out) {
@@ -426,50 +474,42 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
   if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc;
   if ((rc = ensure(ctx, ctx->hit, npix))) return rc;
   if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
-  if ((rc = prepare_queue(ctx, npix))) return rc;
+  uint32_t *qi, *qc;
+  if ((rc = queue_buffers(ctx, &qi, &qc, npix))) return rc;
   hipStream_t st = ctx->stream;
   HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, st));
   DevOut o{(uint8_t*)ctx->rgb.p, (uint8_t*)ctx->hit.p, nullptr, nullptr, nullptr};
   DevScene* ds = (DevScene*)ctx->scene.p;
-  uint32_t* qi = (uint32_t*)ctx->qitems.p;
-  uint32_t* qc = (uint32_t*)ctx->qcount.p;
+  DevQueue q{qi, qc, (uint32_t)npix};
   const bool fast = fp32_eligible(d);
   // warm-up
-  if ((rc = launch_frame(ctx, d, ds, o, qi, qc, (uint32_t)npix, st))) return rc;
+  if ((rc = launch_frame(ctx, d, ds, o, q, st))) return rc;
   HIP_TRY(ctx, hipStreamSynchronize(st));
   double main_ms = 0, total_ms = 0;
+  uint32_t nband = 0;
   for (int it = 0; it < iters; ++it) {
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     if (fast) {
-      HIP_TRY(ctx, bh::launch_zero_u32(qc, 1, st));
-      DevQueue q{qi, qc, (uint32_t)npix};
-      HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
       HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0));
-      HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
-      HIP_TRY(ctx, bh::launch_band_fp64(ds, o, q, st));
     } else {
-      HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
       HIP_TRY(ctx, bh::launch_march_fp64(ds, o, d.W, d.H, st));
-      HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
     }
-    hipEvent_t end;
-    HIP_TRY(ctx, hipEventCreate(&end));
-    HIP_TRY(ctx, hipEventRecord(end, st));
-    HIP_TRY(ctx, hipEventSynchronize(end));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    if (fast) {
+      if (it == iters - 1) HIP_TRY(ctx, hipMemcpyAsync(&nband, qc, 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(ctx, bh::launch_band(ds, o, q, st, d.escape != 0));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
     float a = 0, b = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->ev1, ctx->ev2));
-    HIP_TRY(ctx, hipEventElapsedTime(&b, ctx->ev0, end));
-    (void)hipEventDestroy(end);
+    HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&b, ctx->ev0, ctx->ev2));
     main_ms += a;
     total_ms += b;
   }
   if (avg_ms_main) *avg_ms_main = main_ms / iters;
   if (avg_ms_total) *avg_ms_total = total_ms / iters;
-  if (band_rays) {
-    uint32_t n = 0;
-    if (fast) HIP_TRY(ctx, hipMemcpy(&n, qc, sizeof n, hipMemcpyDeviceToHost));
-    *band_rays = n;
-  }
+  if (band_rays) *band_rays = nband;
   return BH_OK;
 }
 
@@ -506,24 +546,51 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
   if ((e = hipMalloc(&p->d_scenes, sizeof(DevScene) * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc scenes");
   if ((e = hipMalloc(&p->d_rgb, npix * 3 * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc rgb");
   if ((e = hipMalloc(&p->d_hit, npix * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc hit");
-  if ((e = hipMalloc(&p->d_qitems, npix * sizeof(uint32_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
-  if ((e = hipMalloc(&p->d_qcount, 256)) != hipSuccess) return cleanup(e, "hipMalloc count");
+  // one queue slice and count pair per frame, so that the FP64 band kernel of
+  // frame i (side stream) overlaps the FP32 march of frame i+1
+  const size_t qcap = npix / 16 > 1024 ? npix / 16 : 1024;
+  if ((e = hipMalloc(&p->d_qitems, qcap * n_frames * sizeof(uint32_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
+  if ((e = hipMalloc(&p->d_qcount, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMalloc queue count");
+  if ((e = hipMemset(p->d_qcount, 0, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMemset queue count");
   if ((e = hipMemcpy(p->d_scenes, frames.data(), sizeof(DevScene) * n_frames, hipMemcpyHostToDevice)) != hipSuccess)
     return cleanup(e, "hipMemcpy scenes");
-  hipStream_t st = ctx->stream;
+  p->events.resize((size_t)n_frames + 1, nullptr);
+  for (auto& ev : p->events)
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return cleanup(e, "hipEventCreate");
+  hipStream_t st = ctx->stream, side = ctx->side;
   if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return cleanup(e, "capture");
-  for (int i = 0; i < n_frames; ++i) {
+  int rc = BH_OK;
+  bool used_side = false;
+  for (int i = 0; i < n_frames && rc == BH_OK; ++i) {
     DevOut o{p->d_rgb + npix * 3 * i, p->d_hit + npix * i, nullptr, nullptr, nullptr};
-    int rc = launch_frame(ctx, frames[(size_t)i], p->d_scenes + i, o, p->d_qitems, p->d_qcount,
-                          (uint32_t)npix, st);
-    if (rc) {
-      hipGraph_t g;
-      (void)hipStreamEndCapture(st, &g);
-      if (g) (void)hipGraphDestroy(g);
-      std::string m = ctx->err;
-      bh_path_destroy(p);
-      return fail(ctx, rc, m);
+    DevQueue q{p->d_qitems + qcap * i, p->d_qcount + 16 * i, (uint32_t)qcap};
+    const DevScene& f = frames[(size_t)i];
+    if (fp32_eligible(f)) {
+      const bool esc = f.escape != 0;
+      hipError_t le = bh::launch_march_fp32(p->d_scenes + i, o, p->W, p->H, q, st, esc);
+      if (le == hipSuccess) le = hipEventRecord(p->events[(size_t)i], st);
+      if (le == hipSuccess) le = hipStreamWaitEvent(side, p->events[(size_t)i], 0);
+      if (le == hipSuccess) le = bh::launch_band(p->d_scenes + i, o, q, side, esc);
+      if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
+      used_side = true;
+    } else {
+      hipError_t le = bh::launch_march_fp64(p->d_scenes + i, o, p->W, p->H, st);
+      if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
     }
+  }
+  if (rc == BH_OK && used_side) {
+    // join the side stream back into the captured stream
+    hipError_t le = hipEventRecord(p->events[(size_t)n_frames], side);
+    if (le == hipSuccess) le = hipStreamWaitEvent(st, p->events[(size_t)n_frames], 0);
+    if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture join");
+  }
+  if (rc != BH_OK) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    std::string m = ctx->err;
+    bh_path_destroy(p);
+    return fail(ctx, rc, m);
   }
   if ((e = hipStreamEndCapture(st, &p->graph)) != hipSuccess) return cleanup(e, "end capture");
   if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess)
@@ -570,6 +637,8 @@ void bh_path_destroy(bh_path* p) {
   if (p->d_hit) (void)hipFree(p->d_hit);
   if (p->d_qitems) (void)hipFree(p->d_qitems);
   if (p->d_qcount) (void)hipFree(p->d_qcount);
+  for (auto ev : p->events)
+    if (ev) (void)hipEventDestroy(ev);
   delete p;
 }
 

@@ -46,6 +46,21 @@ struct DevScene {
   // options
   int32_t nstep, escape, step_cap, filter, root, precision;
   double r_esc, band, edge;
+
+  // ---- per-frame constants of the FP32 march (host-computed) ----
+  double fvB[3];          // focus vector relative to B
+  double bc2, lo2, hi2;   // b_c^2 and the band's bounds on b^2
+  double Uc;              // 1/(3M)
+  double cell, inv_cell;  // SolveG grid: (1/(3M) - l0) / 2^20 and its inverse
+  float e1f[3];           // F / |F|
+  float nf[3];            // disc normal
+  float alpha_f;          // n . e1
+  float r_in2, r_out2;    // radius window, squared (r_in2 < 0 when r_in <= 0)
+  float Bf[3];
+  float tv1[3], ts1[3];   // disc texture square: p1 and s1 = p2 - p1
+  float tka, tkb;         // cols / |s1|^2 and rows / (|s1| |s2|)
+  int32_t fast_ok;        // the FP32 march applies to this scene
+  int32_t pad1;
 };
 
 // Output planes (device pointers, any may be null).  Pixel (x, y) of the
@@ -58,11 +73,12 @@ struct DevOut {
   int32_t* steps;
 };
 
-// FP64 band queue written by the FP32 march: pixel indices of the rays whose
-// impact distance lies within the photon-sphere band.
+// Photon-sphere band queue: the FP32 march appends the pixel indices of rays
+// with |b/b_c - 1| < band; the FP64 band kernel drains it and its last
+// workgroup clears count and done for the next frame.
 struct DevQueue {
   uint32_t* items;
-  uint32_t* count;
+  uint32_t* count;  // count[0] = items, count[1] = finished workgroups
   uint32_t capacity;
 };
 
@@ -70,8 +86,8 @@ hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, in
                              hipStream_t stream);
 hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, int H,
                              const DevQueue& q, hipStream_t stream, bool escape);
-hipError_t launch_band_fp64(const DevScene* scene, const DevOut& out, const DevQueue& q,
-                            hipStream_t stream);
+hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,
+                       hipStream_t stream, bool escape);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
 
 }  // namespace bh

@@ -2,15 +2,17 @@
 //
 //  march64_kernel  FP64, reference operation order, one lane per pixel
 //                  (include/blackhole/blackhole_solution_test.cc:161-298).
-//  march32_kernel  FP32 event march: FP64 set-up, a 3-FMA + 1-rsqrt step in
-//                  the orbit plane, exact object tests only on the steps
-//                  where an object can be reached, FP64 band queue.
-//  band64_kernel   drains the band queue with the FP64 march.
+//  march_kernel    FP32 event march: FP64 set-up, a 3-FMA + 1-rsqrt step in
+//                  the orbit plane, exact object tests only on the steps where
+//                  an object can be reached; rays in the photon-sphere band
+//                  are queued for band_kernel.
+//  band_kernel     the same event march in FP64 for the queued band rays.
 #include "bh_fp64.h"
 
 namespace bh {
 
 constexpr double kPiD = 3.141592653589793238462643383279;
+constexpr double kInvPiD = 0.318309886183790671537767526745;
 
 // ---------------------------------------------------------------------------
 // FP64 reference march
@@ -26,135 +28,215 @@ __global__ __launch_bounds__(256) void march64_kernel(const DevScene* __restrict
   store_pixel64(out, (size_t)y * s.W + x, px);
 }
 
-__global__ __launch_bounds__(64) void band64_kernel(const DevScene* __restrict__ sp, DevOut out,
-                                                    const uint32_t* __restrict__ items,
-                                                    const uint32_t* __restrict__ count,
-                                                    uint32_t capacity) {
-  const DevScene& s = *sp;
-  const uint32_t n = min(*count, capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t idx = items[i];
-    const int x = (int)(idx % (uint32_t)s.W), y = (int)(idx / (uint32_t)s.W);
-    Pixel64 px;
-    trace64(s, x, y, px);
-    store_pixel64(out, idx, px);
-  }
+// ---------------------------------------------------------------------------
+// small math
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T rsq(T x);
+template <> __device__ __forceinline__ float rsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+// FP64 1/sqrt: hardware estimate refined by two Newton steps y (3 - x y^2)/2
+template <> __device__ __forceinline__ double rsq<double>(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * __builtin_fma(-0.5 * x, y * y, 1.5);
+  y = y * __builtin_fma(-0.5 * x, y * y, 1.5);
+  return y;
+}
+template <typename T> __device__ __forceinline__ T rcp(T x);
+template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double rcp<double>(double x) { return 1.0 / x; }
+template <typename T> __device__ __forceinline__ T sqrt_t(T x);
+template <> __device__ __forceinline__ float sqrt_t<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <> __device__ __forceinline__ double sqrt_t<double>(double x) { return sqrt(x); }
+template <typename T> __device__ __forceinline__ T fma_t(T a, T b, T c);
+template <> __device__ __forceinline__ float fma_t<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double fma_t<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// sin and cos of r, |r| <= pi/4 + a little: Taylor series to r^9 / r^10
+// (truncation below 2e-9 on that interval), evaluated by Horner.
+__device__ __forceinline__ void sincos_core(float r, float* s, float* c) {
+  const float r2 = r * r;
+  float ps = __builtin_fmaf(r2, 2.7557319e-6f, -1.9841270e-4f);  // 1/9!, -1/7!
+  ps = __builtin_fmaf(ps, r2, 8.3333333e-3f);                      // 1/5!
+  ps = __builtin_fmaf(ps, r2, -1.6666667e-1f);                     // -1/3!
+  *s = __builtin_fmaf(ps * r2, r, r);
+  float pc = __builtin_fmaf(r2, -2.7557319e-7f, 2.4801587e-5f);   // -1/10!, 1/8!
+  pc = __builtin_fmaf(pc, r2, -1.3888889e-3f);                     // -1/6!
+  pc = __builtin_fmaf(pc, r2, 4.1666667e-2f);                      // 1/4!
+  pc = __builtin_fmaf(pc, r2, -0.5f);
+  *c = __builtin_fmaf(pc, r2, 1.0f);
 }
 
-__global__ void zero_u32_kernel(uint32_t* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+__device__ __forceinline__ void quadrant(int q, float sr, float cr, float* s, float* c) {
+  const float ss = (q & 1) ? cr : sr;
+  const float cc = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
 }
+
+// FP32 sin/cos for |x| up to a few times 2 pi: quadrant reduction with a
+// two-part pi/2 (Cody and Waite), then the core series.
+__device__ __forceinline__ void sincos_small(float x, float* s, float* c) {
+  const float k = __builtin_rintf(x * 0.63661977f);
+  float r = __builtin_fmaf(-k, 1.5707964f, x);
+  r = __builtin_fmaf(-k, -4.3711388e-8f, r);
+  float sr, cr;
+  sincos_core(r, &sr, &cr);
+  quadrant((int)k & 3, sr, cr, s, c);
+}
+
+// sin/cos of an FP64 angle of any size: reduce in FP64, evaluate in FP32.
+__device__ __forceinline__ void sincos_angle(double a, float* s, float* c) {
+  const double k = rint(a * (2.0 * kInvPiD));
+  const float r = (float)__builtin_fma(-k, kPiD / 2.0, a);
+  float sr, cr;
+  sincos_core(r, &sr, &cr);
+  quadrant((int)(int64_t)k & 3, sr, cr, s, c);
+}
+
+// FP64 sin/cos for |x| up to a few times 2 pi: three-part pi/2 reduction
+// (Cody and Waite), Taylor series to r^21 / r^22 on |r| <= pi/4.
+__device__ inline void sincos_small64(double x, double* s, double* c) {
+  const double k = rint(x * (2.0 * kInvPiD));
+  double r = __builtin_fma(-k, 1.5707963267341256, x);
+  r = __builtin_fma(-k, 6.077100506303966e-11, r);
+  r = __builtin_fma(-k, 2.0222662487959506e-21, r);
+  const double r2 = r * r;
+  double ps = 1.9572941063391263e-20;   // 1/21!
+  ps = __builtin_fma(ps, r2, -8.22063524662433e-18);
+  ps = __builtin_fma(ps, r2, 2.8114572543455206e-15);
+  ps = __builtin_fma(ps, r2, -7.647163731819816e-13);
+  ps = __builtin_fma(ps, r2, 1.6059043836821613e-10);
+  ps = __builtin_fma(ps, r2, -2.505210838544172e-08);
+  ps = __builtin_fma(ps, r2, 2.7557319223985893e-06);
+  ps = __builtin_fma(ps, r2, -0.0001984126984126984);
+  ps = __builtin_fma(ps, r2, 0.008333333333333333);
+  ps = __builtin_fma(ps, r2, -0.16666666666666666);
+  const double sr = __builtin_fma(ps * r2, r, r);
+  double pc = -8.896791392450574e-22;  // -1/22!
+  pc = __builtin_fma(pc, r2, 4.110317623312165e-19);
+  pc = __builtin_fma(pc, r2, -1.5619206968586225e-16);
+  pc = __builtin_fma(pc, r2, 4.779477332387385e-14);
+  pc = __builtin_fma(pc, r2, -1.1470745597729725e-11);
+  pc = __builtin_fma(pc, r2, 2.08767569878681e-09);
+  pc = __builtin_fma(pc, r2, -2.755731922398589e-07);
+  pc = __builtin_fma(pc, r2, 2.48015873015873e-05);
+  pc = __builtin_fma(pc, r2, -0.001388888888888889);
+  pc = __builtin_fma(pc, r2, 0.041666666666666664);
+  pc = __builtin_fma(pc, r2, -0.5);
+  const double cr = __builtin_fma(pc, r2, 1.0);
+  const int q = (int)(int64_t)k & 3;
+  const double ss = (q & 1) ? cr : sr;
+  const double cs = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cs : cs;
+}
+
+template <typename T> __device__ __forceinline__ void sincos_t(T x, T* s, T* c);
+template <> __device__ __forceinline__ void sincos_t<float>(float x, float* s, float* c) { sincos_small(x, s, c); }
+template <> __device__ __forceinline__ void sincos_t<double>(double x, double* s, double* c) { sincos_small64(x, s, c); }
 
 // ---------------------------------------------------------------------------
 // FP32 event march
 // ---------------------------------------------------------------------------
 //
 // Orbit-plane reduction.  The reference places step k at
-//   P_k = B + r_k (cos phi_k * yv + sin phi_k * xv),  r_k = 1/u_k,
+//   P_k = B + r_k (cos phi_k yv + sin phi_k xv),  r_k = 1/u_k,
 // so every chord lies in the plane through B spanned by (yv, xv).  With
 // e1 = cos phi0 yv + sin phi0 xv = sign(Y0) F/|F| and e2 = e1 rotated by +90
 // degrees in that plane, P_k = B + r_k (cos a_k e1 + sin a_k e2) where
-// a_k = phi_k - phi0 is the angle swept since the camera.
+// a_k = phi_k - phi0 is the angle swept since phi0.
 //
 // G on the march.  With l the turning point and s = l - u,
 //   G(u) = G(l) + s [2 l d + s (1 - 2 d) - 2 M s^2],  d = 1 - 3 M l,
 // a cubic in s whose terms do not cancel away from the photon sphere; the
 // coefficients come from FP64.  The reference's u_k = u0 + kappa_k du gives
-// s_k = (nstep - kappa_k) du exactly, so each step is s = sigma * du, three
-// FMAs, one rsqrt, and the trapezoid update of a (accumulated in FP64).
+// s_k = sigma_k du with sigma_k = nstep - kappa_k exact, so a step is one
+// multiply, three FMAs, one rsqrt and the trapezoid update of a (FP64 sum).
 //
 // Events.  A chord can only reach the disc plane if sign(n . (P - B)) =
-// sign(cos(a - psi)) changes, i.e. a crosses chi_j = psi + pi/2 + j pi; the
-// step is tested exactly when a passes the next chi_j (minus a guard for the
-// FP32 angle).  A chord between radii r_a, r_b >= r_min subtending Da stays at
-// distance >= r_min cos(Da/2) from B, so it can only reach the horizon when
-// Da > 2 acos(R_h / r_min); that step is tested exactly too.  Every other
-// step provably hits nothing.
-struct Lane32 {
-  float c0, c1, c2, c3;  // G(s) cubic
-  float duf, h;          // du and du/2 in FP32
-  float phi_safe;        // horizon guard angle
-  float guard;           // disc crossing-angle guard
-  double du64;           // du in FP64 (event path positions)
-  double chi0;           // first disc crossing angle (chi_0 = psi + pi/2)
-  double a_ev;           // next disc event angle (crossing - guard)
+// sign(alpha cos a + beta sin a) changes, i.e. a passes a crossing angle
+// chi_0 + j pi; the step is tested exactly when a passes the next one (minus
+// a guard for the FP32 angle).  A chord between radii >= r_min subtending Da
+// stays at distance >= r_min cos(Da/2) from B, so it can only reach the
+// horizon when Da > 2 acos(R_h / r_min) >= pi (1 - R_h / r_min); that step is
+// tested exactly too.  Every other step provably hits nothing.
+template <typename T>
+struct Lane {
+  T c0, c1, c2, c3;   // G(s) cubic
+  T du, h;            // du and du/2
+  float phi_safe;     // horizon guard angle
+  float guard;        // disc crossing-angle guard
+  double du64;        // du in FP64 (event positions)
+  double chi0;        // a disc crossing angle (crossings at chi0 + j pi)
+  double a_ev;        // next disc event angle (crossing - guard)
 };
 
-struct Hit32 {
-  int cls;      // BH_HIT_*
-  double ang;   // swept angle of the hit point
-  float rho;    // distance of the hit point from B
+struct HitRec {
+  int cls;       // BH_HIT_*
+  double ang;    // rotation of the frame the hit point is expressed in
+  float hx, hy;  // hit point relative to B in that frame
 };
 
-// Swept angle of the crossing chi_j nearest below or at a, and the count of
-// crossings in (a_lo, a_hi].
-__device__ __forceinline__ double crossing_floor(double a, double chi0) {
-  return floor((a - chi0) * (1.0 / kPiD));
-}
-
-// Exact tests of the chord between step (sa_, aa) and step (sb_, ab), where
-// sigma = nstep - kappa is the distance to the turning point in units of du
-// (u = u0 + kappa du) and a the swept angle.  Returns true on a hit.
-__device__ inline bool event32(const DevScene& s, const Lane32& L, float sa_, double aa,
-                               float sb_, double ab, float da, bool test_h, bool test_d,
-                               Hit32& hit) {
-  const double ua = s.u0 + ((double)s.nstep - (double)sa_) * L.du64;
-  const double ub = s.u0 + ((double)s.nstep - (double)sb_) * L.du64;
-  const float ra = 1.0f / (float)ua;
-  const float rb = 1.0f / (float)ub;
+// Exact tests of the chord between (sigma_a, a_a) and (sigma_b, a_b), where
+// u = u0 + (nstep - sigma) du and a is the swept angle.
+template <typename T>
+__device__ inline bool event_chord(const DevScene& s, const Lane<T>& L, T sga, double aa, T sgb,
+                                   double ab, bool th, bool td, HitRec& hit) {
+  const double ua = __builtin_fma((double)s.nstep - (double)sga, L.du64, s.u0);
+  const double ub = __builtin_fma((double)s.nstep - (double)sgb, L.du64, s.u0);
+  const T ra = rcp<T>((T)ua);
+  const T rb = rcp<T>((T)ub);
   bool found = false;
-  float best = 0.f;
-  const float R = (float)s.Rh;
-  if (test_h) {
-    // frame rotated by aa: P_a = (ra, 0), P_b = rb (cos da, sin da)
-    float sd, cd;
-    sincosf(da, &sd, &cd);
-    const float qx = rb * cd - ra, qy = rb * sd;
-    const float qd = qx * ra;
-    const float qq = qx * qx + qy * qy;
+  T best = 0;
+  if (th) {
+    // frame rotated by aa: P_a = (ra, 0), P_b = rb (cos da, sin da);
+    // StaticBlackhole::Collide (blackhole_solution.h:65-88) in the plane
+    const T R = (T)s.Rh;
+    T sd, cd;
+    sincos_t<T>((T)(ab - aa), &sd, &cd);
+    const T qx = rb * cd - ra, qy = rb * sd;
+    const T qd = qx * ra;
+    const T qq = qx * qx + qy * qy;
     if (!(ra < R && rb < R)) {
-      const float cc = qd * qd - qq * (ra * ra - R * R);
-      if (cc > 0.f) {
-        const float t = (-qd - sqrtf(cc)) / qq;
-        if (t > 0.f && t < 1.f) {
-          const float hx = ra + t * qx, hy = t * qy;
+      const T cc = qd * qd - qq * (ra * ra - R * R);
+      if (cc > (T)0) {
+        const T t = (-qd - sqrt_t<T>(cc)) * rcp<T>(qq);
+        if (t > (T)0 && t < (T)1) {
           found = true;
           best = t * t * qq;
           hit.cls = BH_HIT_HORIZON;
-          hit.ang = aa + (double)atan2f(hy, hx);
-          hit.rho = sqrtf(hx * hx + hy * hy);
+          hit.ang = aa;
+          hit.hx = (float)(ra + t * qx);
+          hit.hy = (float)(t * qy);
         }
       }
     }
   }
-  if (test_d) {
-    const double fa = crossing_floor(aa, L.chi0);
-    const double fb = crossing_floor(ab, L.chi0);
-    const double n = fb - fa;
-    if (fmod(n, 2.0) != 0.0) {
-      // crossing chi_k inside (aa, ab]; reference interpolation on the chord
-      // with weights |t_b|, |t_a|, t = n.(P - B) ~ r sin(a - chi_k)
-      const double chik = L.chi0 + fb * kPiD;
-      const float xa = (float)(aa - chik), xb = (float)(ab - chik);
-      float sa, ca, sb, cb;
-      sincosf(xa, &sa, &ca);
-      sincosf(xb, &sb, &cb);
-      const float wa = fabsf(ra * sa), wb = fabsf(rb * sb);  // |t_a|, |t_b|
-      const float inv = 1.0f / (wa + wb);
-      // crossing point in the frame whose x axis points along chi_k
-      const float px = (wb * ra * ca + wa * rb * cb) * inv;
-      const float py = (wb * ra * sa + wa * rb * sb) * inv;
-      const float rho = sqrtf(px * px + py * py);
-      if (!(rho > (float)s.r_out) && !(rho < (float)s.r_in)) {
-        const float dx = ra * ca - px, dy = ra * sa - py;
-        const float d2 = dx * dx + dy * dy;
+  if (td) {
+    // Annulus::Collide (vector_object.h:328-347): n.(P - B) ~ r sin(a - chi)
+    // changes sign an odd number of times; crossing by linear interpolation
+    // on the chord with weights |t_b|, |t_a|
+    const double fa = floor((aa - L.chi0) * kInvPiD);
+    const double fb = floor((ab - L.chi0) * kInvPiD);
+    const double n = fabs(fb - fa);
+    if (fmod(n, 2.0) == 1.0) {
+      const double chik = L.chi0 + fmax(fa, fb) * kPiD;
+      T sa, ca, sb, cb;
+      sincos_t<T>((T)(aa - chik), &sa, &ca);
+      sincos_t<T>((T)(ab - chik), &sb, &cb);
+      const T wa = fabs(ra * sa), wb = fabs(rb * sb);
+      const T inv = rcp<T>(wa + wb);
+      const T px = (wb * ra * ca + wa * rb * cb) * inv;
+      const T py = (wb * ra * sa + wa * rb * sb) * inv;
+      const T rho2 = px * px + py * py;
+      if (!(rho2 > (T)s.r_out2) && !(rho2 < (T)s.r_in2)) {
+        const T dx = ra * ca - px, dy = ra * sa - py;
+        const T d2 = dx * dx + dy * dy;
         if (!found || d2 < best) {
           found = true;
-          best = d2;
           hit.cls = BH_HIT_DISC;
-          hit.ang = chik + (double)atan2f(py, px);
-          hit.rho = rho;
+          hit.ang = chik;
+          hit.hx = (float)px;
+          hit.hy = (float)py;
         }
       }
     }
@@ -162,26 +244,102 @@ __device__ inline bool event32(const DevScene& s, const Lane32& L, float sa_, do
   return found;
 }
 
-// Disc texel of a world point (Rectangle::color, vector_object.h:163-179), with
-// r cos(theta) = s1.v/|s1| and r sin(theta) = |s1 x v|/|s1| in place of the
-// acos round trip.
+__device__ __forceinline__ double next_event(double a, double chi0, float guard) {
+  return chi0 + (floor((a - (double)guard - chi0) * kInvPiD) + 1.0) * kPiD - (double)guard;
+}
+
+struct MarchOut {
+  HitRec hit;
+  int steps;
+  double a;      // swept angle after the last step
+  double du64;   // du of the ray
+  float d;       // last increment (for the escape direction)
+};
+
+// The march of one ray in precision T.  first_aa is the angle of the camera
+// in the (e1, e2) frame: 0 when Y0 > 0, pi when Y0 < 0 (quirk Q2: the
+// reference's phi0 = atan(b/Y0) then points away from the camera, and only
+// the first chord starts at the camera itself).
+template <typename T, bool ESCAPE>
+__device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, double first_aa,
+                             MarchOut& o) {
+  const int nstep = s.nstep;
+  T w = (T)0;
+  double a = 0.0;
+  bool alive = true;
+  o.hit.cls = BH_HIT_NONE;
+  o.hit.ang = 0.0;
+  o.hit.hx = o.hit.hy = 0.f;
+  o.du64 = L.du64;
+  T d = (T)0;
+  int n = 1;
+
+  auto step = [&](T sg, T sg_prev, bool force) {
+    const T sv = sg * L.du;
+    const T G = fma_t<T>(fma_t<T>(fma_t<T>(L.c3, sv, L.c2), sv, L.c1), sv, L.c0);
+    const T wn = rsq<T>(G);
+    d = (w + wn) * L.h;
+    w = wn;
+    a += (double)d;
+    const bool eh = (float)d > L.phi_safe;
+    const bool ed = a >= L.a_ev;
+    if (eh || ed || force) {
+      const double aa = force ? first_aa : a - (double)d;
+      if (event_chord<T>(s, L, sg_prev, aa, sg, a, eh || force, ed || force, o.hit)) {
+        alive = false;
+        o.steps = n;
+      } else if (ed || force) {
+        L.a_ev = next_event(a, L.chi0, L.guard);
+      }
+    }
+  };
+
+  // inbound: u = u0 + i du (blackhole_solution_test.cc:217-239)
+  step((T)(nstep - 1), (T)nstep, first_aa != 0.0);
+  for (n = 2; n < nstep && alive; ++n) step((T)(nstep - n), (T)(nstep - n + 1), false);
+  // final approach: u += 0.9 du, weighted du/2 (:240-262)
+  if (alive) {
+    n = nstep;
+    step((T)0.1, (T)1, false);
+  }
+  if (alive && captured) {
+    // chord to the centre: the horizon is always entered (r > 3M > 2M) and
+    // the disc test sees n.(B - B) = 0 (:264-272)
+    o.hit.cls = BH_HIT_HORIZON;
+    o.hit.ang = a;
+    o.hit.hx = (float)s.Rh;
+    o.hit.hy = 0.f;
+    o.steps = nstep;
+    alive = false;
+  }
+  // outbound: u -= du (:274-296)
+  for (n = nstep + 1; n < 2 * nstep && alive; ++n)
+    step((T)0.1 + (T)(n - nstep), (T)0.1 + (T)(n - nstep - 1), false);
+  if (ESCAPE) {
+    // extension: keep stepping outward until the next point would lie at
+    // r >= r_escape (u <= 1/r_escape), or the step cap is reached
+    for (; alive && n <= s.step_cap; ++n) {
+      const T sg = (T)0.1 + (T)(n - nstep);
+      const double un = __builtin_fma((double)nstep - (double)sg, L.du64, s.u0);
+      if (un * s.r_esc <= 1.0) break;
+      step(sg, sg - (T)1, false);
+    }
+  }
+  if (alive) o.steps = n - 1;
+  o.a = a;
+  o.d = (float)d;
+}
+
+// Disc texel of a world point q (Rectangle::color, vector_object.h:163-179)
+// with r cos(theta) = s1.v/|s1| and r sin(theta) = |s1 x v|/|s1|.
 __device__ inline uint32_t disc_color32(const DevScene& s, float qx, float qy, float qz) {
   const DevTex& t = s.disc_tex;
-  if (!t.data) return 0;
-  const float v1x = (float)s.dv1[0], v1y = (float)s.dv1[1], v1z = (float)s.dv1[2];
-  const float s1x = (float)(s.dv2[0] - s.dv1[0]), s1y = (float)(s.dv2[1] - s.dv1[1]),
-              s1z = (float)(s.dv2[2] - s.dv1[2]);
-  const float s2x = (float)(s.dv4[0] - s.dv1[0]), s2y = (float)(s.dv4[1] - s.dv1[1]),
-              s2z = (float)(s.dv4[2] - s.dv1[2]);
-  const float vx = qx - v1x, vy = qy - v1y, vz = qz - v1z;
-  const float l1 = sqrtf(s1x * s1x + s1y * s1y + s1z * s1z);
-  const float l2 = sqrtf(s2x * s2x + s2y * s2y + s2z * s2z);
-  const float proj = (s1x * vx + s1y * vy + s1z * vz) / l1;
-  const float cx = s1y * vz - s1z * vy, cy = s1z * vx - s1x * vz, cz = s1x * vy - s1y * vx;
-  const float perp = sqrtf(cx * cx + cy * cy + cz * cz) / l1;
-  const float a = (proj / l1) * (float)t.cols;
-  const float b = (perp / l2) * (float)t.rows;
-  if (s.filter == BH_FILTER_BILINEAR) return bilinear64(t, (double)a, (double)b);
+  const float vx = qx - s.tv1[0], vy = qy - s.tv1[1], vz = qz - s.tv1[2];
+  const float a = (s.ts1[0] * vx + s.ts1[1] * vy + s.ts1[2] * vz) * s.tka;
+  const float cx = s.ts1[1] * vz - s.ts1[2] * vy;
+  const float cy = s.ts1[2] * vx - s.ts1[0] * vz;
+  const float cz = s.ts1[0] * vy - s.ts1[1] * vx;
+  const float b = __builtin_amdgcn_sqrtf(cx * cx + cy * cy + cz * cz) * s.tkb;
   return texel(t, tex_index((double)a, t.cols), tex_index((double)b, t.rows));
 }
 
@@ -195,179 +353,228 @@ __device__ inline uint32_t equirect32(const DevTex& t, float dx, float dy, float
   return texel(t, tex_index((double)floorf(c), t.cols), tex_index((double)floorf(r), t.rows));
 }
 
-template <bool ESCAPE>
-__global__ __launch_bounds__(256) void march32_kernel(const DevScene* __restrict__ sp,
-                                                      DevOut out, DevQueue q) {
-  const DevScene& s = *sp;
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (x >= s.W || y >= s.H) return;
-  const size_t idx = (size_t)y * s.W + x;
-
-  // ---- FP64 set-up ------------------------------------------------------
-  const d3 B = ld3(s.B);
-  const d3 F = ld3(s.F);
-  const d3 pv = pixel_point(s, x, y) - B;
-  const d3 dv = F - pv;                  // along yv
-  const d3 cv = cross(pv, F);            // |cv| = b |dv|
-  const double cc = dot(cv, cv);
-  const double dd = dot(dv, dv);
-  const double b = sqrt(cc / dd);
+// Lane set-up in precision T from the FP64 quantities of the ray.
+template <typename T>
+__device__ inline void setup_lane(const DevScene& s, double l, double ib2, float e2x, float e2y,
+                                  float e2z, float sgn, Lane<T>& L) {
   const double M = s.M;
-  const bool captured = b < s.bc;
-  // photon-sphere band and degenerate rays go to the FP64 queue
-  const double dF = dot(dv, F);
-  // rays in the photon-sphere band, degenerate rays, and rays whose start
-  // angle atan(b/Y0) is not the camera's own direction (Y0 <= 0, quirk Q2)
-  // go to the FP64 queue
-  if (fabs(b / s.bc - 1.0) < s.band || cc == 0.0 || dd == 0.0 || !(dF > 0.0)) {
-    const uint32_t slot = atomicAdd(q.count, 1u);
-    if (slot < q.capacity) q.items[slot] = (uint32_t)idx;
-    return;
-  }
-  const double l = captured ? 1.0 / (3 * M) : solve_g64(b, M, s.l0, s.root);
   const double du = (l - s.u0) * (1.0 / s.nstep);
   const double dl = 1.0 - 3.0 * M * l;
-  Lane32 L;
-  L.c0 = (float)G64(l, b, M);
-  L.c1 = (float)(2.0 * l * dl);
-  L.c2 = (float)(1.0 - 2.0 * dl);
-  L.c3 = (float)(-2.0 * M);
-  L.duf = (float)du;
-  L.h = (float)(du / 2.);
+  L.c0 = (T)__builtin_fma(l * l, __builtin_fma(2.0 * M, l, -1.0), ib2);
+  L.c1 = (T)(2.0 * l * dl);
+  L.c2 = (T)(1.0 - 2.0 * dl);
+  L.c3 = (T)(-2.0 * M);
+  L.du = (T)du;
+  L.h = (T)(du / 2.);
   L.du64 = du;
-  const double r0 = 1.0 / s.u0;
-  // Y0 > 0 here, so e1 = F / r0 and e2 = ((dv.F) F - r0^2 dv) / (r0 |cv|)
-  const double ic = 1.0 / (r0 * sqrt(cc));
-  const float e1x = (float)(F.x / r0), e1y = (float)(F.y / r0), e1z = (float)(F.z / r0);
-  const float e2x = (float)((dF * F.x - r0 * r0 * dv.x) * ic);
-  const float e2y = (float)((dF * F.y - r0 * r0 * dv.y) * ic);
-  const float e2z = (float)((dF * F.z - r0 * r0 * dv.z) * ic);
-  // horizon guard from the smallest radius on the march, u = u0 + (n - 0.1) du
-  {
-    const double umax = s.u0 + ((double)s.nstep - 0.1) * du;
-    const float cr = (float)(s.Rh * umax);
-    L.phi_safe = cr < 1.f ? 2.0f * acosf(fminf(1.f, cr * 1.0001f)) : 0.f;
-  }
-  // disc crossing angles: n.(P-B) = r (al cos a + be sin a)
+  // smallest radius on the march: u = u0 + (nstep - 0.1) du
+  const float x = (float)(s.Rh * __builtin_fma((double)s.nstep - 0.1, du, s.u0));
+  L.phi_safe = x < 1.f ? 3.14159265f * (1.f - x) * 0.9999f : 0.f;
   L.chi0 = 0.0;
   L.a_ev = 1e300;
   L.guard = 0.f;
   if (s.has_disc) {
-    const float nx = (float)s.dn[0], ny = (float)s.dn[1], nz = (float)s.dn[2];
-    const float al = nx * e1x + ny * e1y + nz * e1z;
-    const float be = nx * e2x + ny * e2y + nz * e2z;
-    const float amp = sqrtf(al * al + be * be);
+    const float al = sgn * s.alpha_f;
+    const float be = s.nf[0] * e2x + s.nf[1] * e2y + s.nf[2] * e2z;
+    const float amp = __builtin_amdgcn_sqrtf(al * al + be * be);
     if (amp > 0.f) {
-      const double psi = (double)atan2f(be, al);
-      L.chi0 = psi + kPiD / 2.0;
-      L.guard = 4e-6f + 2e-7f / amp;
-      const double j = floor((0.0 - (double)L.guard - L.chi0) * (1.0 / kPiD)) + 1.0;
-      L.a_ev = L.chi0 + j * kPiD - (double)L.guard;
+      // al cos a + be sin a vanishes at a = atan2(be, al) + pi/2 (mod pi)
+      L.chi0 = (double)atan2f(be, al) + kPiD / 2.0;
+      L.guard = 4e-6f + 4e-7f / amp;
+      L.a_ev = next_event(0.0, L.chi0, L.guard);
     }
   }
+}
 
-  // ---- FP32 march ---------------------------------------------------------
-  // Step argument sigma: s = l - u = sigma du.  Inbound sigma = nstep - i,
-  // final approach 0.1, outbound 0.1 + j.
-  const int nstep = s.nstep;
-  float wprev = 0.f;
-  double a = 0.0;          // swept angle
-  float da = 0.f;          // last increment
-  float sprev = (float)nstep;  // the camera sits at sigma = nstep
-  int steps = 0;
-  bool alive = true;
-  Hit32 hit;
-  hit.cls = BH_HIT_NONE;
-  hit.ang = 0.0;
-  hit.rho = 0.f;
+// Per-ray set-up shared by the FP32 march and the FP64 band kernel.
+struct RaySetup {
+  double cc, dd, dF, ib2, l, first_aa;
+  float e2x, e2y, e2z, sgn;
+  bool degenerate, captured, band;
+};
 
-  auto step = [&](float sg) {
-    const float sv = sg * L.duf;
-    const float G = fmaf(fmaf(fmaf(L.c3, sv, L.c2), sv, L.c1), sv, L.c0);
-    const float w = __builtin_amdgcn_rsqf(G);
-    const float d = (wprev + w) * L.h;
-    const double an = a + (double)d;
-    ++steps;
-    const bool eh = d > L.phi_safe;
-    const bool ed = an >= L.a_ev;
-    if (eh || ed) {
-      if (event32(s, L, sprev, a, sg, an, d, eh, ed, hit)) alive = false;
-      if (ed && alive) {
-        const double j = floor((an - (double)L.guard - L.chi0) * (1.0 / kPiD)) + 1.0;
-        L.a_ev = L.chi0 + j * kPiD - (double)L.guard;
-      }
+__device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bisect,
+                                 RaySetup& r) {
+  const double sx = s.W / 2.0 - x, sy = s.H / 2.0 - y;
+  const double pvx = __builtin_fma(-s.vz[0], sy, __builtin_fma(-s.vy[0], sx, s.fvB[0]));
+  const double pvy = __builtin_fma(-s.vz[1], sy, __builtin_fma(-s.vy[1], sx, s.fvB[1]));
+  const double pvz = __builtin_fma(-s.vz[2], sy, __builtin_fma(-s.vy[2], sx, s.fvB[2]));
+  const double Fx = s.F[0], Fy = s.F[1], Fz = s.F[2];
+  const double dvx = Fx - pvx, dvy = Fy - pvy, dvz = Fz - pvz;  // along yv
+  const double cvx = pvy * Fz - pvz * Fy;                        // pv x F, |cv| = b |dv|
+  const double cvy = pvz * Fx - pvx * Fz;
+  const double cvz = pvx * Fy - pvy * Fx;
+  const double cc = __builtin_fma(cvz, cvz, __builtin_fma(cvy, cvy, cvx * cvx));
+  const double dd = __builtin_fma(dvz, dvz, __builtin_fma(dvy, dvy, dvx * dvx));
+  const double dF = __builtin_fma(dvz, Fz, __builtin_fma(dvy, Fy, dvx * Fx));
+  r.cc = cc;
+  r.dd = dd;
+  r.dF = dF;
+  r.degenerate = !(cc > 0.0) || !(dd > 0.0);
+  r.sgn = dF > 0.0 ? 1.f : -1.f;
+  r.first_aa = dF > 0.0 ? 0.0 : kPiD;
+  r.captured = cc < dd * s.bc2;  // b < b_c
+  r.band = cc > dd * s.lo2 && cc < dd * s.hi2;
+  r.e2x = r.e2y = r.e2z = 0.f;
+  r.ib2 = 0.0;
+  r.l = s.Uc;
+  if (r.degenerate) return;
+  r.ib2 = dd / cc;  // 1 / b^2
+  // e2 = sign(Y0) ((dv.F) F - r0^2 dv) / (r0 |cv|)
+  const double r02 = 1.0 / (s.u0 * s.u0);
+  const double k = (double)r.sgn * s.u0 * rsqrt(cc);
+  r.e2x = (float)(__builtin_fma(dF, Fx, -r02 * dvx) * k);
+  r.e2y = (float)(__builtin_fma(dF, Fy, -r02 * dvy) * k);
+  r.e2z = (float)(__builtin_fma(dF, Fz, -r02 * dvz) * k);
+  if (r.captured) return;
+  // Turning point: StaticBlackhole::SolveG's lower bisection end.  The root
+  // u* = (1/(3M)) (1/2 + sin(asin(2q - 1)/3)), q = (b_c/b)^2, is refined by
+  // two Newton steps in FP64; the bisection cell containing it is then exact
+  // unless u* sits within 1e-5 cells of a cell edge, where (as in the
+  // photon-sphere band) the reference's bisection is rerun.
+  bool bisect = force_bisect || r.band || s.root == BH_ROOT_EXACT;
+  if (!bisect) {
+    const float q = fminf(1.f, (float)(s.bc2 * r.ib2));
+    float sn, cs;
+    sincos_core(asinf(2.f * q - 1.f) * (1.f / 3.f), &sn, &cs);
+    double u = s.Uc * (0.5 + (double)sn);
+    const double M2 = 2.0 * s.M, M6 = 6.0 * s.M;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const double g = __builtin_fma(u * u, __builtin_fma(M2, u, -1.0), r.ib2);
+      const double gp = u * __builtin_fma(M6, u, -2.0);
+      u = u - g / gp;
     }
-    a = an;
-    da = d;
-    wprev = w;
-    sprev = sg;
-  };
-
-  // inbound: u = u0 + i du, i = 1 .. nstep-1 (blackhole_solution_test.cc:217-239)
-  for (int i = 1; i < nstep && alive; ++i) step((float)(nstep - i));
-  // final approach: u += 0.9 du, still weighted du/2 (:240-262)
-  if (alive) step(0.1f);
-  if (alive && captured) {
-    // chord to the centre: the horizon is always entered (r > 3M > 2M) and
-    // the disc test sees n.(B - B) = 0 (:264-272)
-    hit.cls = BH_HIT_HORIZON;
-    hit.ang = a;
-    hit.rho = (float)s.Rh;
-    alive = false;
+    const double t = (u - s.l0) * s.inv_cell;
+    const double kc = floor(t);
+    const double fr = t - kc;
+    if (fr > 1e-5 && fr < 1.0 - 1e-5 && kc >= 0.0 && kc < 1048576.0)
+      r.l = __builtin_fma(kc, s.cell, s.l0);
+    else
+      bisect = true;
   }
-  // outbound: u -= du (:274-296)
-  for (int j = 1; j < nstep && alive; ++j) step(0.1f + (float)j);
-  if (ESCAPE) {
-    // keep stepping outward until the next point would lie at r >= r_escape
-    for (int j = nstep; alive && steps < s.step_cap; ++j) {
-      const float sg = 0.1f + (float)j;
-      const double un = s.u0 + ((double)nstep - (double)sg) * L.du64;
-      if (un * s.r_esc <= 1.0) break;
-      step(sg);
-    }
-  }
+  if (bisect) r.l = solve_g64(sqrt(cc / dd), s.M, s.l0, s.root);
+}
 
-  // ---- finalisation -------------------------------------------------------
+// Hit point, colour and output planes of one ray.
+template <bool ESCAPE>
+__device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t idx,
+                                  const RaySetup& r, const MarchOut& mo) {
+  const float e1x = r.sgn * s.e1f[0], e1y = r.sgn * s.e1f[1], e1z = r.sgn * s.e1f[2];
   uint32_t rgb = 0;
   float qx = __builtin_nanf(""), qy = qx, qz = qx;
-  if (hit.cls != BH_HIT_NONE) {
-    float sh, ch;
-    sincosf((float)hit.ang, &sh, &ch);
-    qx = (float)s.B[0] + hit.rho * (ch * e1x + sh * e2x);
-    qy = (float)s.B[1] + hit.rho * (ch * e1y + sh * e2y);
-    qz = (float)s.B[2] + hit.rho * (ch * e1z + sh * e2z);
-    if (hit.cls == BH_HIT_DISC) rgb = disc_color32(s, qx, qy, qz);
+  if (mo.hit.cls != BH_HIT_NONE) {
+    if (!r.degenerate) {
+      float sa, ca;
+      sincos_angle(mo.hit.ang, &sa, &ca);
+      const float X = mo.hit.hx * ca - mo.hit.hy * sa;
+      const float Y = mo.hit.hx * sa + mo.hit.hy * ca;
+      qx = __builtin_fmaf(Y, r.e2x, __builtin_fmaf(X, e1x, s.Bf[0]));
+      qy = __builtin_fmaf(Y, r.e2y, __builtin_fmaf(X, e1y, s.Bf[1]));
+      qz = __builtin_fmaf(Y, r.e2z, __builtin_fmaf(X, e1z, s.Bf[2]));
+      if (mo.hit.cls == BH_HIT_DISC) rgb = disc_color32(s, qx, qy, qz);
+    }
   } else if (ESCAPE && s.has_bg) {
-    // direction of the last chord
-    // outbound steps are one du apart: the previous point sits at sigma - 1
-    const float rb = 1.0f / (float)(s.u0 + ((double)nstep - (double)sprev) * L.du64);
-    const float ra = 1.0f / (float)(s.u0 + ((double)nstep - (double)sprev + 1.0) * L.du64);
+    // direction of the last chord; outbound points are one du apart, the
+    // last one at sigma = 0.1 + steps - nstep
+    const double kb = (double)s.nstep - (0.1 + (double)(mo.steps - s.nstep));
+    const float rb = 1.0f / (float)__builtin_fma(kb, mo.du64, s.u0);
+    const float ra = 1.0f / (float)__builtin_fma(kb + 1.0, mo.du64, s.u0);
     float sb, cb, sa, ca;
-    sincosf((float)a, &sb, &cb);
-    sincosf((float)(a - (double)da), &sa, &ca);
-    const float px = rb * cb - ra * ca, py = rb * sb - ra * sa;
-    float dx = px * e1x + py * e2x, dy = px * e1y + py * e2y, dz = px * e1z + py * e2z;
-    const float inv = rsqrtf(dx * dx + dy * dy + dz * dz);
+    sincos_angle(mo.a, &sb, &cb);
+    sincos_angle(mo.a - (double)mo.d, &sa, &ca);
+    const float X = rb * cb - ra * ca, Y = rb * sb - ra * sa;
+    const float dx = X * e1x + Y * r.e2x, dy = X * e1y + Y * r.e2y, dz = X * e1z + Y * r.e2z;
+    const float inv = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
     rgb = equirect32(s.bg, dx * inv, dy * inv, dz * inv);
   }
-
   if (out.rgb) {
     out.rgb[3 * idx + 0] = (uint8_t)(rgb & 0xff);
     out.rgb[3 * idx + 1] = (uint8_t)((rgb >> 8) & 0xff);
     out.rgb[3 * idx + 2] = (uint8_t)((rgb >> 16) & 0xff);
   }
-  if (out.hit) out.hit[idx] = (uint8_t)hit.cls;
-  if (out.steps) out.steps[idx] = steps;
+  if (out.hit) out.hit[idx] = (uint8_t)mo.hit.cls;
+  if (out.steps) out.steps[idx] = mo.steps;
   if (out.hitpt) {
     out.hitpt[3 * idx + 0] = (double)qx;
     out.hitpt[3 * idx + 1] = (double)qy;
     out.hitpt[3 * idx + 2] = (double)qz;
   }
-  if (out.phi) {
-    // phi0 = atan(b / Y0) (quirk Q2); b / Y0 = |cv| / (dv . F)
-    out.phi[idx] = atan(sqrt(cc) / dF) + a;
+  if (out.phi) out.phi[idx] = r.degenerate ? mo.a : atan(sqrt(r.cc) / r.dF) + mo.a;
+}
+
+__device__ inline void degenerate_out(MarchOut& mo) {
+  // the pixel line passes through B (quirk Q13): captured / black
+  mo.hit.cls = BH_HIT_HORIZON;
+  mo.hit.ang = 0.0;
+  mo.hit.hx = mo.hit.hy = 0.f;
+  mo.steps = 0;
+  mo.a = __builtin_nan("");
+  mo.du64 = 0.0;
+  mo.d = 0.f;
+}
+
+template <bool ESCAPE>
+__global__ __launch_bounds__(256) void march_kernel(const DevScene* __restrict__ sp, DevOut out,
+                                                    DevQueue q) {
+  const DevScene& s = *sp;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= s.W || y >= s.H) return;
+  const size_t idx = (size_t)y * s.W + x;
+  RaySetup r;
+  setup_ray(s, x, y, false, r);
+  if (r.band && !r.degenerate) {
+    // queued for the FP64 march; a full queue marches the ray here in FP32
+    const uint32_t slot = atomicAdd(q.count, 1u);
+    if (slot < q.capacity) {
+      q.items[slot] = (uint32_t)idx;
+      return;
+    }
   }
+  MarchOut mo;
+  if (r.degenerate) {
+    degenerate_out(mo);
+  } else {
+    Lane<float> L;
+    setup_lane<float>(s, r.l, r.ib2, r.e2x, r.e2y, r.e2z, r.sgn, L);
+    march<float, ESCAPE>(s, L, r.captured, r.first_aa, mo);
+  }
+  finish_ray<ESCAPE>(s, out, idx, r, mo);
+}
+
+// FP64 march of the photon-sphere band rays queued by march_kernel.  The last
+// workgroup to finish clears the queue for the next frame.
+template <bool ESCAPE>
+__global__ __launch_bounds__(64) void band_kernel(const DevScene* __restrict__ sp, DevOut out,
+                                                  DevQueue q) {
+  const DevScene& s = *sp;
+  const uint32_t n = min(q.count[0], q.capacity);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t idx = q.items[i];
+    const int x = (int)(idx % (uint32_t)s.W), y = (int)(idx / (uint32_t)s.W);
+    RaySetup r;
+    setup_ray(s, x, y, true, r);
+    MarchOut mo;
+    Lane<double> L;
+    setup_lane<double>(s, r.l, r.ib2, r.e2x, r.e2y, r.e2z, r.sgn, L);
+    march<double, ESCAPE>(s, L, r.captured, r.first_aa, mo);
+    finish_ray<ESCAPE>(s, out, idx, r, mo);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t done = atomicAdd(q.count + 1, 1u);
+    if (done == gridDim.x - 1) {
+      atomicExch(q.count, 0u);
+      atomicExch(q.count + 1, 0u);
+    }
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -384,16 +591,20 @@ hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, in
                              const DevQueue& q, hipStream_t stream, bool escape) {
   dim3 grid((W + 15) / 16, (H + 15) / 16);
   if (escape)
-    hipLaunchKernelGGL(march32_kernel<true>, grid, dim3(256), 0, stream, scene, out, q);
+    hipLaunchKernelGGL(march_kernel<true>, grid, dim3(256), 0, stream, scene, out, q);
   else
-    hipLaunchKernelGGL(march32_kernel<false>, grid, dim3(256), 0, stream, scene, out, q);
+    hipLaunchKernelGGL(march_kernel<false>, grid, dim3(256), 0, stream, scene, out, q);
   return hipGetLastError();
 }
 
-hipError_t launch_band_fp64(const DevScene* scene, const DevOut& out, const DevQueue& q,
-                            hipStream_t stream) {
-  hipLaunchKernelGGL(band64_kernel, dim3(512), dim3(64), 0, stream, scene, out, q.items,
-                     q.count, q.capacity);
+// Band grid: 64 workgroups of one wave; every workgroup reaches the queue
+// reset, so the queue is clear when the launch completes.
+hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,
+                       hipStream_t stream, bool escape) {
+  if (escape)
+    hipLaunchKernelGGL(band_kernel<true>, dim3(64), dim3(64), 0, stream, scene, out, q);
+  else
+    hipLaunchKernelGGL(band_kernel<false>, dim3(64), dim3(64), 0, stream, scene, out, q);
   return hipGetLastError();
 }
 
