@@ -215,6 +215,7 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
   d->lo2 = o.band > 0.0 ? blo * blo : d->bc2;
   d->hi2 = o.band > 0.0 ? bhi * bhi : d->bc2;
   d->Uc = 1.0 / (3.0 * d->M);
+  d->r02 = r0 * r0;
   d->cell = (d->Uc - d->l0) * (1.0 / 1048576.0);
   d->inv_cell = 1048576.0 / (d->Uc - d->l0);
   d->alpha_f = d->nf[0] * d->e1f[0] + d->nf[1] * d->e1f[1] + d->nf[2] * d->e1f[2];

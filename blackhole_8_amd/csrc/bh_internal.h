@@ -51,6 +51,7 @@ struct DevScene {
   double fvB[3];          // focus vector relative to B
   double bc2, lo2, hi2;   // b_c^2 and the band's bounds on b^2
   double Uc;              // 1/(3M)
+  double r02;             // |C - B|^2
   double cell, inv_cell;  // SolveG grid: (1/(3M) - l0) / 2^20 and its inverse
   float e1f[3];           // F / |F|
   float nf[3];            // disc normal

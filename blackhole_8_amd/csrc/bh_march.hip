@@ -163,9 +163,10 @@ template <typename T>
 struct Lane {
   T c0, c1, c2, c3;   // G(s) cubic
   T du, h;            // du and du/2
+  T du01;             // 0.1 du
+  T u0;               // 1/|C - B|
   float phi_safe;     // horizon guard angle
   float guard;        // disc crossing-angle guard
-  double du64;        // du in FP64 (event positions)
   double chi0;        // a disc crossing angle (crossings at chi0 + j pi)
   double a_ev;        // next disc event angle (crossing - guard)
 };
@@ -176,15 +177,13 @@ struct HitRec {
   float hx, hy;  // hit point relative to B in that frame
 };
 
-// Exact tests of the chord between (sigma_a, a_a) and (sigma_b, a_b), where
-// u = u0 + (nstep - sigma) du and a is the swept angle.
+// Exact tests of the chord from (kappa_a, a_a) to (kappa_b, a_b), where
+// u = u0 + kappa du and a is the swept angle; da = a_b - a_a.
 template <typename T>
-__device__ inline bool event_chord(const DevScene& s, const Lane<T>& L, T sga, double aa, T sgb,
-                                   double ab, bool th, bool td, HitRec& hit) {
-  const double ua = __builtin_fma((double)s.nstep - (double)sga, L.du64, s.u0);
-  const double ub = __builtin_fma((double)s.nstep - (double)sgb, L.du64, s.u0);
-  const T ra = rcp<T>((T)ua);
-  const T rb = rcp<T>((T)ub);
+__device__ inline bool event_chord(const DevScene& s, const Lane<T>& L, T ka, double aa, T kb,
+                                   double ab, T da, bool th, bool td, HitRec& hit) {
+  const T ra = rcp<T>(fma_t<T>(ka, L.du, L.u0));
+  const T rb = rcp<T>(fma_t<T>(kb, L.du, L.u0));
   bool found = false;
   T best = 0;
   if (th) {
@@ -192,7 +191,7 @@ __device__ inline bool event_chord(const DevScene& s, const Lane<T>& L, T sga, d
     // StaticBlackhole::Collide (blackhole_solution.h:65-88) in the plane
     const T R = (T)s.Rh;
     T sd, cd;
-    sincos_t<T>((T)(ab - aa), &sd, &cd);
+    sincos_t<T>(da, &sd, &cd);
     const T qx = rb * cd - ra, qy = rb * sd;
     const T qd = qx * ra;
     const T qq = qx * qx + qy * qy;
@@ -217,8 +216,7 @@ __device__ inline bool event_chord(const DevScene& s, const Lane<T>& L, T sga, d
     // on the chord with weights |t_b|, |t_a|
     const double fa = floor((aa - L.chi0) * kInvPiD);
     const double fb = floor((ab - L.chi0) * kInvPiD);
-    const double n = fabs(fb - fa);
-    if (fmod(n, 2.0) == 1.0) {
+    if (((int)fb - (int)fa) & 1) {
       const double chik = L.chi0 + fmax(fa, fb) * kPiD;
       T sa, ca, sb, cb;
       sincos_t<T>((T)(aa - chik), &sa, &ca);
@@ -250,84 +248,147 @@ __device__ __forceinline__ double next_event(double a, double chi0, float guard)
 
 struct MarchOut {
   HitRec hit;
-  int steps;
-  double a;      // swept angle after the last step
+  int steps;     // steps taken
+  double a;      // swept angle after the last step taken
   double du64;   // du of the ray
-  float d;       // last increment (for the escape direction)
+  float d;       // last increment taken (for the escape direction)
 };
 
 // The march of one ray in precision T.  first_aa is the angle of the camera
 // in the (e1, e2) frame: 0 when Y0 > 0, pi when Y0 < 0 (quirk Q2: the
 // reference's phi0 = atan(b/Y0) then points away from the camera, and only
 // the first chord starts at the camera itself).
+//
+// Step n has u = u0 + kappa_n du and s = l - u = sigma_n du with
+//   inbound   n < nstep:  kappa = n,                 sigma = nstep - n
+//   approach  n = nstep:  kappa = nstep - 0.1,       sigma = 0.1
+//   outbound  n > nstep:  kappa = 2 nstep - 0.1 - n, sigma = 0.1 + (n - nstep)
+//
+// Every lane of a wave is at the same step, so the loops run wave-uniform
+// trip counts and keep no per-lane flag: a ray that has ended is disarmed by
+// setting its event thresholds to +inf (its results were recorded when it
+// ended; it keeps stepping harmlessly), and every four steps the wave leaves
+// once no lane is armed.  This keeps the exec-mask bookkeeping of the hot
+// loop to a handful of scalar instructions.
 template <typename T, bool ESCAPE>
 __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, double first_aa,
-                             MarchOut& o) {
+                             double du64, MarchOut& o) {
   const int nstep = s.nstep;
   T w = (T)0;
   double a = 0.0;
-  bool alive = true;
+  T d = (T)0;
   o.hit.cls = BH_HIT_NONE;
   o.hit.ang = 0.0;
   o.hit.hx = o.hit.hy = 0.f;
-  o.du64 = L.du64;
-  T d = (T)0;
-  int n = 1;
-
-  auto step = [&](T sg, T sg_prev, bool force) {
-    const T sv = sg * L.du;
+  o.du64 = du64;
+  o.steps = 0;
+  o.a = 0.0;
+  o.d = 0.f;
+  const float kInf = __builtin_inff();
+  auto end_ray = [&](int n) {
+    o.steps = n;
+    o.a = a;
+    o.d = (float)d;
+    L.phi_safe = kInf;
+    L.a_ev = 1e308;
+  };
+  auto armed = [&]() { return L.phi_safe < kInf; };
+  auto step = [&](int n, T sv, T ka, T kb, bool force) {
     const T G = fma_t<T>(fma_t<T>(fma_t<T>(L.c3, sv, L.c2), sv, L.c1), sv, L.c0);
     const T wn = rsq<T>(G);
     d = (w + wn) * L.h;
     w = wn;
     a += (double)d;
     const bool eh = (float)d > L.phi_safe;
-    const bool ed = a >= L.a_ev;
-    if (eh || ed || force) {
+    if (eh || a >= L.a_ev || force) {
       const double aa = force ? first_aa : a - (double)d;
-      if (event_chord<T>(s, L, sg_prev, aa, sg, a, eh || force, ed || force, o.hit)) {
-        alive = false;
-        o.steps = n;
-      } else if (ed || force) {
+      const T da = force ? (T)(a - first_aa) : d;
+      if (event_chord<T>(s, L, ka, aa, kb, a, da, eh || force, s.has_disc != 0, o.hit))
+        end_ray(n);
+      else if (L.chi0 != 1e300)
         L.a_ev = next_event(a, L.chi0, L.guard);
-      }
     }
   };
-
-  // inbound: u = u0 + i du (blackhole_solution_test.cc:217-239)
-  step((T)(nstep - 1), (T)nstep, first_aa != 0.0);
-  for (n = 2; n < nstep && alive; ++n) step((T)(nstep - n), (T)(nstep - n + 1), false);
-  // final approach: u += 0.9 du, weighted du/2 (:240-262)
-  if (alive) {
-    n = nstep;
-    step((T)0.1, (T)1, false);
-  }
-  if (alive && captured) {
-    // chord to the centre: the horizon is always entered (r > 3M > 2M) and
-    // the disc test sees n.(B - B) = 0 (:264-272)
-    o.hit.cls = BH_HIT_HORIZON;
-    o.hit.ang = a;
-    o.hit.hx = (float)s.Rh;
-    o.hit.hy = 0.f;
-    o.steps = nstep;
-    alive = false;
-  }
-  // outbound: u -= du (:274-296)
-  for (n = nstep + 1; n < 2 * nstep && alive; ++n)
-    step((T)0.1 + (T)(n - nstep), (T)0.1 + (T)(n - nstep - 1), false);
-  if (ESCAPE) {
-    // extension: keep stepping outward until the next point would lie at
-    // r >= r_escape (u <= 1/r_escape), or the step cap is reached
-    for (; alive && n <= s.step_cap; ++n) {
-      const T sg = (T)0.1 + (T)(n - nstep);
-      const double un = __builtin_fma((double)nstep - (double)sg, L.du64, s.u0);
-      if (un * s.r_esc <= 1.0) break;
-      step(sg, sg - (T)1, false);
+  // inbound (blackhole_solution_test.cc:217-239)
+  step(1, (T)(nstep - 1) * L.du, (T)0, (T)1, first_aa != 0.0);
+  int n = 2;
+  for (; n + 3 < nstep; n += 4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const T kn = (T)(n + k);
+      step(n + k, ((T)nstep - kn) * L.du, kn - (T)1, kn, false);
     }
+    if (!__any(armed())) goto done;
   }
-  if (alive) o.steps = n - 1;
-  o.a = a;
-  o.d = (float)d;
+  for (; n < nstep; ++n) {
+    const T kn = (T)n;
+    step(n, ((T)nstep - kn) * L.du, kn - (T)1, kn, false);
+  }
+  {
+    // final approach: u += 0.9 du, weighted du/2 (:240-262)
+    const T kt = (T)nstep - (T)0.1;
+    step(nstep, L.du01, kt - (T)0.9, kt, false);
+    // captured: chord to the centre; the horizon is always entered
+    // (r > 3M > 2M) and the disc test sees n.(B - B) = 0 (:264-272)
+    if (captured && armed()) {
+      o.hit.cls = BH_HIT_HORIZON;
+      o.hit.ang = a;
+      o.hit.hx = (float)s.Rh;
+      o.hit.hy = 0.f;
+      end_ray(nstep);
+    }
+    if (!__any(armed())) goto done;
+    // outbound: u -= du (:274-296)
+    n = nstep + 1;
+    for (; n + 3 < 2 * nstep; n += 4) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const T j = (T)(n + k - nstep);
+        step(n + k, fma_t<T>(j, L.du, L.du01), kt - j + (T)1, kt - j, false);
+      }
+      if (!__any(armed())) goto done;
+    }
+    for (; n < 2 * nstep; ++n) {
+      const T j = (T)(n - nstep);
+      step(n, fma_t<T>(j, L.du, L.du01), kt - j + (T)1, kt - j, false);
+    }
+    if (ESCAPE) {
+      // extension: keep stepping outward until the next point would lie at
+      // r >= r_escape (u <= 1/r_escape), or the step cap is reached
+      for (; n <= s.step_cap; ++n) {
+        if (!__any(armed())) goto done;
+        const T j = (T)(n - nstep);
+        const double un = __builtin_fma((double)kt - (double)j, du64, s.u0);
+        if (armed() && un * s.r_esc <= 1.0) end_ray(n - 1);
+        step(n, fma_t<T>(j, L.du, L.du01), kt - j + (T)1, kt - j, false);
+      }
+    }
+    if (armed()) end_ray(n - 1);
+  }
+done:;
+}
+
+// atan2 in FP32 to ~1e-7 rad: octant reduction and an odd polynomial for
+// atan on [0, 1] (coefficients from a least-squares fit at Chebyshev nodes;
+// fit error 3e-8).
+__device__ inline float atan2_fast(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+  const float t2 = t * t;
+  float p = 0.0024682467f;
+  p = __builtin_fmaf(p, t2, -0.014458697f);
+  p = __builtin_fmaf(p, t2, 0.039899562f);
+  p = __builtin_fmaf(p, t2, -0.072479509f);
+  p = __builtin_fmaf(p, t2, 0.10507320f);
+  p = __builtin_fmaf(p, t2, -0.14164333f);
+  p = __builtin_fmaf(p, t2, 0.19986537f);
+  p = __builtin_fmaf(p, t2, -0.33332658f);
+  p = __builtin_fmaf(p, t2, 0.99999988f);
+  float r = p * t;
+  if (ay > ax) r = 1.57079633f - r;
+  if (x < 0.f) r = 3.14159265f - r;
+  return y < 0.f ? -r : r;
 }
 
 // Disc texel of a world point q (Rectangle::color, vector_object.h:163-179)
@@ -355,18 +416,18 @@ __device__ inline uint32_t equirect32(const DevTex& t, float dx, float dy, float
 
 // Lane set-up in precision T from the FP64 quantities of the ray.
 template <typename T>
-__device__ inline void setup_lane(const DevScene& s, double l, double ib2, float e2x, float e2y,
-                                  float e2z, float sgn, Lane<T>& L) {
+__device__ inline void setup_lane(const DevScene& s, double l, double ib2, double du, float e2x,
+                                  float e2y, float e2z, float sgn, Lane<T>& L) {
   const double M = s.M;
-  const double du = (l - s.u0) * (1.0 / s.nstep);
   const double dl = 1.0 - 3.0 * M * l;
   L.c0 = (T)__builtin_fma(l * l, __builtin_fma(2.0 * M, l, -1.0), ib2);
   L.c1 = (T)(2.0 * l * dl);
   L.c2 = (T)(1.0 - 2.0 * dl);
   L.c3 = (T)(-2.0 * M);
   L.du = (T)du;
-  L.h = (T)(du / 2.);
-  L.du64 = du;
+  L.h = (T)(0.5 * du);
+  L.du01 = (T)(0.1 * du);
+  L.u0 = (T)s.u0;
   // smallest radius on the march: u = u0 + (nstep - 0.1) du
   const float x = (float)(s.Rh * __builtin_fma((double)s.nstep - 0.1, du, s.u0));
   L.phi_safe = x < 1.f ? 3.14159265f * (1.f - x) * 0.9999f : 0.f;
@@ -376,14 +437,15 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, float
   if (s.has_disc) {
     const float al = sgn * s.alpha_f;
     const float be = s.nf[0] * e2x + s.nf[1] * e2y + s.nf[2] * e2z;
-    const float amp = __builtin_amdgcn_sqrtf(al * al + be * be);
-    if (amp > 0.f) {
+    const float amp2 = al * al + be * be;
+    if (amp2 > 0.f) {
       // al cos a + be sin a vanishes at a = atan2(be, al) + pi/2 (mod pi)
-      L.chi0 = (double)atan2f(be, al) + kPiD / 2.0;
-      L.guard = 4e-6f + 4e-7f / amp;
+      L.chi0 = (double)atan2_fast(be, al) + kPiD / 2.0;
+      L.guard = 4e-6f + 4e-7f * __builtin_amdgcn_rsqf(amp2);
       L.a_ev = next_event(0.0, L.chi0, L.guard);
     }
   }
+  if (L.a_ev == 1e300) L.chi0 = 1e300;  // no crossings: parity tests stay even
 }
 
 // Per-ray set-up shared by the FP32 march and the FP64 band kernel.
@@ -421,11 +483,10 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
   if (r.degenerate) return;
   r.ib2 = dd / cc;  // 1 / b^2
   // e2 = sign(Y0) ((dv.F) F - r0^2 dv) / (r0 |cv|)
-  const double r02 = 1.0 / (s.u0 * s.u0);
-  const double k = (double)r.sgn * s.u0 * rsqrt(cc);
-  r.e2x = (float)(__builtin_fma(dF, Fx, -r02 * dvx) * k);
-  r.e2y = (float)(__builtin_fma(dF, Fy, -r02 * dvy) * k);
-  r.e2z = (float)(__builtin_fma(dF, Fz, -r02 * dvz) * k);
+  const double k = (double)r.sgn * s.u0 * rsq<double>(cc);
+  r.e2x = (float)(__builtin_fma(dF, Fx, -s.r02 * dvx) * k);
+  r.e2y = (float)(__builtin_fma(dF, Fy, -s.r02 * dvy) * k);
+  r.e2z = (float)(__builtin_fma(dF, Fz, -s.r02 * dvz) * k);
   if (r.captured) return;
   // Turning point: StaticBlackhole::SolveG's lower bisection end.  The root
   // u* = (1/(3M)) (1/2 + sin(asin(2q - 1)/3)), q = (b_c/b)^2, is refined by
@@ -443,7 +504,9 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
     for (int it = 0; it < 2; ++it) {
       const double g = __builtin_fma(u * u, __builtin_fma(M2, u, -1.0), r.ib2);
       const double gp = u * __builtin_fma(M6, u, -2.0);
-      u = u - g / gp;
+      double ig = __builtin_amdgcn_rcp(gp);
+      ig = ig * __builtin_fma(-gp, ig, 2.0);
+      u = __builtin_fma(-g, ig, u);
     }
     const double t = (u - s.l0) * s.inv_cell;
     const double kc = floor(t);
@@ -537,8 +600,9 @@ __global__ __launch_bounds__(256) void march_kernel(const DevScene* __restrict__
     degenerate_out(mo);
   } else {
     Lane<float> L;
-    setup_lane<float>(s, r.l, r.ib2, r.e2x, r.e2y, r.e2z, r.sgn, L);
-    march<float, ESCAPE>(s, L, r.captured, r.first_aa, mo);
+    const double du = (r.l - s.u0) * (1.0 / s.nstep);
+    setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
+    march<float, ESCAPE>(s, L, r.captured, r.first_aa, du, mo);
   }
   finish_ray<ESCAPE>(s, out, idx, r, mo);
 }
@@ -557,8 +621,9 @@ __global__ __launch_bounds__(64) void band_kernel(const DevScene* __restrict__ s
     setup_ray(s, x, y, true, r);
     MarchOut mo;
     Lane<double> L;
-    setup_lane<double>(s, r.l, r.ib2, r.e2x, r.e2y, r.e2z, r.sgn, L);
-    march<double, ESCAPE>(s, L, r.captured, r.first_aa, mo);
+    const double du = (r.l - s.u0) * (1.0 / s.nstep);
+    setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
+    march<double, ESCAPE>(s, L, r.captured, r.first_aa, du, mo);
     finish_ray<ESCAPE>(s, out, idx, r, mo);
   }
   __syncthreads();
