@@ -111,7 +111,7 @@ def main() -> int:
     kern_ms, total_ms, work, band = 0.0, 0.0, 0.0, 0
     for k in range(F):
         sc = p.with_camera(p.cameras[k]).scene
-        m, t, n = r.time_render(sc, p.options, iters=2)
+        m, t, n = r.time_render(sc, p.options, iters=4)
         out = r.render(sc, p.options, ("hit", "steps"))
         kern_ms += m
         total_ms += t
@@ -127,7 +127,7 @@ def main() -> int:
             traffic = None
     roofline = {
         "bound": "valu",
-        "kernel": "march32_kernel",
+        "kernel": "march_kernel<false> (FP32 event march)",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",

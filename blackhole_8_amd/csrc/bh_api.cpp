@@ -486,28 +486,34 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
   // warm-up
   if ((rc = launch_frame(ctx, d, ds, o, q, st))) return rc;
   HIP_TRY(ctx, hipStreamSynchronize(st));
-  double main_ms = 0, total_ms = 0;
+  // main kernel: iters back-to-back launches between one event pair (the
+  // queue is cleared afterwards); whole frame: iters march + band launches
+  float main_ms = 0, total_ms = 0;
   uint32_t nband = 0;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
   for (int it = 0; it < iters; ++it) {
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     if (fast) {
       HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0));
     } else {
       HIP_TRY(ctx, bh::launch_march_fp64(ds, o, d.W, d.H, st));
     }
-    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-    if (fast) {
-      if (it == iters - 1) HIP_TRY(ctx, hipMemcpyAsync(&nband, qc, 4, hipMemcpyDeviceToHost, st));
-      HIP_TRY(ctx, bh::launch_band(ds, o, q, st, d.escape != 0));
-    }
-    HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
-    HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
-    float a = 0, b = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->ev0, ctx->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(&b, ctx->ev0, ctx->ev2));
-    main_ms += a;
-    total_ms += b;
   }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  if (fast) {
+    HIP_TRY(ctx, hipMemcpyAsync(&nband, qc, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, bh::launch_zero_u32(qc, 2, st));
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+  for (int it = 0; it < iters; ++it)
+    if ((rc = launch_frame(ctx, d, ds, o, q, st))) return rc;
+  hipEvent_t end;
+  HIP_TRY(ctx, hipEventCreate(&end));
+  HIP_TRY(ctx, hipEventRecord(end, st));
+  HIP_TRY(ctx, hipEventSynchronize(end));
+  HIP_TRY(ctx, hipEventElapsedTime(&main_ms, ctx->ev0, ctx->ev1));
+  HIP_TRY(ctx, hipEventElapsedTime(&total_ms, ctx->ev2, end));
+  (void)hipEventDestroy(end);
+  if (fast) nband /= (uint32_t)iters;
   if (avg_ms_main) *avg_ms_main = main_ms / iters;
   if (avg_ms_total) *avg_ms_total = total_ms / iters;
   if (band_rays) *band_rays = nband;
