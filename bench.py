@@ -49,6 +49,30 @@ def host_threads() -> int:
     return max(1, min(16, n))
 
 
+def adjoint_timing(iters: int = 50) -> dict:
+    """Forward+adjoint against the forward on the 512x512 inverse-fit scene
+    (FP32): the differentiable forward, the fused forward+adjoint launch, and
+    the plain march (reference rule) of the same scene."""
+    from blackhole_8_amd import Renderer, abi, diff, scenes
+
+    r = Renderer(0)
+    try:
+        diff.load_target_textures(r)
+        dr = diff.DiffRenderer(r, diff.DiffConfig())
+        fwd, fwd_adj = dr.time(diff.FIT_TRUE, iters)
+        pl = scenes.config4_plain(diff.FIT_TRUE, dr.cfg.width, dr.cfg.height, dr.cfg.fov,
+                                  dr.cfg.r_outer)
+        plain, _, _ = r.time_render(pl.scene, pl.options, iters)
+    finally:
+        r.close()
+    return {"workload": "config4: 512x512 hole + disc, r_s 1.2, inclination 72 deg, camera at "
+                        "18 r_s, r_inner 3.5 r_s, FP32; d/d(r_s, incl, dist, r_inner)",
+            "fwd_ms": round(fwd, 5), "fwd_adj_ms": round(fwd_adj, 5),
+            "ratio_vs_diff_fwd": round(fwd_adj / fwd, 3),
+            "plain_fwd_ms": round(plain, 5), "ratio_vs_plain_fwd": round(fwd_adj / plain, 3),
+            "iters": iters}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +81,7 @@ def main() -> int:
     ap.add_argument("--frames", type=int, default=120)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=3)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times), for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -171,8 +196,10 @@ def main() -> int:
     path.close()
     r.close()
 
+    adjoint = None if args.no_adjoint else adjoint_timing()
+
     line = {
-        "metric": "Mrays/s at 1080p BH+disc",
+        "metric": "Mrays/s at 1080p BH+disc; fwd+adjoint ms/frame vs plain fwd; % FP32 peak",
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -193,6 +220,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy_frame0": accuracy,
+        "adjoint": adjoint,
     }
     print(json.dumps(line))
     return 0

@@ -78,6 +78,13 @@ class Outputs(C.Structure):
                 ("hit_point", C.c_void_p), ("steps", C.c_void_p)]
 
 
+class DiffParams(C.Structure):
+    _fields_ = [("r_s", C.c_double), ("incl", C.c_double), ("dist", C.c_double),
+                ("r_inner", C.c_double), ("r_outer", C.c_double), ("azim", C.c_double),
+                ("fov", C.c_double), ("edge_width", C.c_double), ("width", C.c_int32),
+                ("height", C.c_int32), ("texture", C.c_int32), ("pad_", C.c_int32)]
+
+
 class Rect(C.Structure):
     _fields_ = [("vertex", d3 * 5), ("texture", C.c_int32), ("pad_", C.c_int32)]
 
@@ -116,6 +123,10 @@ SIGNATURES = {
     "bh_time_render": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options), C.c_int,
                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int64)]),
+    "bh_diff_render_device": (C.c_int, [P, C.POINTER(DiffParams), C.c_int, P, P]),
+    "bh_diff_adjoint_device": (C.c_int, [P, C.POINTER(DiffParams), C.c_int, P, P, P, P, P]),
+    "bh_diff_time": (C.c_int, [P, C.POINTER(DiffParams), C.c_int, C.c_int,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bh_render_flat": (C.c_int, [P, C.POINTER(FlatScene), C.POINTER(Outputs)]),
     "bh_focal_length": (C.c_double, [C.c_int, C.c_double]),
     "bh_object_rotate": (None, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
@@ -143,6 +154,15 @@ def load(path: os.PathLike | None = None):
     p = Path(path) if path else LIB_PATH
     if not p.exists():
         raise RuntimeError(f"{p} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    # One HIP runtime per process: torch ships its own libamdhip64 under the
+    # same soname (libamdhip64.so.7) but asks for it by another file name, so
+    # loading this library first would bring a second runtime in when torch
+    # is imported later.  Importing torch first makes the library bind to
+    # torch's runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -36,6 +36,7 @@ struct bh_ctx {
   std::string err;
   DeviceTexture tex[BH_MAX_TEXTURES];
   DevBuf scene, rgb, hit, phi, hitpt, steps, qitems, qcount;
+  DevBuf dpart, dgrad, drgb, dhit, dgin;  // differentiable render
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   char name[256] = {0};
   int n_cu = 0, clock_khz = 0;
@@ -275,6 +276,103 @@ int queue_buffers(bh_ctx* ctx, uint32_t** items, uint32_t** count, size_t npix) 
   return BH_OK;
 }
 
+
+// ---- differentiable scene ---------------------------------------------------
+// FP64 number carrying d/d(inclination), for the camera frame.
+struct HD {
+  double v, t;
+};
+HD operator+(HD a, HD b) { return {a.v + b.v, a.t + b.t}; }
+HD operator-(HD a, HD b) { return {a.v - b.v, a.t - b.t}; }
+HD operator*(HD a, HD b) { return {a.v * b.v, a.t * b.v + a.v * b.t}; }
+struct HV {
+  HD x[3];
+};
+HD hdot(const HV& a, const HV& b) {
+  HD s{0.0, 0.0};
+  for (int i = 0; i < 3; ++i) s = s + a.x[i] * b.x[i];
+  return s;
+}
+HV hscale(const HV& a, HD k) { return {{a.x[0] * k, a.x[1] * k, a.x[2] * k}}; }
+HV hsub(const HV& a, const HV& b) { return {{a.x[0] - b.x[0], a.x[1] - b.x[1], a.x[2] - b.x[2]}}; }
+HV hcross(const HV& a, const HV& b) {
+  return {{a.x[1] * b.x[2] - a.x[2] * b.x[1], a.x[2] * b.x[0] - a.x[0] * b.x[2],
+           a.x[0] * b.x[1] - a.x[1] * b.x[0]}};
+}
+// v * (1/|v|), the OpenCV normalisation the camera frame uses
+HV hnormalize(const HV& a) {
+  const HD n2 = hdot(a, a);
+  const double nv = std::sqrt(n2.v);
+  const double k = 1.0 / nv;
+  const HD kk{k, -0.5 * k * k * k * n2.t};
+  return hscale(a, kk);
+}
+
+// Resolve bh_diff_params: camera at dist (sin i cos az, sin i sin az, cos i)
+// looking at the hole with +z up (x toward the hole, z = -(up orthogonal to
+// x), y = z x x), hole at the origin, disc in z = 0 with the texture square
+// (+-r_out, +-r_out).
+int build_diff(bh_ctx* ctx, const bh_diff_params* p, int precision, bh::DiffScene* d) {
+  if (!p) return fail(ctx, BH_ERR_INVALID_ARG, "params is NULL");
+  if (precision != BH_PREC_FP32 && precision != BH_PREC_FP64)
+    return fail(ctx, BH_ERR_INVALID_ARG, "unknown precision");
+  if (p->width <= 0 || p->height <= 0 || p->width > 16384 || p->height > 16384)
+    return fail(ctx, BH_ERR_INVALID_ARG, "width/height must be in 1..16384");
+  const double vals[] = {p->r_s, p->incl, p->dist, p->r_inner, p->r_outer, p->azim, p->fov,
+                         p->edge_width};
+  for (double v : vals)
+    if (!std::isfinite(v)) return fail(ctx, BH_ERR_INVALID_ARG, "parameters must be finite");
+  if (!(p->r_s > 0.0)) return fail(ctx, BH_ERR_INVALID_ARG, "r_s must be positive");
+  if (!(p->dist > p->r_s)) return fail(ctx, BH_ERR_INVALID_ARG, "camera must be outside the horizon");
+  if (!(p->r_outer > 0.0) || !(p->r_inner >= 0.0) || !(p->r_inner < p->r_outer))
+    return fail(ctx, BH_ERR_INVALID_ARG, "need 0 <= r_inner < r_outer");
+  if (!(p->edge_width >= 0.0)) return fail(ctx, BH_ERR_INVALID_ARG, "edge_width must be >= 0");
+  std::memset(d, 0, sizeof *d);
+  if (p->texture >= 0) {
+    d->tex = devtex(ctx, p->texture);
+    if (!d->tex.data) return fail(ctx, BH_ERR_EMPTY_TEXTURE, "disc texture slot is empty");
+  }
+  const double si = std::sin(p->incl), ci = std::cos(p->incl);
+  const double ca = std::cos(p->azim), sa = std::sin(p->azim);
+  const double dir[3] = {si * ca, si * sa, ci}, dir_i[3] = {ci * ca, ci * sa, -si};
+  HV C;
+  for (int k = 0; k < 3; ++k) {
+    C.x[k] = {p->dist * dir[k], p->dist * dir_i[k]};
+    d->C[k] = C.x[k].v;
+    d->C_i[k] = C.x[k].t;
+    d->C_d[k] = dir[k];
+  }
+  const HD m1{-1.0, 0.0};
+  const HV x = hnormalize(hscale(C, m1));
+  const HV up{{{0.0, 0.0}, {0.0, 0.0}, {1.0, 0.0}}};
+  const HD k = hdot(up, x);
+  const HV z = hnormalize(hscale(hsub(up, hscale(x, k)), m1));
+  const HV y = hnormalize(hcross(z, x));
+  const double f = bh_focal_length(p->width, p->fov);
+  for (int i = 0; i < 3; ++i) {
+    d->fx[i] = x.x[i].v * f;
+    d->fx_i[i] = x.x[i].t * f;
+    d->ay[i] = y.x[i].v;
+    d->ay_i[i] = y.x[i].t;
+    d->az[i] = z.x[i].v;
+    d->az_i[i] = z.x[i].t;
+  }
+  d->M = p->r_s / 2.0;
+  d->bc = 3.0 * std::sqrt(3.0) * d->M;
+  d->Rh = 2 * d->M;
+  d->Uc = 1.0 / (3 * d->M);
+  d->r_in = p->r_inner;
+  d->r_out = p->r_outer;
+  d->edge = p->edge_width;
+  d->half = p->r_outer;
+  d->uscale = d->tex.cols / (2.0 * p->r_outer);
+  d->vscale = d->tex.rows / (2.0 * p->r_outer);
+  d->W = p->width;
+  d->H = p->height;
+  d->nstep = 20;
+  return BH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -322,7 +420,8 @@ void bh_ctx_destroy(bh_ctx* ctx) {
   for (auto& t : ctx->tex)
     if (t.d) (void)hipFree(t.d);
   for (DevBuf* b : {&ctx->scene, &ctx->rgb, &ctx->hit, &ctx->phi, &ctx->hitpt, &ctx->steps,
-                    &ctx->qitems, &ctx->qcount})
+                    &ctx->qitems, &ctx->qcount, &ctx->dpart, &ctx->dgrad, &ctx->drgb, &ctx->dhit,
+                    &ctx->dgin})
     if (b->p) (void)hipFree(b->p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -647,6 +746,76 @@ void bh_path_destroy(bh_path* p) {
   for (auto ev : p->events)
     if (ev) (void)hipEventDestroy(ev);
   delete p;
+}
+
+/* ---- differentiable render -------------------------------------------------- */
+int bh_diff_render_device(bh_ctx* ctx, const bh_diff_params* p, int precision, float* rgb_out,
+                          uint8_t* hit_out) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  bh::DiffScene d;
+  int rc = build_diff(ctx, p, precision, &d);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  HIP_TRY(ctx, bh::launch_diff_forward(d, precision == BH_PREC_FP64, rgb_out, hit_out, ctx->stream));
+  return BH_OK;
+}
+
+int bh_diff_adjoint_device(bh_ctx* ctx, const bh_diff_params* p, int precision,
+                           const float* grad_rgb, float* rgb_out, uint8_t* hit_out,
+                           double* grad_out, uint32_t* overflow_out) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (!grad_rgb || !grad_out) return fail(ctx, BH_ERR_INVALID_ARG, "grad_rgb and grad_out are required");
+  bh::DiffScene d;
+  int rc = build_diff(ctx, p, precision, &d);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const size_t nb = (size_t)bh::diff_partial_blocks(d.W, d.H);
+  if ((rc = ensure(ctx, ctx->dpart, nb * 5 * sizeof(double)))) return rc;
+  HIP_TRY(ctx, bh::launch_diff_adjoint(d, precision == BH_PREC_FP64, grad_rgb, rgb_out, hit_out,
+                                       (double*)ctx->dpart.p, grad_out, overflow_out, ctx->stream));
+  return BH_OK;
+}
+
+int bh_diff_time(bh_ctx* ctx, const bh_diff_params* p, int precision, int iters, double* fwd_ms,
+                 double* fwd_adj_ms) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
+  bh::DiffScene d;
+  int rc = build_diff(ctx, p, precision, &d);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const size_t npix = (size_t)d.W * d.H;
+  const size_t nb = (size_t)bh::diff_partial_blocks(d.W, d.H);
+  if ((rc = ensure(ctx, ctx->dpart, nb * 5 * sizeof(double)))) return rc;
+  if ((rc = ensure(ctx, ctx->dgrad, 8 * sizeof(double)))) return rc;
+  if ((rc = ensure(ctx, ctx->drgb, npix * 3 * sizeof(float)))) return rc;
+  if ((rc = ensure(ctx, ctx->dhit, npix))) return rc;
+  if ((rc = ensure(ctx, ctx->dgin, npix * 3 * sizeof(float)))) return rc;
+  hipStream_t st = ctx->stream;
+  // dL/drgb: every byte 0x3c, i.e. 0.0115 per channel
+  HIP_TRY(ctx, hipMemsetAsync(ctx->dgin.p, 0x3c, npix * 3 * sizeof(float), st));
+  float* rgb = (float*)ctx->drgb.p;
+  uint8_t* hit = (uint8_t*)ctx->dhit.p;
+  const float* gin = (const float*)ctx->dgin.p;
+  double* part = (double*)ctx->dpart.p;
+  double* grad = (double*)ctx->dgrad.p;
+  const bool f64 = precision == BH_PREC_FP64;
+  HIP_TRY(ctx, bh::launch_diff_forward(d, f64, rgb, hit, st));
+  HIP_TRY(ctx, bh::launch_diff_adjoint(d, f64, gin, rgb, hit, part, grad, nullptr, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  float ms_f = 0, ms_a = 0;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  for (int i = 0; i < iters; ++i) HIP_TRY(ctx, bh::launch_diff_forward(d, f64, rgb, hit, st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  for (int i = 0; i < iters; ++i)
+    HIP_TRY(ctx, bh::launch_diff_adjoint(d, f64, gin, rgb, hit, part, grad, nullptr, st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+  HIP_TRY(ctx, hipEventElapsedTime(&ms_f, ctx->ev0, ctx->ev1));
+  HIP_TRY(ctx, hipEventElapsedTime(&ms_a, ctx->ev1, ctx->ev2));
+  if (fwd_ms) *fwd_ms = ms_f / iters;
+  if (fwd_adj_ms) *fwd_adj_ms = ms_a / iters;
+  return BH_OK;
 }
 
 int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* scene, const bh_outputs* out) {

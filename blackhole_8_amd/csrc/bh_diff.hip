@@ -1,0 +1,678 @@
+// bh_diff.hip — differentiable render of a hole + disc scene and its adjoint
+// with respect to (r_s, inclination, camera distance, disc inner radius).
+//
+// The march is the reference rule (include/blackhole/blackhole_solution_test.cc:
+// 161-298: orbit-plane frame, trapezoid sum with w_prev = 0 at the first step,
+// 0.9 du final approach, captured chord to the centre, outbound sweep) with
+// three smooth replacements that make the image a differentiable function of
+// the parameters: the exact turning point instead of the 20-step bisection,
+// bilinear texels instead of nearest, and a soft disc edge
+//   alpha = S((rho - r_in)/w) S((r_out - rho)/w),  S(x) = smoothstep(clamp(x + 1/2)),
+// composited front to back over the disc crossings of a ray until the horizon
+// or full coverage.  One lane per ray.
+//
+// Adjoint (reverse mode, fused with the forward in one kernel):
+//   pass 1   forward march; each disc crossing records (phi_{j-1}, phi_j, T_k)
+//            in LDS and sets bit j of a chord mask;
+//   pass 2   reverse over chords j = j_max .. 1.  The trajectory is not
+//            stored: the ray set-up is the one checkpoint, u_j and w_j are
+//            recomputed in closed form, and phi at a crossing comes from its
+//            record.  Compositing adjoint with the behind colour B built back
+//            to front; crossing point, coverage and bilinear texel adjoints
+//            give P_bar, hence (phi_bar, u_bar, yv_bar, xv_bar); suffix sums
+//            A_j of phi_bar give w_bar_j = h (A_j + A_{j+1}) and
+//            G_bar = -w^3 w_bar / 2, hence (u, M, 1/b^2);
+//   set-up   turning point by implicit differentiation of G(l) = 0, then the
+//            orbit-plane frame in forward mode (tangents: inclination,
+//            distance) contracted with the set-up adjoints.
+// Per-block partial sums are reduced in a fixed order by a second kernel, so
+// the gradient is deterministic.
+#include "bh_internal.h"
+#include "bh_math.h"
+
+namespace bh {
+namespace {
+
+constexpr int kTile = 16;
+constexpr int kThreads = kTile * kTile;
+constexpr int kMaxRec = BH_DIFF_MAX_CROSSINGS;
+constexpr int kPartial = 5;  // r_s, incl, dist, r_in, overflowed rays
+
+// ---------------------------------------------------------------------------
+// Forward-mode FP64 number: value and derivatives along inclination, distance
+// ---------------------------------------------------------------------------
+struct D2 {
+  double v, i, d;
+};
+__device__ __forceinline__ D2 operator+(D2 a, D2 b) { return {a.v + b.v, a.i + b.i, a.d + b.d}; }
+__device__ __forceinline__ D2 operator-(D2 a, D2 b) { return {a.v - b.v, a.i - b.i, a.d - b.d}; }
+__device__ __forceinline__ D2 operator*(D2 a, D2 b) {
+  return {a.v * b.v, a.i * b.v + a.v * b.i, a.d * b.v + a.v * b.d};
+}
+__device__ __forceinline__ double val(double a) { return a; }
+__device__ __forceinline__ double val(D2 a) { return a.v; }
+__device__ __forceinline__ double rsqrt_n(double a) { return 1.0 / sqrt(a); }
+__device__ __forceinline__ D2 rsqrt_n(D2 a) {
+  const double y = 1.0 / sqrt(a.v);
+  const double k = -0.5 * y * y * y;
+  return {y, k * a.i, k * a.d};
+}
+__device__ __forceinline__ double sqrt_n(double a) { return sqrt(a); }
+__device__ __forceinline__ D2 sqrt_n(D2 a) {
+  const double y = sqrt(a.v);
+  const double k = 0.5 / y;
+  return {y, k * a.i, k * a.d};
+}
+__device__ __forceinline__ void lift(double v, double ti, double td, double& o) { o = v; }
+__device__ __forceinline__ void lift(double v, double ti, double td, D2& o) { o = {v, ti, td}; }
+
+template <typename N> __device__ __forceinline__ N dot3(const N* a, const N* b) {
+  N s = a[0] * b[0];
+  s = s + a[1] * b[1];
+  return s + a[2] * b[2];
+}
+template <typename N> __device__ __forceinline__ void cross3(const N* a, const N* b, N* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Orbit-plane frame of pixel (px, py) (blackhole_solution_test.cc:167-193):
+// the ray is the line from the camera F = C through the pixel point
+// pv = f axis_x - axis_y (W/2 - px) - axis_z (H/2 - py) (camera.h:55-63, the
+// hole at the origin); yv = N(F - pv), zv = N(pv x yv), xv = yv x zv;
+// b = xv.F, Y0 = yv.F, r0 = |F|.  A point of the orbit at angle phi and
+// radius r is r (cos(phi) yv + sin(phi) xv).
+template <typename N> struct Frame {
+  N yv[3], xv[3], F[3];
+  N b, Y0, r0;
+  bool degenerate;
+};
+
+template <typename N>
+__device__ inline void diff_frame(const DiffScene& s, int px, int py, Frame<N>& f) {
+  const double sx = s.W / 2.0 - px, sy = s.H / 2.0 - py;
+  N pv[3], dv[3], cv[3], zv[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lift(s.fx[k] - s.ay[k] * sx - s.az[k] * sy, s.fx_i[k] - s.ay_i[k] * sx - s.az_i[k] * sy,
+         0.0, pv[k]);
+    lift(s.C[k], s.C_i[k], s.C_d[k], f.F[k]);
+    dv[k] = f.F[k] - pv[k];
+  }
+  const N ny = rsqrt_n(dot3(dv, dv));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) f.yv[k] = dv[k] * ny;
+  cross3(pv, f.yv, cv);
+  const N cc = dot3(cv, cv);
+  f.degenerate = !(val(cc) > 0.0);
+  const N nz = rsqrt_n(f.degenerate ? dot3(dv, dv) : cc);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) zv[k] = cv[k] * nz;
+  cross3(f.yv, zv, f.xv);
+  f.b = dot3(f.xv, f.F);
+  f.Y0 = dot3(f.yv, f.F);
+  f.r0 = sqrt_n(dot3(f.F, f.F));
+}
+
+// Inbound turning point of G(u) = 2M u^3 - u^2 + 1/b^2 for b >= b_c: the
+// closed form u* = (1/(3M)) (1/2 + sin(asin(2q - 1)/3)), q = 27 M^2 / b^2,
+// polished by Newton steps while G' < 0.
+__device__ inline double turning_point(double ib2, double M, double Uc) {
+  const double q = fmin(1.0, 27.0 * M * M * ib2);
+  double sn, cs;
+  sincos_small64(asin(2.0 * q - 1.0) * (1.0 / 3.0), &sn, &cs);
+  double u = Uc * (0.5 + sn);
+#pragma unroll
+  for (int it = 0; it < 3; ++it) {
+    const double g = __builtin_fma(u * u, __builtin_fma(2.0 * M, u, -1.0), ib2);
+    const double gp = u * __builtin_fma(6.0 * M, u, -2.0);
+    if (gp < 0.0) u = u - g / gp;
+  }
+  return u;
+}
+
+// Per-ray march constants in the march precision T.
+template <typename T> struct DRay {
+  T yv[3], xv[3], C[3];
+  T u0, du, h;
+  double phi0;  // the orbit angle is accumulated in FP64 in both precisions
+  T c0, c1, c2, c3;  // G(l + s) = c0 + s (c1 + s (c2 + s c3))
+  bool captured, degenerate;
+};
+
+// FP64 set-up values shared by the march and the adjoint's epilogue.
+struct Setup64 {
+  Frame<double> f;
+  double ib2, l, u0, du, phi0;
+  bool captured;
+};
+
+__device__ inline void diff_setup(const DiffScene& s, int px, int py, Setup64& r) {
+  diff_frame(s, px, py, r.f);
+  const double b = r.f.b;
+  r.ib2 = 1.0 / (b * b);
+  r.captured = b < s.bc;
+  r.l = r.captured ? s.Uc : turning_point(r.ib2, s.M, s.Uc);
+  r.u0 = 1.0 / r.f.r0;
+  r.du = (r.l - r.u0) * (1.0 / s.nstep);
+  // atan (not atan2) of b / Y0, as blackhole_solution_test.cc:193
+  r.phi0 = atan(b / r.f.Y0);
+}
+
+template <typename T>
+__device__ inline void to_ray(const DiffScene& s, const Setup64& r, DRay<T>& d) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    d.yv[k] = (T)r.f.yv[k];
+    d.xv[k] = (T)r.f.xv[k];
+    d.C[k] = (T)s.C[k];
+  }
+  d.u0 = (T)r.u0;
+  d.du = (T)r.du;
+  d.h = d.du * (T)0.5;
+  d.phi0 = r.phi0;
+  const double l = r.l, M = s.M;
+  d.c0 = (T)__builtin_fma(l * l, __builtin_fma(2.0 * M, l, -1.0), r.ib2);
+  d.c1 = (T)(l * __builtin_fma(6.0 * M, l, -2.0));
+  d.c2 = (T)__builtin_fma(6.0 * M, l, -1.0);
+  d.c3 = (T)(2.0 * M);
+  d.captured = r.captured;
+  d.degenerate = r.f.degenerate;
+}
+
+// Step j (1-based) of the reference rule: u_j = u0 + kappa_j du with kappa
+// = j inbound (j < n), n - 0.1 at the final approach (j = n) and n - 0.1 - i
+// on the i-th outbound step; sigma_j = n - kappa_j so that u_j = l - sigma_j du.
+template <typename T> __device__ __forceinline__ T kappa(int j, int n) {
+  return j < n ? (T)j : (T)(2 * n - j) - (T)0.1;
+}
+template <typename T> __device__ __forceinline__ T sigma(int j, int n) {
+  return j < n ? (T)(n - j) : (T)(j - n) + (T)0.1;
+}
+
+template <typename T> __device__ __forceinline__ T g_at(const DRay<T>& d, T sj) {
+  return fma_t(sj, fma_t(sj, fma_t(sj, d.c3, d.c2), d.c1), d.c0);
+}
+
+// sin/cos of an FP64 orbit angle: FP64 reduction, then the series in T
+__device__ __forceinline__ void angle_sincos(double a, float* s, float* c) { sincos_angle(a, s, c); }
+__device__ __forceinline__ void angle_sincos(double a, double* s, double* c) {
+  sincos_small64(a, s, c);
+}
+
+template <typename T>
+__device__ __forceinline__ void orbit_point(const DRay<T>& d, double phi, T r, T* P, T* cph,
+                                            T* sph) {
+  T sn, cs;
+  angle_sincos(phi, &sn, &cs);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) P[k] = r * fma_t(cs, d.yv[k], sn * d.xv[k]);
+  *cph = cs;
+  *sph = sn;
+}
+
+// StaticBlackhole::Collide (blackhole_solution.h:65-88) for the horizon at the
+// origin: the entry point's squared distance from P1 when the chord enters.
+template <typename T>
+__device__ inline bool seg_horizon(const T* P1, const T* P2, T R, T* dh) {
+  const T R2 = R * R;
+  const T d1 = P1[0] * P1[0] + P1[1] * P1[1] + P1[2] * P1[2];
+  const T d2 = P2[0] * P2[0] + P2[1] * P2[1] + P2[2] * P2[2];
+  if (d1 < R2 && d2 < R2) return false;
+  const T Q[3] = {P2[0] - P1[0], P2[1] - P1[1], P2[2] - P1[2]};
+  const T qd = Q[0] * P1[0] + Q[1] * P1[1] + Q[2] * P1[2];
+  const T QQ = Q[0] * Q[0] + Q[1] * Q[1] + Q[2] * Q[2];
+  const T cc = qd * qd - QQ * (d1 - R2);
+  if (!(cc > (T)0)) return false;
+  const T t = (-qd - sqrt_t(cc)) / QQ;
+  if (t <= (T)0 || t >= (T)1) return false;
+  *dh = t * t * QQ;
+  return true;
+}
+
+template <typename T> __device__ __forceinline__ T sstep(T x) {
+  T t = x + (T)0.5;
+  t = t < (T)0 ? (T)0 : (t > (T)1 ? (T)1 : t);
+  return t * t * ((T)3 - (T)2 * t);
+}
+template <typename T> __device__ __forceinline__ T sstep_d(T x) {
+  const T t = x + (T)0.5;
+  return (t > (T)0 && t < (T)1) ? (T)6 * t * ((T)1 - t) : (T)0;
+}
+
+// Soft coverage of the disc at radius rho and its derivatives.
+template <typename T, bool GRAD>
+__device__ inline T coverage(const DiffScene& s, T rho, T* drho, T* drin) {
+  const T w = (T)s.edge;
+  if (!(w > (T)0)) {
+    if (GRAD) *drho = *drin = (T)0;
+    return (rho > (T)s.r_out || rho < (T)s.r_in) ? (T)0 : (T)1;
+  }
+  const T iw = (T)(1.0 / s.edge);
+  const T x1 = (rho - (T)s.r_in) * iw, x2 = ((T)s.r_out - rho) * iw;
+  const T a1 = sstep(x1), a2 = sstep(x2);
+  if (GRAD) {
+    const T g1 = sstep_d(x1) * a2 * iw, g2 = a1 * sstep_d(x2) * iw;
+    *drho = g1 - g2;
+    *drin = -g1;
+  }
+  return a1 * a2;
+}
+
+// Bilinear texel of a disc point q (texel centres at +1/2, clamped indices):
+// U = (half - q_x) cols / (2 half), V = |(q_y - half, q_z)| rows / (2 half),
+// the smooth form of Rectangle::color (vector_object.h:159-179) on the
+// square (+-half, +-half).  With GRAD, dq receives d(colour . cw)/dq.
+template <typename T, bool GRAD>
+__device__ inline void disc_texel(const DiffScene& s, const T* q, T* c, const T* cw, T* dq) {
+  c[0] = c[1] = c[2] = (T)0;
+  if (GRAD) dq[0] = dq[1] = dq[2] = (T)0;
+  if (!s.tex.data) return;
+  const T half = (T)s.half;
+  const T U = (half - q[0]) * (T)s.uscale;
+  const T vy = q[1] - half, vz = q[2];
+  const T rv = sqrt_t(vy * vy + vz * vz);
+  const T V = rv * (T)s.vscale;
+  const T x = U - (T)0.5, y = V - (T)0.5;
+  T fx = floor(x), fy = floor(y);
+  const T tx = x - fx, ty = y - fy;
+  const int cols = s.tex.cols, rows = s.tex.rows;
+  fx = fx > (T)-1 ? (fx < (T)cols ? fx : (T)cols) : (T)-1;  // NaN maps to -1
+  fy = fy > (T)-1 ? (fy < (T)rows ? fy : (T)rows) : (T)-1;
+  const int ix = (int)fx, iy = (int)fy;
+  const int x0 = min(max(ix, 0), cols - 1), x1 = min(max(ix + 1, 0), cols - 1);
+  const int y0 = min(max(iy, 0), rows - 1), y1 = min(max(iy + 1, 0), rows - 1);
+  const uint32_t* t = s.tex.data;
+  const uint32_t t00 = t[(size_t)y0 * cols + x0], t01 = t[(size_t)y0 * cols + x1];
+  const uint32_t t10 = t[(size_t)y1 * cols + x0], t11 = t[(size_t)y1 * cols + x1];
+  T gU = (T)0, gV = (T)0;
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    const T v00 = (T)((t00 >> (8 * ch)) & 0xff), v01 = (T)((t01 >> (8 * ch)) & 0xff);
+    const T v10 = (T)((t10 >> (8 * ch)) & 0xff), v11 = (T)((t11 >> (8 * ch)) & 0xff);
+    const T top = v00 + (v01 - v00) * tx;
+    const T bot = v10 + (v11 - v10) * tx;
+    c[ch] = top + (bot - top) * ty;
+    if (GRAD) {
+      gU += cw[ch] * ((v01 - v00) * ((T)1 - ty) + (v11 - v10) * ty);
+      gV += cw[ch] * (bot - top);
+    }
+  }
+  if (GRAD) {
+    dq[0] = -gU * (T)s.uscale;
+    if (rv > (T)0) {
+      const T k = gV * (T)s.vscale / rv;
+      dq[1] = k * vy;
+      dq[2] = k * vz;
+    }
+  }
+}
+
+// Disc crossing of the chord P1 -> P2 (Annulus::Collide, vector_object.h:
+// 328-347: sign change across z = 0, crossing by linear interpolation on the
+// chord) with the soft radius window; returns the coverage (0: no crossing).
+template <typename T>
+__device__ inline T disc_cross(const DiffScene& s, const T* P1, const T* P2, T* q, T* sfrac,
+                               T* dd) {
+  const T z1 = P1[2], z2 = P2[2];
+  if (!(z1 * z2 < (T)0)) return (T)0;
+  const T sf = z1 / (z1 - z2);
+  T D[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    D[k] = P2[k] - P1[k];
+    q[k] = fma_t(sf, D[k], P1[k]);
+  }
+  const T rho = sqrt_t(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+  T g0, g1;
+  const T al = coverage<T, false>(s, rho, &g0, &g1);
+  *sfrac = sf;
+  *dd = sf * sf * (D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+  return al;
+}
+
+template <typename T> struct Acc {
+  T rgb[3];
+  T tr;  // transmittance
+  int cls, n;
+  bool done, overflow;
+  uint64_t mask;
+};
+
+// Per-lane crossing records in LDS, strided by the block size so that a
+// wave's lanes hit consecutive words.
+template <typename T> struct Rec {
+  double* phi;  // phi_{j-1} at phi[2k kThreads], phi_j at phi[(2k + 1) kThreads]
+  T* tr;        // T_k at tr[k kThreads]
+};
+
+// Pass 1: the forward march.  With RECORD, crossing k stores
+// (phi_{j-1}, phi_j, T_k) in rec.
+template <typename T, bool RECORD>
+__device__ inline void diff_march(const DiffScene& s, const DRay<T>& d, Rec<T> rec, Acc<T>& a) {
+  a.rgb[0] = a.rgb[1] = a.rgb[2] = (T)0;
+  a.tr = (T)1;
+  a.cls = BH_HIT_NONE;
+  a.n = 0;
+  a.done = false;
+  a.overflow = false;
+  a.mask = 0;
+  if (d.degenerate) {  // pixel line through the centre: captured, black
+    a.cls = BH_HIT_HORIZON;
+    a.tr = (T)0;
+    return;
+  }
+  const int n = s.nstep;
+  const T Rh = (T)s.Rh;
+  T P1[3] = {d.C[0], d.C[1], d.C[2]};
+  double phi = d.phi0;
+  T wp = (T)0;
+  for (int j = 1; j <= 2 * n - 1; ++j) {
+    if (j > n && d.captured) {
+      // chord to the centre: the horizon is entered, the disc plane is not
+      // crossed (n.(B - B) = 0)
+      if (a.cls == BH_HIT_NONE) a.cls = BH_HIT_HORIZON;
+      a.tr = (T)0;
+      break;
+    }
+    const T sj = -sigma<T>(j, n) * d.du;
+    const T w = rsq(g_at(d, sj));
+    const double phi_prev = phi;
+    phi += (double)((wp + w) * d.h);
+    wp = w;
+    const T u = fma_t(kappa<T>(j, n), d.du, d.u0);
+    T P2[3], cs, sn;
+    orbit_point(d, phi, rcp(u), P2, &cs, &sn);
+    T dh = (T)0, q[3], sf, dd;
+    const bool hh = seg_horizon(P1, P2, Rh, &dh);
+    const T al = disc_cross(s, P1, P2, q, &sf, &dd);
+    if (al > (T)0 && (!hh || dd < dh)) {
+      T c[3];
+      disc_texel<T, false>(s, q, c, nullptr, nullptr);
+      if (RECORD) {
+        if (a.n < kMaxRec) {
+          rec.phi[(2 * a.n + 0) * kThreads] = phi_prev;
+          rec.phi[(2 * a.n + 1) * kThreads] = phi;
+          rec.tr[a.n * kThreads] = a.tr;
+          a.mask |= 1ull << j;
+        } else {
+          a.overflow = true;
+        }
+      }
+      const T ta = a.tr * al;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.rgb[k] = fma_t(ta, c[k], a.rgb[k]);
+      a.tr = a.tr * ((T)1 - al);
+      if (a.cls == BH_HIT_NONE) a.cls = BH_HIT_DISC;
+      ++a.n;
+      if (!(a.tr > (T)0)) break;
+    }
+    if (hh) {
+      if (a.cls == BH_HIT_NONE) a.cls = BH_HIT_HORIZON;
+      a.tr = (T)0;
+      break;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P1[k] = P2[k];
+  }
+}
+
+// Adjoint of P = r (cos(phi) yv + sin(phi) xv), r = 1/u.
+template <typename T>
+__device__ __forceinline__ void point_adjoint(const DRay<T>& d, const T* Pb, T cs, T sn, T r,
+                                              T* yvb, T* xvb, T* phib, T* ub) {
+  T py = (T)0, px = (T)0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    yvb[k] = fma_t(Pb[k], r * cs, yvb[k]);
+    xvb[k] = fma_t(Pb[k], r * sn, xvb[k]);
+    py = fma_t(Pb[k], d.yv[k], py);
+    px = fma_t(Pb[k], d.xv[k], px);
+  }
+  *phib = r * (cs * px - sn * py);
+  *ub = -(cs * py + sn * px) * r * r;
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <typename T, bool ADJ>
+__global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float* __restrict__ grad_rgb,
+                                                        float* __restrict__ rgb_out,
+                                                        uint8_t* __restrict__ hit_out,
+                                                        double* __restrict__ partial) {
+  __shared__ double rec_phi[ADJ ? 2 * kMaxRec * kThreads : 1];
+  __shared__ T rec_tr[ADJ ? kMaxRec * kThreads : 1];
+  __shared__ double red[ADJ ? kPartial * (kThreads / 64) : 1];
+  const int tid = threadIdx.x;
+  const int px = blockIdx.x * kTile + (tid & (kTile - 1));
+  const int py = blockIdx.y * kTile + (tid / kTile);
+  const bool inside = px < s.W && py < s.H;
+  double g_rs = 0, g_in = 0, g_di = 0, g_ri = 0, g_ov = 0;
+  if (inside) {
+    const size_t idx = (size_t)py * s.W + px;
+    Setup64 su;
+    diff_setup(s, px, py, su);
+    DRay<T> d;
+    to_ray(s, su, d);
+    Acc<T> a;
+    const Rec<T> rec{rec_phi + tid, rec_tr + tid};
+    diff_march<T, ADJ>(s, d, rec, a);
+    if (rgb_out) {
+      rgb_out[3 * idx + 0] = (float)a.rgb[0];
+      rgb_out[3 * idx + 1] = (float)a.rgb[1];
+      rgb_out[3 * idx + 2] = (float)a.rgb[2];
+    }
+    if (hit_out) hit_out[idx] = (uint8_t)a.cls;
+    if (ADJ && a.mask) {
+      g_ov = a.overflow ? 1.0 : 0.0;
+      const T gb[3] = {(T)grad_rgb[3 * idx], (T)grad_rgb[3 * idx + 1], (T)grad_rgb[3 * idx + 2]};
+      const int n = s.nstep;
+      T B[3] = {(T)0, (T)0, (T)0};
+      T yvb[3] = {(T)0, (T)0, (T)0}, xvb[3] = {(T)0, (T)0, (T)0}, Cb[3] = {(T)0, (T)0, (T)0};
+      T A_next = (T)0, pend = (T)0, hb = (T)0, u0b = (T)0, dub = (T)0, Mb = (T)0, ib2b = (T)0,
+        rinb = (T)0;
+      int k = a.n < kMaxRec ? a.n - 1 : kMaxRec - 1;
+      const int jmax = 63 - __builtin_clzll(a.mask);
+      for (int j = jmax; j >= 1; --j) {
+        const T kj = kappa<T>(j, n);
+        const T uj = fma_t(kj, d.du, d.u0);
+        T Aj = A_next + pend;
+        pend = (T)0;
+        T ubj = (T)0;
+        if ((a.mask >> j) & 1) {
+          const double ph_a = rec.phi[(2 * k + 0) * kThreads];
+          const double ph_b = rec.phi[(2 * k + 1) * kThreads];
+          const T Tk = rec.tr[k * kThreads];
+          --k;
+          T P1[3], P2[3], ca = (T)0, sa = (T)0, ra = (T)0, cb, sb;
+          if (j == 1) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) P1[m] = d.C[m];
+          } else {
+            ra = rcp(fma_t(kappa<T>(j - 1, n), d.du, d.u0));
+            orbit_point(d, ph_a, ra, P1, &ca, &sa);
+          }
+          const T rb = rcp(uj);
+          orbit_point(d, ph_b, rb, P2, &cb, &sb);
+          T q[3], sf, dd;
+          const T z1 = P1[2], z2 = P2[2];
+          disc_cross(s, P1, P2, q, &sf, &dd);
+          const T rho = sqrt_t(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+          T da_rho, da_rin;
+          const T al = coverage<T, true>(s, rho, &da_rho, &da_rin);
+          T c[3], cw[3], qb[3];
+#pragma unroll
+          for (int m = 0; m < 3; ++m) cw[m] = gb[m] * Tk * al;
+          disc_texel<T, true>(s, q, c, cw, qb);
+          // compositing: rgb = F + T_k al c + T_k (1 - al) B
+          T alb = (T)0;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            alb = fma_t(gb[m], Tk * (c[m] - B[m]), alb);
+            B[m] = fma_t(al, c[m] - B[m], B[m]);
+          }
+          rinb = fma_t(alb, da_rin, rinb);
+          const T rhob = alb * da_rho;
+          if (rho > (T)0) {
+            const T kr = rhob / rho;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) qb[m] = fma_t(kr, q[m], qb[m]);
+          }
+          // q = P1 + s (P2 - P1), s = z1 / (z1 - z2)
+          T sb_ = (T)0, P1b[3], P2b[3];
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            sb_ = fma_t(qb[m], P2[m] - P1[m], sb_);
+            P1b[m] = ((T)1 - sf) * qb[m];
+            P2b[m] = sf * qb[m];
+          }
+          const T den = z1 - z2;
+          const T kd = sb_ / (den * den);
+          P1b[2] = fma_t(-kd, z2, P1b[2]);
+          P2b[2] = fma_t(kd, z1, P2b[2]);
+          T phb, ub;
+          point_adjoint(d, P2b, cb, sb, rb, yvb, xvb, &phb, &ub);
+          Aj += phb;
+          ubj += ub;
+          if (j == 1) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) Cb[m] += P1b[m];
+          } else {
+            point_adjoint(d, P1b, ca, sa, ra, yvb, xvb, &phb, &ub);
+            pend = phb;
+            u0b += ub;
+            dub = fma_t(kappa<T>(j - 1, n), ub, dub);
+          }
+        }
+        // phi_m = phi0 + h sum_{i <= m} (w_{i-1} + w_i): w_j enters phi_m for
+        // m >= j twice over (increments j and j + 1)
+        const T sj = -sigma<T>(j, n) * d.du;
+        const T G = g_at(d, sj);
+        const T w = rsq(G);
+        const T A2 = Aj + A_next;
+        hb = fma_t(w, A2, hb);
+        const T Gb = (T)-0.5 * w * w * w * (d.h * A2);
+        const T Gu = fma_t(sj, fma_t((T)3 * d.c3, sj, (T)2 * d.c2), d.c1);
+        ubj = fma_t(Gb, Gu, ubj);
+        Mb = fma_t(Gb, (T)2 * uj * uj * uj, Mb);
+        ib2b += Gb;
+        u0b += ubj;
+        dub = fma_t(kj, ubj, dub);
+        A_next = Aj;
+      }
+      // epilogue in FP64: h = du/2, du = (l - u0)/n, l(M, 1/b^2), set-up
+      const double phi0b = (double)(A_next + pend);
+      double dub64 = (double)dub + 0.5 * (double)hb;
+      const double lb = dub64 / n;
+      double u0b64 = (double)u0b - lb;
+      double Mb64 = (double)Mb, ib2b64 = (double)ib2b;
+      const double M = s.M, l = su.l;
+      if (su.captured) {
+        Mb64 += lb * (-1.0 / (3.0 * M * M));
+      } else {
+        const double Gu = l * __builtin_fma(6.0 * M, l, -2.0);
+        Mb64 += lb * (-2.0 * l * l * l / Gu);
+        ib2b64 += lb * (-1.0 / Gu);
+      }
+      const double b = su.f.b, Y0 = su.f.Y0, r0 = su.f.r0;
+      const double den = b * b + Y0 * Y0;
+      const double bb = ib2b64 * (-2.0 / (b * b * b)) + phi0b * (Y0 / den);
+      const double Y0b = -phi0b * (b / den);
+      const double r0b = -u0b64 / (r0 * r0);
+      Frame<D2> fd;
+      diff_frame(s, px, py, fd);
+      double gi = bb * fd.b.i + Y0b * fd.Y0.i + r0b * fd.r0.i;
+      double gd = bb * fd.b.d + Y0b * fd.Y0.d + r0b * fd.r0.d;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        gi += (double)yvb[m] * fd.yv[m].i + (double)xvb[m] * fd.xv[m].i + (double)Cb[m] * s.C_i[m];
+        gd += (double)yvb[m] * fd.yv[m].d + (double)xvb[m] * fd.xv[m].d + (double)Cb[m] * s.C_d[m];
+      }
+      g_rs = 0.5 * Mb64;  // M = r_s / 2
+      g_in = gi;
+      g_di = gd;
+      g_ri = (double)rinb;
+    }
+  }
+  if (ADJ) {
+    double v[kPartial] = {g_rs, g_in, g_di, g_ri, g_ov};
+    const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int m = 0; m < kPartial; ++m) {
+      v[m] = wave_sum(v[m]);
+      if (lane == 0) red[m * (kThreads / 64) + wave] = v[m];
+    }
+    __syncthreads();
+    if (tid < kPartial) {
+      double t = 0;
+      for (int w = 0; w < kThreads / 64; ++w) t += red[tid * (kThreads / 64) + w];
+      partial[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPartial + tid] = t;
+    }
+  }
+}
+
+// Fixed-order sum of the per-block partials: thread t adds blocks t, t + 256,
+// ... then a tree over the 256 threads.
+__global__ __launch_bounds__(256) void diff_reduce_kernel(const double* __restrict__ partial,
+                                                          int nblocks, double* __restrict__ grad,
+                                                          uint32_t* __restrict__ overflow) {
+  __shared__ double sm[kPartial][256];
+  const int t = threadIdx.x;
+  double acc[kPartial] = {0, 0, 0, 0, 0};
+  for (int b = t; b < nblocks; b += 256)
+#pragma unroll
+    for (int m = 0; m < kPartial; ++m) acc[m] += partial[(size_t)b * kPartial + m];
+#pragma unroll
+  for (int m = 0; m < kPartial; ++m) sm[m][t] = acc[m];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int m = 0; m < kPartial; ++m) sm[m][t] += sm[m][t + w];
+    __syncthreads();
+  }
+  if (t < 4) grad[t] = sm[t][0];
+  if (t == 4 && overflow) overflow[0] = (uint32_t)sm[4][0];
+}
+
+}  // namespace
+
+int diff_partial_blocks(int W, int H) {
+  return ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
+}
+
+hipError_t launch_diff_forward(const DiffScene& s, bool fp64, float* rgb, uint8_t* hit,
+                               hipStream_t stream) {
+  const dim3 grid((s.W + kTile - 1) / kTile, (s.H + kTile - 1) / kTile);
+  if (fp64)
+    hipLaunchKernelGGL((diff_kernel<double, false>), grid, dim3(kThreads), 0, stream, s, nullptr,
+                       rgb, hit, nullptr);
+  else
+    hipLaunchKernelGGL((diff_kernel<float, false>), grid, dim3(kThreads), 0, stream, s, nullptr,
+                       rgb, hit, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_adjoint(const DiffScene& s, bool fp64, const float* grad_rgb, float* rgb,
+                               uint8_t* hit, double* partial, double* grad_out,
+                               uint32_t* overflow_out, hipStream_t stream) {
+  const dim3 grid((s.W + kTile - 1) / kTile, (s.H + kTile - 1) / kTile);
+  if (fp64)
+    hipLaunchKernelGGL((diff_kernel<double, true>), grid, dim3(kThreads), 0, stream, s, grad_rgb,
+                       rgb, hit, partial);
+  else
+    hipLaunchKernelGGL((diff_kernel<float, true>), grid, dim3(kThreads), 0, stream, s, grad_rgb,
+                       rgb, hit, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(diff_reduce_kernel, dim3(1), dim3(256), 0, stream, partial,
+                     (int)(grid.x * grid.y), grad_out, overflow_out);
+  return hipGetLastError();
+}
+
+}  // namespace bh

@@ -91,4 +91,32 @@ hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue&
                        hipStream_t stream, bool escape);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
 
+// ---------------------------------------------------------------------------
+// Differentiable hole + disc scene (bh_diff_params), resolved on the host.
+// Passed by value as a kernel argument.  Derivatives are with respect to the
+// inclination (suffix _i) and the camera distance (suffix _d); the camera
+// axes do not depend on the distance.
+// ---------------------------------------------------------------------------
+struct DiffScene {
+  double C[3], C_i[3], C_d[3];        // camera position
+  double fx[3], fx_i[3];              // axis_x * focal length (focus vector)
+  double ay[3], ay_i[3];              // axis_y
+  double az[3], az_i[3];              // axis_z
+  double M, bc, Rh, Uc;               // mass, 3 sqrt(3) M, 2M, 1/(3M)
+  double r_in, r_out, edge;           // disc radii and soft-edge width
+  double half, uscale, vscale;        // texture square half side; cols, rows / (2 half)
+  DevTex tex;                         // disc texture (data null: black disc)
+  int32_t W, H, nstep, pad0;
+};
+
+// Forward: rgb (H*W*3 float) and hit (H*W u8) may be null.
+hipError_t launch_diff_forward(const DiffScene& s, bool fp64, float* rgb, uint8_t* hit,
+                               hipStream_t stream);
+// Forward + adjoint.  partial holds 5 doubles per 256-pixel block (returned by
+// diff_partial_blocks); grad_out receives 4 doubles, overflow_out one uint32.
+int diff_partial_blocks(int W, int H);
+hipError_t launch_diff_adjoint(const DiffScene& s, bool fp64, const float* grad_rgb,
+                               float* rgb, uint8_t* hit, double* partial, double* grad_out,
+                               uint32_t* overflow_out, hipStream_t stream);
+
 }  // namespace bh

@@ -217,6 +217,23 @@ def config2(width: int = 1280, height: int = 720, precision=abi.PREC_FP32,
     return Preset("config2", s, o, tex)
 
 
+def config4_plain(theta, width: int = 512, height: int = 512, fov: float = math.radians(70.0),
+                  r_outer: float = 12.0, azim: float = 0.0,
+                  precision=abi.PREC_FP32) -> Preset:
+    """The inverse-fit scene at theta = (r_s, incl, dist, r_inner) for the
+    plain march (reference rule: 20-step bisection, nearest texel, hard disc
+    edge): the same hole, disc and look-at camera as the differentiable
+    render of those parameters."""
+    r_s, incl, dist, r_in = (float(v) for v in theta)
+    cam = orbit_camera(dist, math.degrees(incl), math.degrees(azim), width, height, fov)
+    s = _scene(cam, mass=r_s / 2.0)
+    s.has_disc = 1
+    s.disc = annulus((r_outer, r_outer, 0.0), (-r_outer, r_outer, 0.0), (-r_outer, -r_outer, 0.0),
+                     (r_outer, -r_outer, 0.0), r_outer, r_in, SLOT_DISC)
+    o = options(precision=precision, escape=False, band=1e-3)
+    return Preset("config4_plain", s, o, {SLOT_DISC: gen_disc_texture()})
+
+
 def shipped(width: int = 960, height: int = 540, precision=abi.PREC_FP64) -> Preset:
     cam = camera(width, height, math.pi / 2, position=(-2000.0, 0.0, 400.0))
     s = _scene(cam, mass=10.0)

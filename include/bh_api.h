@@ -146,6 +146,24 @@ typedef struct bh_outputs {
   int32_t* steps;     /* H*W integration steps taken                         */
 } bh_outputs;
 
+/* ---- differentiable hole + disc scene --------------------------------------- */
+#define BH_DIFF_MAX_CROSSINGS 8
+/* Parameters of the inverse fit: the hole (r_s = 2M) at the origin, a thin
+ * disc in z = 0 centred on it with the texture square (+-r_out, +-r_out) and
+ * normal (0,0,-1), and a camera at distance dist, inclination incl from +z and
+ * azimuth azim, looking at the hole with +z up.  The render is the reference
+ * march (nstep 20) with the exact turning point, bilinear texels and a soft
+ * disc edge: coverage alpha = S((rho - r_in)/w) S((r_out - rho)/w),
+ * S(x) = smoothstep of clamp(x + 1/2, 0, 1), composited front to back over the
+ * disc crossings of a ray until the horizon or full coverage. */
+typedef struct bh_diff_params {
+  double r_s, incl, dist, r_inner; /* the four fitted parameters */
+  double r_outer, azim, fov, edge_width;
+  int32_t width, height;
+  int32_t texture; /* disc texture slot */
+  int32_t pad_;
+} bh_diff_params;
+
 /* ---- flat-space scene (ray_tracer_test path, ray_tracer.h:52-96) -------- */
 typedef struct bh_rect {
   double vertex[5][3]; /* [0] pivot, [1..4] = p1..p4 (vector_object.h:94-98) */
@@ -234,6 +252,31 @@ void bh_path_destroy(bh_path* path);
 int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                    int iters, double* avg_ms_main, double* avg_ms_total,
                    int64_t* band_rays);
+
+/* ======================================================================== */
+/* Differentiable render and its adjoint (no reference counterpart: the     */
+/* reference renders only; these serve the inverse fit of the four          */
+/* parameters of bh_diff_params)                                            */
+/* ======================================================================== */
+/* Forward: rgb_out = H*W*3 float colour in [0, 255], hit_out = H*W class
+ * (either may be NULL).  precision BH_PREC_FP32 or BH_PREC_FP64.  Device
+ * pointers, asynchronous on the context stream. */
+int bh_diff_render_device(bh_ctx* ctx, const bh_diff_params* p, int precision,
+                          float* rgb_out, uint8_t* hit_out);
+/* Forward and adjoint in one launch: given dL/drgb (H*W*3 float), writes
+ * dL/d(r_s, incl, dist, r_inner) to grad_out[4] (4 doubles).  rgb_out and
+ * hit_out (optional) receive the forward render.  A ray records at most
+ * BH_DIFF_MAX_CROSSINGS disc crossings for its adjoint; overflow_out
+ * (optional, one uint32) receives the number of rays that had more, whose
+ * later crossings are left out of the gradient.  All pointers are device
+ * pointers; asynchronous on the context stream. */
+int bh_diff_adjoint_device(bh_ctx* ctx, const bh_diff_params* p, int precision,
+                           const float* grad_rgb, float* rgb_out, uint8_t* hit_out,
+                           double* grad_out, uint32_t* overflow_out);
+/* Average ms of the forward kernel and of the forward+adjoint kernel over
+ * iters back-to-back launches (HIP events on the context stream). */
+int bh_diff_time(bh_ctx* ctx, const bh_diff_params* p, int precision, int iters,
+                 double* fwd_ms, double* fwd_adj_ms);
 
 /* ======================================================================== */
 /* Flat-space march — RayTracer::Prograde (ray_tracer.h:52-96)              */

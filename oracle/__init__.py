@@ -63,6 +63,10 @@ def load():
         "or_inbound_sweep": (D, [D, D, D, C.c_int, C.c_int]),
         "or_render_flat": (C.c_int, [C.POINTER(abi.FlatScene), C.POINTER(abi.Image), C.c_int,
                                      C.POINTER(abi.Outputs)]),
+        "or_render_diff": (C.c_int, [C.POINTER(abi.DiffParams), C.POINTER(abi.Image), C.c_int,
+                                     C.c_int, P, P, P]),
+        "or_diff_scene": (None, [C.POINTER(abi.DiffParams), C.POINTER(abi.Scene),
+                                 C.POINTER(abi.Options)]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(lib, n)
@@ -142,3 +146,28 @@ def render_flat(scene, textures: dict, want=("rgb", "hit")):
     tex, n, keep = _textures(textures)
     lib.or_render_flat(C.byref(scene), tex, n, C.byref(o))
     return arrs
+
+
+def render_diff(params, textures: dict, threads: int = 1):
+    """Differentiable-mode render of abi.DiffParams: (rgb H*W*3 float64 in
+    [0, 255], hit class u8, structure signature int32)."""
+    lib = load()
+    w, h = params.width, params.height
+    rgb = np.zeros((h, w, 3), np.float64)
+    hit = np.zeros((h, w), np.uint8)
+    sig = np.zeros((h, w), np.int32)
+    tex, n, keep = _textures(textures)
+    rc = lib.or_render_diff(C.byref(params), tex, n, threads, rgb.ctypes.data, hit.ctypes.data,
+                            sig.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle diff render failed: {rc}")
+    del keep
+    return rgb, hit, sig
+
+
+def diff_scene(params):
+    """The reference-march scene and options the differentiable mode renders."""
+    lib = load()
+    s, o = abi.Scene(), abi.Options()
+    lib.or_diff_scene(C.byref(params), C.byref(s), C.byref(o))
+    return s, o

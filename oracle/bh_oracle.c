@@ -611,6 +611,268 @@ double or_inbound_sweep(double b, double M, double u0, int nstep, int root_mode)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Differentiable mode (extension)                                           */
+/* ------------------------------------------------------------------------ */
+void or_diff_scene(const bh_diff_params* p, bh_scene* s, bh_options* o) {
+  memset(s, 0, sizeof *s);
+  memset(o, 0, sizeof *o);
+  double si = sin(p->incl), ci = cos(p->incl);
+  v3 C = mk(p->dist * si * cos(p->azim), p->dist * si * sin(p->azim), p->dist * ci);
+  /* look-at the origin with +z up: x toward the hole, z = -(up orthogonal), y = z x x */
+  v3 x = normalize(scl(C, -1.0));
+  v3 up = mk(0, 0, 1);
+  double k = dot(up, x);
+  v3 z = normalize(scl(sub(up, scl(x, k)), -1.0));
+  v3 y = normalize(cross(z, x));
+  memcpy(s->camera.position, C.v, sizeof C.v);
+  memcpy(s->camera.axis_x, x.v, sizeof x.v);
+  memcpy(s->camera.axis_y, y.v, sizeof y.v);
+  memcpy(s->camera.axis_z, z.v, sizeof z.v);
+  s->camera.width = p->width;
+  s->camera.height = p->height;
+  s->camera.fov = p->fov;
+  s->hole.mass = p->r_s / 2.0;
+  s->has_disc = 1;
+  double R = p->r_outer;
+  double P[5][3] = {{0, 0, 0}, {R, R, 0}, {-R, R, 0}, {-R, -R, 0}, {R, -R, 0}};
+  memcpy(s->disc.vertex, P, sizeof P);
+  v3 n = normalize(cross(sub(ld(P[3]), ld(P[1])), sub(ld(P[2]), ld(P[1]))));
+  memcpy(s->disc.normal, n.v, sizeof n.v);
+  s->disc.r_outer = p->r_outer;
+  s->disc.r_inner = p->r_inner;
+  s->disc.texture = p->texture;
+  s->background = -1;
+  o->precision = BH_PREC_FP64;
+  o->nstep = 20;
+  o->root = BH_ROOT_EXACT;
+  o->filter = BH_FILTER_BILINEAR;
+  o->edge_width = p->edge_width;
+}
+
+/* smoothstep of clamp(x + 1/2, 0, 1) */
+static double soft_step(double x) {
+  double t = x + 0.5;
+  t = t < 0 ? 0 : (t > 1 ? 1 : t);
+  return t * t * (3.0 - 2.0 * t);
+}
+
+static double soft_alpha(const bh_annulus* a, double rho, double w) {
+  if (w <= 0) return (rho > a->r_outer || rho < a->r_inner) ? 0.0 : 1.0;
+  return soft_step((rho - a->r_inner) / w) * soft_step((a->r_outer - rho) / w);
+}
+
+/* bilinear sample (texel centres at +0.5, clamped), channels as doubles;
+ * *cell receives the clamped top-left texel index (col + cols * row) */
+static void bilinear_d(const bh_image* img, double a, double b, double out[3], int* cell) {
+  double x = a - 0.5, y = b - 0.5;
+  double fx = floor(x), fy = floor(y);
+  double tx = x - fx, ty = y - fy;
+  int x0 = (int)fx, y0 = (int)fy, x1 = x0 + 1, y1 = y0 + 1;
+  int W = img->cols, H = img->rows, C = img->channels;
+  x0 = x0 < 0 ? 0 : (x0 > W - 1 ? W - 1 : x0);
+  x1 = x1 < 0 ? 0 : (x1 > W - 1 ? W - 1 : x1);
+  y0 = y0 < 0 ? 0 : (y0 > H - 1 ? H - 1 : y0);
+  y1 = y1 < 0 ? 0 : (y1 > H - 1 ? H - 1 : y1);
+  for (int ch = 0; ch < 3; ++ch) {
+    double v00 = img->data[((size_t)y0 * W + x0) * C + ch];
+    double v01 = img->data[((size_t)y0 * W + x1) * C + ch];
+    double v10 = img->data[((size_t)y1 * W + x0) * C + ch];
+    double v11 = img->data[((size_t)y1 * W + x1) * C + ch];
+    double top = v00 + (v01 - v00) * tx;
+    double bot = v10 + (v11 - v10) * tx;
+    out[ch] = top + (bot - top) * ty;
+  }
+  *cell = x0 + W * y0;
+}
+
+/* FNV-1a style fold of one value into a structure signature */
+static int32_t sig_mix(int32_t h, int32_t v) {
+  return (int32_t)(((uint32_t)h ^ (uint32_t)v) * 16777619u);
+}
+
+/* continuous texel coordinates of a disc point: r cos(theta) = s1.v/|s1|,
+ * r sin(theta) = |s1 x v|/|s1| (the smooth form of Rectangle::color) */
+static void disc_coords(const bh_annulus* a, const bh_image* img, v3 q, double* U, double* V) {
+  v3 v1 = ld(a->vertex[1]);
+  v3 s1 = sub(ld(a->vertex[2]), v1);
+  v3 s2 = sub(ld(a->vertex[4]), v1);
+  v3 v = sub(q, v1);
+  double l1 = sqrt(dot(s1, s1)), l2 = sqrt(dot(s2, s2));
+  v3 c = cross(s1, v);
+  *U = dot(s1, v) / (l1 * l1) * img->cols;
+  *V = sqrt(dot(c, c)) / (l1 * l2) * img->rows;
+}
+
+typedef struct diff_acc {
+  double T;        /* transmittance */
+  double rgb[3];
+  int n, first_cls;
+  int32_t sig;
+  int done;
+} diff_acc;
+
+static void diff_disc(const bh_scene* s, const bh_image* img, double w, v3 q, int step,
+                      diff_acc* acc) {
+  double rho = sqrt(dot(q, q));
+  double al = soft_alpha(&s->disc, rho, w);
+  if (al <= 0) return;
+  double c[3] = {0, 0, 0};
+  int cell = -1;
+  if (img) {
+    double U, V;
+    disc_coords(&s->disc, img, q, &U, &V);
+    bilinear_d(img, U, V, c, &cell);
+  }
+  for (int k = 0; k < 3; ++k) acc->rgb[k] += acc->T * al * c[k];
+  acc->T *= 1.0 - al;
+  if (!acc->first_cls) acc->first_cls = BH_HIT_DISC;
+  acc->sig = sig_mix(sig_mix(acc->sig, step + 1), cell);
+  acc->n++;
+  if (acc->T <= 0) acc->done = 1;
+}
+
+/* One chord of the differentiable march: disc crossing (soft window) and
+ * horizon, nearer first. */
+static void diff_chord(const bh_scene* s, const bh_image* img, double w, v3 p1, v3 p2, int step,
+                       diff_acc* acc) {
+  v3 B = ld(s->hole.position);
+  v3 qh;
+  int hh = seg_sphere(p1, p2, B, 2 * s->hole.mass, &qh);
+  double dh = 0;
+  if (hh) {
+    v3 d = sub(p1, qh);
+    dh = dot(d, d);
+  }
+  /* disc crossing without the radius window */
+  v3 c0 = ld(s->disc.vertex[0]);
+  v3 n = ld(s->disc.normal);
+  v3 a1 = sub(p1, c0), a2 = sub(p2, c0);
+  int hd = 0;
+  v3 qd;
+  double dd = 0;
+  if (dot(n, a1) * dot(n, a2) < 0) {
+    double t1 = dot(n, a1), t2 = dot(n, a2);
+    v3 c = scl(add(scl(a1, fabs(t2)), scl(a2, fabs(t1))), 1. / (fabs(t1) + fabs(t2)));
+    if (soft_alpha(&s->disc, sqrt(dot(c, c)), w) > 0) {
+      hd = 1;
+      qd = add(c, c0);
+      v3 d = sub(p1, qd);
+      dd = dot(d, d);
+    }
+  }
+  if (hd && (!hh || dd < dh)) {
+    diff_disc(s, img, w, qd, step, acc);
+    if (acc->done) return;
+  }
+  if (hh) {
+    if (!acc->first_cls) acc->first_cls = BH_HIT_HORIZON;
+    acc->T = 0;
+    acc->done = 1;
+    return;
+  }
+}
+
+static void diff_pixel(const bh_scene* s, const bh_options* o, const bh_image* img, double w,
+                       int x, int y, double* rgb, uint8_t* hit, int32_t* sig) {
+  or_ray ray;
+  or_ray_setup(s, o, x, y, &ray);
+  diff_acc acc = {1.0, {0, 0, 0}, 0, 0, (int32_t)2166136261u, 0};
+  int steps = 0;
+  if (ray.degenerate) {
+    acc.first_cls = BH_HIT_HORIZON;
+    goto out;
+  }
+  {
+    int nstep = o->nstep;
+    double M = s->hole.mass;
+    v3 B = ld(s->hole.position);
+    double u = ray.u0, wp = 0, phi = ray.phi0;
+    v3 Pp = ld(s->camera.position);
+#define DSTEP()                                                   \
+    do {                                                          \
+      double wn = 1.0 / sqrt(or_G(u, ray.b, M));                  \
+      phi += (wp + wn) * ray.du_h;                                \
+      double r = 1. / u;                                          \
+      v3 P = add(matvec(ray.R, mk(0, r * cos(phi), r * sin(phi))), B); \
+      wp = wn;                                                    \
+      ++steps;                                                    \
+      diff_chord(s, img, w, Pp, P, steps, &acc);                  \
+      Pp = P;                                                     \
+    } while (0)
+    for (int i = 0; i < nstep - 1 && !acc.done; ++i) {
+      u += ray.du;
+      DSTEP();
+    }
+    if (!acc.done) {
+      u += ray.du * 0.9;
+      DSTEP();
+    }
+    if (!acc.done && ray.captured) {
+      /* chord to the centre: the disc sees n.(B - B) = 0, the horizon is entered */
+      if (!acc.first_cls) acc.first_cls = BH_HIT_HORIZON;
+      acc.T = 0;
+      acc.done = 1;
+    }
+    for (int i = 0; i < nstep - 1 && !acc.done; ++i) {
+      u -= ray.du;
+      DSTEP();
+    }
+#undef DSTEP
+  }
+out:
+  if (rgb)
+    for (int k = 0; k < 3; ++k) rgb[k] = acc.rgb[k];
+  if (hit) *hit = (uint8_t)acc.first_cls;
+  if (sig) *sig = sig_mix(sig_mix(acc.sig, steps), acc.first_cls);
+}
+
+typedef struct diff_job {
+  const bh_scene* s;
+  const bh_options* o;
+  const bh_image* img;
+  double w;
+  int tid, nthreads;
+  double* rgb;
+  uint8_t* hit;
+  int32_t* sig;
+} diff_job;
+
+static void* diff_rows(void* arg) {
+  diff_job* j = (diff_job*)arg;
+  int W = j->s->camera.width, H = j->s->camera.height;
+  for (int y = j->tid; y < H; y += j->nthreads)
+    for (int x = 0; x < W; ++x) {
+      size_t idx = (size_t)y * W + x;
+      diff_pixel(j->s, j->o, j->img, j->w, x, y, j->rgb ? j->rgb + 3 * idx : NULL,
+                 j->hit ? j->hit + idx : NULL, j->sig ? j->sig + idx : NULL);
+    }
+  return NULL;
+}
+
+int or_render_diff(const bh_diff_params* p, const bh_image* tex, int ntex, int threads,
+                   double* rgb, uint8_t* hit, int32_t* sig) {
+  if (!p || p->width <= 0 || p->height <= 0) return BH_ERR_INVALID_ARG;
+  bh_scene s;
+  bh_options o;
+  or_diff_scene(p, &s, &o);
+  const bh_image* img = tex_at(tex, ntex, p->texture);
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  pthread_t th[256];
+  diff_job jobs[256];
+  for (int t = 0; t < threads; ++t) {
+    jobs[t] = (diff_job){&s, &o, img, p->edge_width, t, threads, rgb, hit, sig};
+    if (threads == 1)
+      diff_rows(&jobs[t]);
+    else if (pthread_create(&th[t], NULL, diff_rows, &jobs[t]) != 0)
+      return BH_ERR_INVALID_ARG;
+  }
+  if (threads > 1)
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Flat march: RayTracer::Prograde (ray_tracer.h:17-96)                      */
 /* ------------------------------------------------------------------------ */
 /* Rectangle::Collide (vector_object.h:107-127) */

@@ -68,6 +68,17 @@ int or_render_rows(const bh_scene* s, const bh_options* o, const bh_image* textu
  * phi0, for convergence checks against the exact integral. */
 double or_inbound_sweep(double b, double M, double u0, int nstep, int root_mode);
 
+/* Differentiable mode (no reference counterpart): the march of
+ * bh_diff_params with the exact turning point, bilinear texels and soft disc
+ * edges composited front to back.  rgb: H*W*3 doubles in [0, 255]; hit:
+ * class of the first contribution; sig: per-pixel hash of the discrete
+ * structure (step and bilinear texel cell of every contribution, the
+ * terminating step and the class), for masking finite differences.  Any output may be NULL. */
+int or_render_diff(const bh_diff_params* p, const bh_image* textures, int n_textures,
+                   int threads, double* rgb, uint8_t* hit, int32_t* sig);
+/* The scene and options the differentiable mode renders (reference march). */
+void or_diff_scene(const bh_diff_params* p, bh_scene* s, bh_options* o);
+
 /* Flat march (ray_tracer.h:52-96). */
 int or_render_flat(const bh_flat_scene* s, const bh_image* textures, int n_textures,
                    const bh_outputs* out);

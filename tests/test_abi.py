@@ -40,7 +40,7 @@ STRUCTS = {
     "bh_image": abi.Image, "bh_camera": abi.Camera, "bh_hole": abi.Hole,
     "bh_annulus": abi.Annulus, "bh_sphere": abi.Sphere, "bh_scene": abi.Scene,
     "bh_options": abi.Options, "bh_outputs": abi.Outputs, "bh_rect": abi.Rect,
-    "bh_plane": abi.Plane, "bh_flat_scene": abi.FlatScene,
+    "bh_plane": abi.Plane, "bh_flat_scene": abi.FlatScene, "bh_diff_params": abi.DiffParams,
 }
 
 
