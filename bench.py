@@ -82,6 +82,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time the 512x512 forward / forward+adjoint and exit")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times), for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -97,6 +99,10 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n, "copies": 3}))
+        return 0
+
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_timing()}))
         return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

@@ -12,19 +12,23 @@
 // or full coverage.  One lane per ray.
 //
 // Adjoint (reverse mode, fused with the forward in one kernel):
-//   pass 1   forward march; each disc crossing records (phi_{j-1}, phi_j, T_k)
+//   pass 1   forward march; each disc crossing records (psi_{j-1}, psi_j, T_k)
 //            in LDS and sets bit j of a chord mask;
 //   pass 2   reverse over chords j = j_max .. 1.  The trajectory is not
 //            stored: the ray set-up is the one checkpoint, u_j and w_j are
-//            recomputed in closed form, and phi at a crossing comes from its
+//            recomputed in closed form, and psi at a crossing comes from its
 //            record.  Compositing adjoint with the behind colour B built back
 //            to front; crossing point, coverage and bilinear texel adjoints
-//            give P_bar, hence (phi_bar, u_bar, yv_bar, xv_bar); suffix sums
-//            A_j of phi_bar give w_bar_j = h (A_j + A_{j+1}) and
+//            give P_bar, hence (psi_bar, u_bar, e1_bar, e2_bar); suffix sums
+//            A_j of psi_bar give w_bar_j = h (A_j + A_{j+1}) and
 //            G_bar = -w^3 w_bar / 2, hence (u, M, 1/b^2);
 //   set-up   turning point by implicit differentiation of G(l) = 0, then the
 //            orbit-plane frame in forward mode (tangents: inclination,
 //            distance) contracted with the set-up adjoints.
+// The forward visits a chord's geometry only where something can happen on
+// it: when the orbit angle passes a crossing of the disc plane (closed form
+// per ray) or when a bound on the chord's distance from the centre does not
+// exclude the horizon; every other chord provably has neither.
 // Per-block partial sums are reduced in a fixed order by a second kernel, so
 // the gradient is deterministic.
 #include "bh_internal.h"
@@ -51,9 +55,9 @@ __device__ __forceinline__ D2 operator*(D2 a, D2 b) {
 }
 __device__ __forceinline__ double val(double a) { return a; }
 __device__ __forceinline__ double val(D2 a) { return a.v; }
-__device__ __forceinline__ double rsqrt_n(double a) { return 1.0 / sqrt(a); }
+__device__ __forceinline__ double rsqrt_n(double a) { return rsq<double>(a); }
 __device__ __forceinline__ D2 rsqrt_n(D2 a) {
-  const double y = 1.0 / sqrt(a.v);
+  const double y = rsq<double>(a.v);
   const double k = -0.5 * y * y * y;
   return {y, k * a.i, k * a.d};
 }
@@ -61,6 +65,12 @@ __device__ __forceinline__ double sqrt_n(double a) { return sqrt(a); }
 __device__ __forceinline__ D2 sqrt_n(D2 a) {
   const double y = sqrt(a.v);
   const double k = 0.5 / y;
+  return {y, k * a.i, k * a.d};
+}
+__device__ __forceinline__ double rcp_n(double a) { return 1.0 / a; }
+__device__ __forceinline__ D2 rcp_n(D2 a) {
+  const double y = 1.0 / a.v;
+  const double k = -y * y;
   return {y, k * a.i, k * a.d};
 }
 __device__ __forceinline__ void lift(double v, double ti, double td, double& o) { o = v; }
@@ -81,70 +91,96 @@ template <typename N> __device__ __forceinline__ void cross3(const N* a, const N
 // the ray is the line from the camera F = C through the pixel point
 // pv = f axis_x - axis_y (W/2 - px) - axis_z (H/2 - py) (camera.h:55-63, the
 // hole at the origin); yv = N(F - pv), zv = N(pv x yv), xv = yv x zv;
-// b = xv.F, Y0 = yv.F, r0 = |F|.  A point of the orbit at angle phi and
-// radius r is r (cos(phi) yv + sin(phi) xv).
+// b = xv.F, Y0 = yv.F, r0 = |F|.  The reference puts the orbit point at angle
+// phi and radius r at r (cos(phi) yv + sin(phi) xv) starting from
+// phi0 = atan(b / Y0); here the angle psi = phi - phi0 is measured from the
+// camera direction e1 = F / r0 with e2 = (Y0 xv - b yv) / r0, the same point
+// r (cos(psi) e1 + sin(psi) e2) without an arctangent.  (Y0 > 0 for a camera
+// looking at the hole: F.(F - pv) = dist^2 + dist f, so atan(b/Y0) is the
+// polar angle of F and psi = 0 is the camera.)
 template <typename N> struct Frame {
-  N yv[3], xv[3], F[3];
-  N b, Y0, r0;
+  N e1[3], e2[3];
+  N b, r0;
   bool degenerate;
 };
 
 template <typename N>
 __device__ inline void diff_frame(const DiffScene& s, int px, int py, Frame<N>& f) {
   const double sx = s.W / 2.0 - px, sy = s.H / 2.0 - py;
-  N pv[3], dv[3], cv[3], zv[3];
+  N pv[3], F[3], dv[3], yv[3], cv[3], zv[3], xv[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     lift(s.fx[k] - s.ay[k] * sx - s.az[k] * sy, s.fx_i[k] - s.ay_i[k] * sx - s.az_i[k] * sy,
          0.0, pv[k]);
-    lift(s.C[k], s.C_i[k], s.C_d[k], f.F[k]);
-    dv[k] = f.F[k] - pv[k];
+    lift(s.C[k], s.C_i[k], s.C_d[k], F[k]);
+    dv[k] = F[k] - pv[k];
   }
   const N ny = rsqrt_n(dot3(dv, dv));
 #pragma unroll
-  for (int k = 0; k < 3; ++k) f.yv[k] = dv[k] * ny;
-  cross3(pv, f.yv, cv);
+  for (int k = 0; k < 3; ++k) yv[k] = dv[k] * ny;
+  cross3(pv, yv, cv);
   const N cc = dot3(cv, cv);
   f.degenerate = !(val(cc) > 0.0);
   const N nz = rsqrt_n(f.degenerate ? dot3(dv, dv) : cc);
 #pragma unroll
   for (int k = 0; k < 3; ++k) zv[k] = cv[k] * nz;
-  cross3(f.yv, zv, f.xv);
-  f.b = dot3(f.xv, f.F);
-  f.Y0 = dot3(f.yv, f.F);
-  f.r0 = sqrt_n(dot3(f.F, f.F));
+  cross3(yv, zv, xv);
+  f.b = dot3(xv, F);
+  const N Y0 = dot3(yv, F);
+  f.r0 = sqrt_n(dot3(F, F));
+  const N ir0 = rcp_n(f.r0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f.e1[k] = F[k] * ir0;
+    f.e2[k] = (xv[k] * Y0 - yv[k] * f.b) * ir0;
+  }
 }
 
 // Inbound turning point of G(u) = 2M u^3 - u^2 + 1/b^2 for b >= b_c: the
 // closed form u* = (1/(3M)) (1/2 + sin(asin(2q - 1)/3)), q = 27 M^2 / b^2,
-// polished by Newton steps while G' < 0.
+// evaluated in FP32 (in FP64 within 1e-3 of the photon sphere, where the
+// root is ill-conditioned) and polished by three FP64 Newton steps while
+// G' < 0.
 __device__ inline double turning_point(double ib2, double M, double Uc) {
   const double q = fmin(1.0, 27.0 * M * M * ib2);
-  double sn, cs;
-  sincos_small64(asin(2.0 * q - 1.0) * (1.0 / 3.0), &sn, &cs);
-  double u = Uc * (0.5 + sn);
+  double u;
+  if (q > 0.999) {
+    double sn, cs;
+    sincos_small64(asin(2.0 * q - 1.0) * (1.0 / 3.0), &sn, &cs);
+    u = Uc * (0.5 + sn);
+  } else {
+    float sn, cs;
+    sincos_core(asinf((float)(2.0 * q - 1.0)) * (1.f / 3.f), &sn, &cs);
+    u = Uc * (0.5 + (double)sn);
+  }
 #pragma unroll
   for (int it = 0; it < 3; ++it) {
     const double g = __builtin_fma(u * u, __builtin_fma(2.0 * M, u, -1.0), ib2);
     const double gp = u * __builtin_fma(6.0 * M, u, -2.0);
-    if (gp < 0.0) u = u - g / gp;
+    double ig = __builtin_amdgcn_rcp(gp);
+    ig = ig * __builtin_fma(-gp, ig, 2.0);
+    ig = ig * __builtin_fma(-gp, ig, 2.0);
+    if (gp < 0.0) u = __builtin_fma(-g, ig, u);
   }
   return u;
 }
 
 // Per-ray march constants in the march precision T.
 template <typename T> struct DRay {
-  T yv[3], xv[3], C[3];
+  T e1[3], e2[3], C[3];
   T u0, du, h;
-  double phi0;  // the orbit angle is accumulated in FP64 in both precisions
   T c0, c1, c2, c3;  // G(l + s) = c0 + s (c1 + s (c2 + s c3))
+  // disc-plane crossings: z(psi) = 0 at psi = a0 + k pi; an event is due
+  // once psi >= (crossing - guard)
+  double a0, guard;
   bool captured, degenerate;
 };
 
-// FP64 set-up values shared by the march and the adjoint's epilogue.
+// FP64 set-up values shared by the march and the adjoint's epilogue.  The
+// orbit angle psi is accumulated in FP64 in both precisions.
 struct Setup64 {
   Frame<double> f;
-  double ib2, l, u0, du, phi0;
+  double ib2, l, u0, du;
   bool captured;
 };
 
@@ -156,22 +192,31 @@ __device__ inline void diff_setup(const DiffScene& s, int px, int py, Setup64& r
   r.l = r.captured ? s.Uc : turning_point(r.ib2, s.M, s.Uc);
   r.u0 = 1.0 / r.f.r0;
   r.du = (r.l - r.u0) * (1.0 / s.nstep);
-  // atan (not atan2) of b / Y0, as blackhole_solution_test.cc:193
-  r.phi0 = atan(b / r.f.Y0);
 }
 
 template <typename T>
 __device__ inline void to_ray(const DiffScene& s, const Setup64& r, DRay<T>& d) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    d.yv[k] = (T)r.f.yv[k];
-    d.xv[k] = (T)r.f.xv[k];
+    d.e1[k] = (T)r.f.e1[k];
+    d.e2[k] = (T)r.f.e2[k];
     d.C[k] = (T)s.C[k];
   }
   d.u0 = (T)r.u0;
   d.du = (T)r.du;
   d.h = d.du * (T)0.5;
-  d.phi0 = r.phi0;
+  // z(psi) ~ cos(psi) e1_z + sin(psi) e2_z vanishes at atan2(-e1_z, e2_z) + k pi
+  const float ez1 = (float)r.f.e1[2], ez2 = (float)r.f.e2[2];
+  const float amp2 = ez1 * ez1 + ez2 * ez2;
+  if (amp2 > 0.f) {
+    double a = (double)atan2_fast(-ez1, ez2);
+    if (a < 0.0) a += kPiD;
+    d.a0 = a;
+    d.guard = 4e-6 + 4e-7 * (double)__builtin_amdgcn_rsqf(amp2);
+  } else {  // orbit in the disc plane: z = 0 throughout, never a crossing
+    d.a0 = 1e300;
+    d.guard = 0.0;
+  }
   const double l = r.l, M = s.M;
   d.c0 = (T)__builtin_fma(l * l, __builtin_fma(2.0 * M, l, -1.0), r.ib2);
   d.c1 = (T)(l * __builtin_fma(6.0 * M, l, -2.0));
@@ -202,14 +247,21 @@ __device__ __forceinline__ void angle_sincos(double a, double* s, double* c) {
 }
 
 template <typename T>
-__device__ __forceinline__ void orbit_point(const DRay<T>& d, double phi, T r, T* P, T* cph,
+__device__ __forceinline__ void orbit_point(const DRay<T>& d, double psi, T r, T* P, T* cph,
                                             T* sph) {
   T sn, cs;
-  angle_sincos(phi, &sn, &cs);
+  angle_sincos(psi, &sn, &cs);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) P[k] = r * fma_t(cs, d.yv[k], sn * d.xv[k]);
+  for (int k = 0; k < 3; ++k) P[k] = r * fma_t(cs, d.e1[k], sn * d.e2[k]);
   *cph = cs;
   *sph = sn;
+}
+
+// The first crossing threshold at or after psi: a0 + k pi - guard with
+// a0 + k pi >= psi - guard.
+template <typename T> __device__ __forceinline__ double next_crossing(const DRay<T>& d, double psi) {
+  const double k = fmax(0.0, ceil((psi - d.guard - d.a0) * kInvPiD));
+  return __builtin_fma(k, kPiD, d.a0) - d.guard;
 }
 
 // StaticBlackhole::Collide (blackhole_solution.h:65-88) for the horizon at the
@@ -225,7 +277,7 @@ __device__ inline bool seg_horizon(const T* P1, const T* P2, T R, T* dh) {
   const T QQ = Q[0] * Q[0] + Q[1] * Q[1] + Q[2] * Q[2];
   const T cc = qd * qd - QQ * (d1 - R2);
   if (!(cc > (T)0)) return false;
-  const T t = (-qd - sqrt_t(cc)) / QQ;
+  const T t = (-qd - sqrt_t(cc)) * rcp(QQ);
   if (t <= (T)0 || t >= (T)1) return false;
   *dh = t * t * QQ;
   return true;
@@ -317,7 +369,7 @@ __device__ inline T disc_cross(const DiffScene& s, const T* P1, const T* P2, T* 
                                T* dd) {
   const T z1 = P1[2], z2 = P2[2];
   if (!(z1 * z2 < (T)0)) return (T)0;
-  const T sf = z1 / (z1 - z2);
+  const T sf = z1 * rcp(z1 - z2);
   T D[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -343,12 +395,17 @@ template <typename T> struct Acc {
 // Per-lane crossing records in LDS, strided by the block size so that a
 // wave's lanes hit consecutive words.
 template <typename T> struct Rec {
-  double* phi;  // phi_{j-1} at phi[2k kThreads], phi_j at phi[(2k + 1) kThreads]
+  double* psi;  // psi_{j-1} at psi[2k kThreads], psi_j at psi[(2k + 1) kThreads]
   T* tr;        // T_k at tr[k kThreads]
 };
 
 // Pass 1: the forward march.  With RECORD, crossing k stores
-// (phi_{j-1}, phi_j, T_k) in rec.
+// (psi_{j-1}, psi_j, T_k) in rec.  Chord j is examined only when psi_j has
+// reached a disc-plane crossing threshold or when the bound
+//   dist(chord, centre) >= min(r_{j-1}, r_j) cos(dpsi/2)
+//                       >= (1 - dpsi^2/8) / max(u_{j-1}, u_j)
+// does not clear the horizon radius; other chords have neither a sign change
+// of z nor a horizon entry, so skipping them changes nothing.
 template <typename T, bool RECORD>
 __device__ inline void diff_march(const DiffScene& s, const DRay<T>& d, Rec<T> rec, Acc<T>& a) {
   a.rgb[0] = a.rgb[1] = a.rgb[2] = (T)0;
@@ -365,9 +422,12 @@ __device__ inline void diff_march(const DiffScene& s, const DRay<T>& d, Rec<T> r
   }
   const int n = s.nstep;
   const T Rh = (T)s.Rh;
+  const T Rh_hi = (T)(s.Rh * (1.0 + 1e-5));
   T P1[3] = {d.C[0], d.C[1], d.C[2]};
-  double phi = d.phi0;
-  T wp = (T)0;
+  bool have1 = true;  // P1 holds the point of step j - 1
+  double psi = 0.0;
+  double a_ev = d.a0 - d.guard;
+  T wp = (T)0, u_prev = d.u0;
   for (int j = 1; j <= 2 * n - 1; ++j) {
     if (j > n && d.captured) {
       // chord to the centre: the horizon is entered, the disc plane is not
@@ -378,60 +438,73 @@ __device__ inline void diff_march(const DiffScene& s, const DRay<T>& d, Rec<T> r
     }
     const T sj = -sigma<T>(j, n) * d.du;
     const T w = rsq(g_at(d, sj));
-    const double phi_prev = phi;
-    phi += (double)((wp + w) * d.h);
+    const T dpsi = (wp + w) * d.h;
+    const double psi_prev = psi;
+    psi += (double)dpsi;
     wp = w;
     const T u = fma_t(kappa<T>(j, n), d.du, d.u0);
-    T P2[3], cs, sn;
-    orbit_point(d, phi, rcp(u), P2, &cs, &sn);
-    T dh = (T)0, q[3], sf, dd;
-    const bool hh = seg_horizon(P1, P2, Rh, &dh);
-    const T al = disc_cross(s, P1, P2, q, &sf, &dd);
-    if (al > (T)0 && (!hh || dd < dh)) {
-      T c[3];
-      disc_texel<T, false>(s, q, c, nullptr, nullptr);
-      if (RECORD) {
-        if (a.n < kMaxRec) {
-          rec.phi[(2 * a.n + 0) * kThreads] = phi_prev;
-          rec.phi[(2 * a.n + 1) * kThreads] = phi;
-          rec.tr[a.n * kThreads] = a.tr;
-          a.mask |= 1ull << j;
-        } else {
-          a.overflow = true;
+    const T umax = u > u_prev ? u : u_prev;
+    const bool need_h = !(dpsi < (T)3) || fma_t(-dpsi * dpsi, (T)0.125, (T)1) <= Rh_hi * umax;
+    const bool need_d = psi >= a_ev;
+    if (need_h || need_d) {
+      T cs, sn;
+      if (!have1) orbit_point(d, psi_prev, rcp(u_prev), P1, &cs, &sn);
+      T P2[3];
+      orbit_point(d, psi, rcp(u), P2, &cs, &sn);
+      T dh = (T)0, q[3], sf, dd;
+      const bool hh = need_h && seg_horizon(P1, P2, Rh, &dh);
+      const T al = disc_cross(s, P1, P2, q, &sf, &dd);
+      if (al > (T)0 && (!hh || dd < dh)) {
+        T c[3];
+        disc_texel<T, false>(s, q, c, nullptr, nullptr);
+        if (RECORD) {
+          if (a.n < kMaxRec) {
+            rec.psi[(2 * a.n + 0) * kThreads] = psi_prev;
+            rec.psi[(2 * a.n + 1) * kThreads] = psi;
+            rec.tr[a.n * kThreads] = a.tr;
+            a.mask |= 1ull << j;
+          } else {
+            a.overflow = true;
+          }
         }
+        const T ta = a.tr * al;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a.rgb[k] = fma_t(ta, c[k], a.rgb[k]);
+        a.tr = a.tr * ((T)1 - al);
+        if (a.cls == BH_HIT_NONE) a.cls = BH_HIT_DISC;
+        ++a.n;
+        if (!(a.tr > (T)0)) break;
       }
-      const T ta = a.tr * al;
+      if (hh) {
+        if (a.cls == BH_HIT_NONE) a.cls = BH_HIT_HORIZON;
+        a.tr = (T)0;
+        break;
+      }
 #pragma unroll
-      for (int k = 0; k < 3; ++k) a.rgb[k] = fma_t(ta, c[k], a.rgb[k]);
-      a.tr = a.tr * ((T)1 - al);
-      if (a.cls == BH_HIT_NONE) a.cls = BH_HIT_DISC;
-      ++a.n;
-      if (!(a.tr > (T)0)) break;
+      for (int k = 0; k < 3; ++k) P1[k] = P2[k];
+      have1 = true;
+      if (need_d) a_ev = next_crossing(d, psi);
+    } else {
+      have1 = false;
     }
-    if (hh) {
-      if (a.cls == BH_HIT_NONE) a.cls = BH_HIT_HORIZON;
-      a.tr = (T)0;
-      break;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) P1[k] = P2[k];
+    u_prev = u;
   }
 }
 
-// Adjoint of P = r (cos(phi) yv + sin(phi) xv), r = 1/u.
+// Adjoint of P = r (cos(psi) e1 + sin(psi) e2), r = 1/u.
 template <typename T>
 __device__ __forceinline__ void point_adjoint(const DRay<T>& d, const T* Pb, T cs, T sn, T r,
-                                              T* yvb, T* xvb, T* phib, T* ub) {
-  T py = (T)0, px = (T)0;
+                                              T* e1b, T* e2b, T* psib, T* ub) {
+  T p1 = (T)0, p2 = (T)0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    yvb[k] = fma_t(Pb[k], r * cs, yvb[k]);
-    xvb[k] = fma_t(Pb[k], r * sn, xvb[k]);
-    py = fma_t(Pb[k], d.yv[k], py);
-    px = fma_t(Pb[k], d.xv[k], px);
+    e1b[k] = fma_t(Pb[k], r * cs, e1b[k]);
+    e2b[k] = fma_t(Pb[k], r * sn, e2b[k]);
+    p1 = fma_t(Pb[k], d.e1[k], p1);
+    p2 = fma_t(Pb[k], d.e2[k], p2);
   }
-  *phib = r * (cs * px - sn * py);
-  *ub = -(cs * py + sn * px) * r * r;
+  *psib = r * (cs * p2 - sn * p1);
+  *ub = -(cs * p1 + sn * p2) * r * r;
 }
 
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
@@ -445,7 +518,7 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
                                                         float* __restrict__ rgb_out,
                                                         uint8_t* __restrict__ hit_out,
                                                         double* __restrict__ partial) {
-  __shared__ double rec_phi[ADJ ? 2 * kMaxRec * kThreads : 1];
+  __shared__ double rec_psi[ADJ ? 2 * kMaxRec * kThreads : 1];
   __shared__ T rec_tr[ADJ ? kMaxRec * kThreads : 1];
   __shared__ double red[ADJ ? kPartial * (kThreads / 64) : 1];
   const int tid = threadIdx.x;
@@ -460,7 +533,7 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
     DRay<T> d;
     to_ray(s, su, d);
     Acc<T> a;
-    const Rec<T> rec{rec_phi + tid, rec_tr + tid};
+    const Rec<T> rec{rec_psi + tid, rec_tr + tid};
     diff_march<T, ADJ>(s, d, rec, a);
     if (rgb_out) {
       rgb_out[3 * idx + 0] = (float)a.rgb[0];
@@ -473,7 +546,7 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
       const T gb[3] = {(T)grad_rgb[3 * idx], (T)grad_rgb[3 * idx + 1], (T)grad_rgb[3 * idx + 2]};
       const int n = s.nstep;
       T B[3] = {(T)0, (T)0, (T)0};
-      T yvb[3] = {(T)0, (T)0, (T)0}, xvb[3] = {(T)0, (T)0, (T)0}, Cb[3] = {(T)0, (T)0, (T)0};
+      T e1b[3] = {(T)0, (T)0, (T)0}, e2b[3] = {(T)0, (T)0, (T)0}, Cb[3] = {(T)0, (T)0, (T)0};
       T A_next = (T)0, pend = (T)0, hb = (T)0, u0b = (T)0, dub = (T)0, Mb = (T)0, ib2b = (T)0,
         rinb = (T)0;
       int k = a.n < kMaxRec ? a.n - 1 : kMaxRec - 1;
@@ -485,8 +558,8 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
         pend = (T)0;
         T ubj = (T)0;
         if ((a.mask >> j) & 1) {
-          const double ph_a = rec.phi[(2 * k + 0) * kThreads];
-          const double ph_b = rec.phi[(2 * k + 1) * kThreads];
+          const double ps_a = rec.psi[(2 * k + 0) * kThreads];
+          const double ps_b = rec.psi[(2 * k + 1) * kThreads];
           const T Tk = rec.tr[k * kThreads];
           --k;
           T P1[3], P2[3], ca = (T)0, sa = (T)0, ra = (T)0, cb, sb;
@@ -495,10 +568,10 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
             for (int m = 0; m < 3; ++m) P1[m] = d.C[m];
           } else {
             ra = rcp(fma_t(kappa<T>(j - 1, n), d.du, d.u0));
-            orbit_point(d, ph_a, ra, P1, &ca, &sa);
+            orbit_point(d, ps_a, ra, P1, &ca, &sa);
           }
           const T rb = rcp(uj);
-          orbit_point(d, ph_b, rb, P2, &cb, &sb);
+          orbit_point(d, ps_b, rb, P2, &cb, &sb);
           T q[3], sf, dd;
           const T z1 = P1[2], z2 = P2[2];
           disc_cross(s, P1, P2, q, &sf, &dd);
@@ -536,20 +609,20 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
           P1b[2] = fma_t(-kd, z2, P1b[2]);
           P2b[2] = fma_t(kd, z1, P2b[2]);
           T phb, ub;
-          point_adjoint(d, P2b, cb, sb, rb, yvb, xvb, &phb, &ub);
+          point_adjoint(d, P2b, cb, sb, rb, e1b, e2b, &phb, &ub);
           Aj += phb;
           ubj += ub;
           if (j == 1) {
 #pragma unroll
             for (int m = 0; m < 3; ++m) Cb[m] += P1b[m];
           } else {
-            point_adjoint(d, P1b, ca, sa, ra, yvb, xvb, &phb, &ub);
+            point_adjoint(d, P1b, ca, sa, ra, e1b, e2b, &phb, &ub);
             pend = phb;
             u0b += ub;
             dub = fma_t(kappa<T>(j - 1, n), ub, dub);
           }
         }
-        // phi_m = phi0 + h sum_{i <= m} (w_{i-1} + w_i): w_j enters phi_m for
+        // psi_m = h sum_{i <= m} (w_{i-1} + w_i): w_j enters psi_m for
         // m >= j twice over (increments j and j + 1)
         const T sj = -sigma<T>(j, n) * d.du;
         const T G = g_at(d, sj);
@@ -566,7 +639,7 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
         A_next = Aj;
       }
       // epilogue in FP64: h = du/2, du = (l - u0)/n, l(M, 1/b^2), set-up
-      const double phi0b = (double)(A_next + pend);
+      // (psi starts at 0 whatever the parameters, so A_1 goes nowhere)
       double dub64 = (double)dub + 0.5 * (double)hb;
       const double lb = dub64 / n;
       double u0b64 = (double)u0b - lb;
@@ -579,19 +652,17 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
         Mb64 += lb * (-2.0 * l * l * l / Gu);
         ib2b64 += lb * (-1.0 / Gu);
       }
-      const double b = su.f.b, Y0 = su.f.Y0, r0 = su.f.r0;
-      const double den = b * b + Y0 * Y0;
-      const double bb = ib2b64 * (-2.0 / (b * b * b)) + phi0b * (Y0 / den);
-      const double Y0b = -phi0b * (b / den);
+      const double b = su.f.b, r0 = su.f.r0;
+      const double bb = ib2b64 * (-2.0 / (b * b * b));
       const double r0b = -u0b64 / (r0 * r0);
       Frame<D2> fd;
       diff_frame(s, px, py, fd);
-      double gi = bb * fd.b.i + Y0b * fd.Y0.i + r0b * fd.r0.i;
-      double gd = bb * fd.b.d + Y0b * fd.Y0.d + r0b * fd.r0.d;
+      double gi = bb * fd.b.i + r0b * fd.r0.i;
+      double gd = bb * fd.b.d + r0b * fd.r0.d;
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        gi += (double)yvb[m] * fd.yv[m].i + (double)xvb[m] * fd.xv[m].i + (double)Cb[m] * s.C_i[m];
-        gd += (double)yvb[m] * fd.yv[m].d + (double)xvb[m] * fd.xv[m].d + (double)Cb[m] * s.C_d[m];
+        gi += (double)e1b[m] * fd.e1[m].i + (double)e2b[m] * fd.e2[m].i + (double)Cb[m] * s.C_i[m];
+        gd += (double)e1b[m] * fd.e1[m].d + (double)e2b[m] * fd.e2[m].d + (double)Cb[m] * s.C_d[m];
       }
       g_rs = 0.5 * Mb64;  // M = r_s / 2
       g_in = gi;

@@ -139,17 +139,19 @@ def render(theta: torch.Tensor, dr: DiffRenderer) -> torch.Tensor:
 
 
 # per-parameter step scale of the fit (units: r_s, rad, r_s, r_s)
-FIT_SCALE = (0.05, math.radians(2.0), 0.5, 0.2)
+FIT_SCALE = (0.025, math.radians(1.0), 0.25, 0.1)
 
 
-def fit(dr: DiffRenderer, target: torch.Tensor, theta0, iters: int = 200, lr: float = 0.05,
+def fit(dr: DiffRenderer, target: torch.Tensor, theta0, iters: int = 200, lr: float = 0.2,
         log=None):
-    """Adam on the mean squared error to target.  theta = theta0 + scale * x,
-    x from 0.  Returns (theta, history of (loss, theta))."""
+    """Adam on the mean squared error to target, learning rate annealed on a
+    cosine to 5% over the iterations.  theta = theta0 + scale * x, x from 0.
+    Returns (theta, history of (loss, theta))."""
     scale = torch.tensor(FIT_SCALE, dtype=torch.float64)
     base = torch.tensor([float(v) for v in theta0], dtype=torch.float64)
     x = torch.zeros(4, dtype=torch.float64, requires_grad=True)
     opt = torch.optim.Adam([x], lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=iters, eta_min=0.05 * lr)
     hist = []
     n = float(target.numel())
     for it in range(iters):
@@ -162,6 +164,7 @@ def fit(dr: DiffRenderer, target: torch.Tensor, theta0, iters: int = 200, lr: fl
         if log is not None and (it % 20 == 0 or it == iters - 1):
             log(it, hist[-1])
         opt.step()
+        sched.step()
     theta = (base + scale * x).detach()
     return theta.tolist(), hist
 

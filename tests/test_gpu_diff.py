@@ -13,8 +13,8 @@ Bars:
             crossing counts and classes change discretely).  FP64 at 64x64
             (the gradcheck), FP32 at 64x64 and 256x256.
   fit       200 Adam iterations at 512x512 FP32 from the perturbed start
-            reduce the loss five-fold and move every parameter toward the
-            true one.
+            reduce the loss twenty-fold and bring every parameter to within a
+            tenth of its starting error.
 """
 import os
 
@@ -135,9 +135,9 @@ def test_inverse_fit_moves_toward_truth(dctx):
     print(f"[fit] loss {loss0:.1f} -> {loss1:.1f}, theta {theta} (true {diff.FIT_TRUE})",
           flush=True)
     assert dr.overflow() == 0
-    assert loss1 < 0.2 * loss0
+    assert loss1 < 0.05 * loss0
     for t1, t0, tt in zip(theta, diff.FIT_START, diff.FIT_TRUE):
-        assert abs(t1 - tt) < 0.5 * abs(t0 - tt)
+        assert abs(t1 - tt) < 0.1 * abs(t0 - tt)
 
 
 def test_invalid_parameters_are_rejected(dctx):
