@@ -78,6 +78,17 @@ class Outputs(C.Structure):
                 ("hit_point", C.c_void_p), ("steps", C.c_void_p)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("waves_per_simd", C.c_int32), ("refill_steps", C.c_int32),
+                ("refill_idle", C.c_int32), ("chunk", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("march_ms", C.c_double), ("band_ms", C.c_double),
+                ("rays", C.c_int64), ("band_rays", C.c_int64), ("solo_scenes", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
 class DiffParams(C.Structure):
     _fields_ = [("r_s", C.c_double), ("incl", C.c_double), ("dist", C.c_double),
                 ("r_inner", C.c_double), ("r_outer", C.c_double), ("azim", C.c_double),
@@ -123,6 +134,13 @@ SIGNATURES = {
     "bh_time_render": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options), C.c_int,
                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int64)]),
+    "bh_render_batch_device": (C.c_int, [P, C.c_int, C.POINTER(Scene), C.POINTER(Options),
+                                         C.POINTER(Outputs), C.POINTER(BatchTuning),
+                                         C.POINTER(C.c_double), C.POINTER(BatchStats)]),
+    "bh_batch_autotune": (C.c_int, [P, C.c_int, C.POINTER(Scene), C.POINTER(Options),
+                                    C.POINTER(Outputs), C.POINTER(BatchTuning), C.c_int, C.c_int,
+                                    C.POINTER(BatchTuning), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
     "bh_diff_render_device": (C.c_int, [P, C.POINTER(DiffParams), C.c_int, P, P]),
     "bh_diff_adjoint_device": (C.c_int, [P, C.POINTER(DiffParams), C.c_int, P, P, P, P, P]),
     "bh_diff_time": (C.c_int, [P, C.POINTER(DiffParams), C.c_int, C.c_int,

@@ -2,6 +2,7 @@
 // validation and upload, kernel launches, camera-path graphs, timing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -37,6 +38,8 @@ struct bh_ctx {
   DeviceTexture tex[BH_MAX_TEXTURES];
   DevBuf scene, rgb, hit, phi, hitpt, steps, qitems, qcount;
   DevBuf dpart, dgrad, drgb, dhit, dgin;  // differentiable render
+  DevBuf bscenes, bouts, boff, bctl, bband, bend;  // batches
+  int wall_khz = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   char name[256] = {0};
   int n_cu = 0, clock_khz = 0;
@@ -277,6 +280,138 @@ int queue_buffers(bh_ctx* ctx, uint32_t** items, uint32_t** count, size_t npix) 
 }
 
 
+// ---- batches -----------------------------------------------------------------
+struct BatchPlan {
+  std::vector<DevScene> scenes;
+  std::vector<bh::DevOut> outs;
+  std::vector<uint32_t> off;   // ray offsets of the FP32 scenes (n_fast + 1)
+  std::vector<int> fast;       // indices of the scenes in the persistent march
+  std::vector<int> solo;       // indices rendered one by one
+  uint32_t n_rays = 0;
+};
+
+int plan_batch(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* options,
+               const bh_outputs* outs, BatchPlan* p) {
+  if (n <= 0 || !scenes || !options || !outs)
+    return fail(ctx, BH_ERR_INVALID_ARG, "batch needs n > 0 and scene, option and output arrays");
+  p->scenes.resize(n);
+  p->outs.resize(n);
+  uint64_t rays = 0;
+  p->off.push_back(0);
+  for (int i = 0; i < n; ++i) {
+    int rc = build_scene(ctx, &scenes[i], &options[i], &p->scenes[i]);
+    if (rc) {
+      ctx->err = "scene " + std::to_string(i) + ": " + ctx->err;
+      return rc;
+    }
+    const bh_outputs& o = outs[i];
+    p->outs[i] = DevOut{o.rgb, o.hit, o.phi_end, o.hit_point, o.steps};
+    if (fp32_eligible(p->scenes[i])) {
+      const DevScene& d = p->scenes[i];
+      rays += (uint64_t)((d.W + 7) / 8) * (uint64_t)((d.H + 7) / 8) * 64u;
+      if (rays >= ((uint64_t)1 << 31)) return fail(ctx, BH_ERR_INVALID_ARG, "batch larger than 2^31 rays");
+      p->fast.push_back(i);
+      p->off.push_back((uint32_t)rays);
+    } else {
+      p->solo.push_back(i);
+    }
+  }
+  p->n_rays = (uint32_t)rays;
+  return BH_OK;
+}
+
+bh_batch_tuning normalized_tuning(const bh_batch_tuning* t) {
+  bh_batch_tuning r{256, 4, 4, 64, 4};
+  if (t) {
+    if (t->block) r.block = t->block;
+    if (t->waves_per_simd) r.waves_per_simd = t->waves_per_simd;
+    if (t->refill_steps) r.refill_steps = t->refill_steps;
+    if (t->refill_idle) r.refill_idle = t->refill_idle;
+    if (t->chunk) r.chunk = t->chunk;
+  }
+  return r;
+}
+
+int check_tuning(bh_ctx* ctx, const bh_batch_tuning& t) {
+  if (t.block != 64 && t.block != 128 && t.block != 256)
+    return fail(ctx, BH_ERR_INVALID_ARG, "tuning.block must be 64, 128 or 256");
+  if (t.waves_per_simd < 1 || t.waves_per_simd > 8)
+    return fail(ctx, BH_ERR_INVALID_ARG, "tuning.waves_per_simd must be in 1..8");
+  if (t.refill_steps < 1 || t.refill_steps > 64)
+    return fail(ctx, BH_ERR_INVALID_ARG, "tuning.refill_steps must be in 1..64");
+  if (t.refill_idle < 1 || t.refill_idle > 64)
+    return fail(ctx, BH_ERR_INVALID_ARG, "tuning.refill_idle must be in 1..64");
+  if (t.chunk < 1 || t.chunk > 1024)
+    return fail(ctx, BH_ERR_INVALID_ARG, "tuning.chunk must be in 1..1024");
+  return BH_OK;
+}
+
+// Upload the plan and launch clear + persistent march + band pass (+ solo
+// scenes).  Events: ev0 before the march, ev1 after it, ev2 after the band
+// pass.  The tables go to the device with one synchronous copy each.
+int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::BatchDev* bd_out) {
+  const size_t n = p.scenes.size();
+  int rc;
+  if ((rc = ensure(ctx, ctx->bscenes, n * sizeof(DevScene)))) return rc;
+  if ((rc = ensure(ctx, ctx->bouts, n * sizeof(DevOut)))) return rc;
+  if ((rc = ensure(ctx, ctx->boff, (p.off.size() + 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = ensure(ctx, ctx->bctl, 64))) return rc;
+  if ((rc = ensure(ctx, ctx->bend, (n + 1) * sizeof(unsigned long long)))) return rc;
+  const uint32_t cap = std::max<uint32_t>(p.n_rays / 16u, 4096u);
+  if ((rc = ensure(ctx, ctx->bband, (size_t)cap * sizeof(uint32_t)))) return rc;
+  hipStream_t st = ctx->stream;
+  // the persistent march reads the FP32 scenes by position in the plan
+  std::vector<DevScene> fs;
+  std::vector<DevOut> fo;
+  for (int i : p.fast) {
+    fs.push_back(p.scenes[i]);
+    fo.push_back(p.outs[i]);
+  }
+  DevScene* d_all = (DevScene*)ctx->bscenes.p;
+  DevScene* d_fast = d_all;
+  // FP32 scenes first, then the solo ones
+  std::vector<DevScene> order(fs);
+  for (int i : p.solo) order.push_back(p.scenes[i]);
+  HIP_TRY(ctx, hipMemcpyAsync(d_all, order.data(), n * sizeof(DevScene), hipMemcpyHostToDevice, st));
+  if (!fo.empty())
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->bouts.p, fo.data(), fo.size() * sizeof(DevOut),
+                                hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->boff.p, p.off.data(), p.off.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, st));
+  bh::BatchDev b;
+  b.scenes = d_fast;
+  b.outs = (const DevOut*)ctx->bouts.p;
+  b.ray_off = (const uint32_t*)ctx->boff.p;
+  b.n_scenes = (uint32_t)p.fast.size();
+  b.n_rays = p.n_rays;
+  b.head = (uint32_t*)ctx->bctl.p;
+  b.band_count = (uint32_t*)ctx->bctl.p + 1;
+  b.t_start = (unsigned long long*)((char*)ctx->bctl.p + 8);
+  b.band_items = (uint32_t*)ctx->bband.p;
+  b.band_capacity = cap;
+  b.refill_steps = t.refill_steps;
+  b.refill_idle = t.refill_idle;
+  b.chunk = t.chunk;
+  b.scene_end = (unsigned long long*)ctx->bend.p;
+  HIP_TRY(ctx, bh::launch_batch_clear(b, st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  if (b.n_scenes > 0) {
+    const int waves = std::max(1, ctx->n_cu) * 4 * t.waves_per_simd;
+    const int grid = std::max(1, waves / (t.block / 64));
+    HIP_TRY(ctx, bh::launch_batch(b, t.block, grid, st));
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  if (b.n_scenes > 0) HIP_TRY(ctx, bh::launch_batch_band(b, st));
+  for (size_t k = 0; k < p.solo.size(); ++k) {
+    const int i = p.solo[k];
+    DevQueue q{nullptr, nullptr, 0};
+    if ((rc = launch_frame(ctx, p.scenes[i], d_all + p.fast.size() + k, p.outs[i], q, st))) return rc;
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+  if (bd_out) *bd_out = b;
+  return BH_OK;
+}
+
 // ---- differentiable scene ---------------------------------------------------
 // FP64 number carrying d/d(inclination), for the camera frame.
 struct HD {
@@ -409,6 +544,8 @@ int bh_ctx_create(int device, bh_ctx** out) {
     c->n_cu = prop.multiProcessorCount;
     c->clock_khz = prop.clockRate;
   }
+  if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
+    c->wall_khz = 0;
   *out = c;
   return BH_OK;
 }
@@ -421,7 +558,8 @@ void bh_ctx_destroy(bh_ctx* ctx) {
     if (t.d) (void)hipFree(t.d);
   for (DevBuf* b : {&ctx->scene, &ctx->rgb, &ctx->hit, &ctx->phi, &ctx->hitpt, &ctx->steps,
                     &ctx->qitems, &ctx->qcount, &ctx->dpart, &ctx->dgrad, &ctx->drgb, &ctx->dhit,
-                    &ctx->dgin})
+                    &ctx->dgin, &ctx->bscenes, &ctx->bouts, &ctx->boff, &ctx->bctl, &ctx->bband,
+                    &ctx->bend})
     if (b->p) (void)hipFree(b->p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -815,6 +953,87 @@ int bh_diff_time(bh_ctx* ctx, const bh_diff_params* p, int precision, int iters,
   HIP_TRY(ctx, hipEventElapsedTime(&ms_a, ctx->ev1, ctx->ev2));
   if (fwd_ms) *fwd_ms = ms_f / iters;
   if (fwd_adj_ms) *fwd_adj_ms = ms_a / iters;
+  return BH_OK;
+}
+
+/* ---- batches ------------------------------------------------------------------- */
+int bh_render_batch_device(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* options,
+                           const bh_outputs* outs, const bh_batch_tuning* tune, double* scene_ms,
+                           bh_batch_stats* stats) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  const bh_batch_tuning t = normalized_tuning(tune);
+  int rc = check_tuning(ctx, t);
+  if (rc) return rc;
+  BatchPlan p;
+  if ((rc = plan_batch(ctx, n, scenes, options, outs, &p))) return rc;
+  (void)hipSetDevice(ctx->device);
+  bh::BatchDev b;
+  if ((rc = run_batch(ctx, p, t, &b))) return rc;
+  if (!scene_ms && !stats) return BH_OK;
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+  if (stats) {
+    float m = 0, tot = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&m, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&tot, ctx->ev0, ctx->ev2));
+    uint32_t nb = 0;
+    HIP_TRY(ctx, hipMemcpy(&nb, b.band_count, 4, hipMemcpyDeviceToHost));
+    stats->march_ms = m;
+    stats->band_ms = tot - m;
+    stats->total_ms = tot;
+    int64_t px = 0;
+    for (const auto& d : p.scenes) px += (int64_t)d.W * d.H;
+    stats->rays = px;
+    stats->band_rays = std::min(nb, b.band_capacity);
+    stats->solo_scenes = (int32_t)p.solo.size();
+    stats->pad_ = 0;
+  }
+  if (scene_ms) {
+    std::vector<unsigned long long> end(p.fast.size());
+    unsigned long long t0 = 0;
+    if (!end.empty())
+      HIP_TRY(ctx, hipMemcpy(end.data(), b.scene_end, end.size() * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(&t0, b.t_start, 8, hipMemcpyDeviceToHost));
+    const double khz = ctx->wall_khz > 0 ? ctx->wall_khz : 100000.0;
+    float tot = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&tot, ctx->ev0, ctx->ev2));
+    for (int i = 0; i < n; ++i) scene_ms[i] = tot;  // solo scenes: end of the call
+    for (size_t k = 0; k < p.fast.size(); ++k)
+      scene_ms[p.fast[k]] = end[k] >= t0 ? (double)(end[k] - t0) / khz : 0.0;
+  }
+  return BH_OK;
+}
+
+int bh_batch_autotune(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* options,
+                      const bh_outputs* outs, const bh_batch_tuning* cand, int n_cand, int reps,
+                      bh_batch_tuning* best, double* best_ms, double* cand_ms) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (!cand || n_cand <= 0 || !best) return fail(ctx, BH_ERR_INVALID_ARG, "need candidates and best");
+  if (reps <= 0) reps = 1;
+  BatchPlan p;
+  int rc = plan_batch(ctx, n, scenes, options, outs, &p);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  double bms = 1e300;
+  int bi = -1;
+  for (int c = 0; c < n_cand; ++c) {
+    const bh_batch_tuning t = normalized_tuning(&cand[c]);
+    if ((rc = check_tuning(ctx, t))) return rc;
+    double cms = 1e300;
+    for (int r = 0; r < reps + 1; ++r) {  // first run warms up
+      if ((rc = run_batch(ctx, p, t, nullptr))) return rc;
+      HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+      float ms = 0;
+      HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev2));
+      if (r > 0) cms = std::min(cms, (double)ms);
+    }
+    if (cand_ms) cand_ms[c] = cms;
+    if (cms < bms) {
+      bms = cms;
+      bi = c;
+    }
+  }
+  *best = normalized_tuning(&cand[bi]);
+  if (best_ms) *best_ms = bms;
   return BH_OK;
 }
 

@@ -92,6 +92,32 @@ hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue&
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
+// Batch of independent FP32 scenes marched by one persistent launch.  Ray ids
+// enumerate each scene's pixels in 8x8 tiles (row-major tiles, row-major
+// within a tile), scenes back to back: scene s owns ids
+// [ray_off[s], ray_off[s + 1]), padded to whole tiles.
+// ---------------------------------------------------------------------------
+struct BatchDev {
+  const DevScene* scenes;
+  const DevOut* outs;
+  const uint32_t* ray_off;   // n_scenes + 1 entries
+  uint32_t n_scenes, n_rays;
+  uint32_t* head;            // next ray id of the queue
+  uint32_t* band_items;      // band rays (ray ids) for the FP64 pass
+  uint32_t* band_count;      // [0] queued band rays
+  uint32_t band_capacity;
+  int32_t refill_steps;      // steps between refill checks
+  int32_t refill_idle;       // refill once at least this many lanes of 64 idle
+  int32_t chunk;             // least tiles per queue pop
+  unsigned long long* scene_end;  // per scene: latest finish time (wall clock)
+  unsigned long long* t_start;    // earliest wave start (wall clock)
+};
+
+hipError_t launch_batch(const BatchDev& b, int block, int grid, hipStream_t stream);
+hipError_t launch_batch_band(const BatchDev& b, hipStream_t stream);
+hipError_t launch_batch_clear(const BatchDev& b, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
 // Differentiable hole + disc scene (bh_diff_params), resolved on the host.
 // Passed by value as a kernel argument.  Derivatives are with respect to the
 // inclination (suffix _i) and the camera distance (suffix _d); the camera

@@ -7,7 +7,7 @@
 //                  an object can be reached; rays in the photon-sphere band
 //                  are queued for band_kernel.
 //  band_kernel     the same event march in FP64 for the queued band rays.
-#include "bh_math.h"
+#include "bh_march.h"
 
 namespace bh {
 
@@ -23,426 +23,6 @@ __global__ __launch_bounds__(256) void march64_kernel(const DevScene* __restrict
   Pixel64 px;
   trace64(s, x, y, px);
   store_pixel64(out, (size_t)y * s.W + x, px);
-}
-
-// ---------------------------------------------------------------------------
-// FP32 event march
-// ---------------------------------------------------------------------------
-//
-// Orbit-plane reduction.  The reference places step k at
-//   P_k = B + r_k (cos phi_k yv + sin phi_k xv),  r_k = 1/u_k,
-// so every chord lies in the plane through B spanned by (yv, xv).  With
-// e1 = cos phi0 yv + sin phi0 xv = sign(Y0) F/|F| and e2 = e1 rotated by +90
-// degrees in that plane, P_k = B + r_k (cos a_k e1 + sin a_k e2) where
-// a_k = phi_k - phi0 is the angle swept since phi0.
-//
-// G on the march.  With l the turning point and s = l - u,
-//   G(u) = G(l) + s [2 l d + s (1 - 2 d) - 2 M s^2],  d = 1 - 3 M l,
-// a cubic in s whose terms do not cancel away from the photon sphere; the
-// coefficients come from FP64.  The reference's u_k = u0 + kappa_k du gives
-// s_k = sigma_k du with sigma_k = nstep - kappa_k exact, so a step is one
-// multiply, three FMAs, one rsqrt and the trapezoid update of a (FP64 sum).
-//
-// Events.  A chord can only reach the disc plane if sign(n . (P - B)) =
-// sign(alpha cos a + beta sin a) changes, i.e. a passes a crossing angle
-// chi_0 + j pi; the step is tested exactly when a passes the next one (minus
-// a guard for the FP32 angle).  A chord between radii >= r_min subtending Da
-// stays at distance >= r_min cos(Da/2) from B, so it can only reach the
-// horizon when Da > 2 acos(R_h / r_min) >= pi (1 - R_h / r_min); that step is
-// tested exactly too.  Every other step provably hits nothing.
-template <typename T>
-struct Lane {
-  T c0, c1, c2, c3;   // G(s) cubic
-  T du, h;            // du and du/2
-  T du01;             // 0.1 du
-  T u0;               // 1/|C - B|
-  float phi_safe;     // horizon guard angle
-  float guard;        // disc crossing-angle guard
-  double chi0;        // a disc crossing angle (crossings at chi0 + j pi)
-  double a_ev;        // next disc event angle (crossing - guard)
-};
-
-struct HitRec {
-  int cls;       // BH_HIT_*
-  double ang;    // rotation of the frame the hit point is expressed in
-  float hx, hy;  // hit point relative to B in that frame
-};
-
-// Exact tests of the chord from (kappa_a, a_a) to (kappa_b, a_b), where
-// u = u0 + kappa du and a is the swept angle; da = a_b - a_a.
-template <typename T>
-__device__ inline bool event_chord(const DevScene& s, const Lane<T>& L, T ka, double aa, T kb,
-                                   double ab, T da, bool th, bool td, HitRec& hit) {
-  const T ra = rcp<T>(fma_t<T>(ka, L.du, L.u0));
-  const T rb = rcp<T>(fma_t<T>(kb, L.du, L.u0));
-  bool found = false;
-  T best = 0;
-  if (th) {
-    // frame rotated by aa: P_a = (ra, 0), P_b = rb (cos da, sin da);
-    // StaticBlackhole::Collide (blackhole_solution.h:65-88) in the plane
-    const T R = (T)s.Rh;
-    T sd, cd;
-    sincos_t<T>(da, &sd, &cd);
-    const T qx = rb * cd - ra, qy = rb * sd;
-    const T qd = qx * ra;
-    const T qq = qx * qx + qy * qy;
-    if (!(ra < R && rb < R)) {
-      const T cc = qd * qd - qq * (ra * ra - R * R);
-      if (cc > (T)0) {
-        const T t = (-qd - sqrt_t<T>(cc)) * rcp<T>(qq);
-        if (t > (T)0 && t < (T)1) {
-          found = true;
-          best = t * t * qq;
-          hit.cls = BH_HIT_HORIZON;
-          hit.ang = aa;
-          hit.hx = (float)(ra + t * qx);
-          hit.hy = (float)(t * qy);
-        }
-      }
-    }
-  }
-  if (td) {
-    // Annulus::Collide (vector_object.h:328-347): n.(P - B) ~ r sin(a - chi)
-    // changes sign an odd number of times; crossing by linear interpolation
-    // on the chord with weights |t_b|, |t_a|
-    const double fa = floor((aa - L.chi0) * kInvPiD);
-    const double fb = floor((ab - L.chi0) * kInvPiD);
-    if (((int)fb - (int)fa) & 1) {
-      const double chik = L.chi0 + fmax(fa, fb) * kPiD;
-      T sa, ca, sb, cb;
-      sincos_t<T>((T)(aa - chik), &sa, &ca);
-      sincos_t<T>((T)(ab - chik), &sb, &cb);
-      const T wa = fabs(ra * sa), wb = fabs(rb * sb);
-      const T inv = rcp<T>(wa + wb);
-      const T px = (wb * ra * ca + wa * rb * cb) * inv;
-      const T py = (wb * ra * sa + wa * rb * sb) * inv;
-      const T rho2 = px * px + py * py;
-      if (!(rho2 > (T)s.r_out2) && !(rho2 < (T)s.r_in2)) {
-        const T dx = ra * ca - px, dy = ra * sa - py;
-        const T d2 = dx * dx + dy * dy;
-        if (!found || d2 < best) {
-          found = true;
-          hit.cls = BH_HIT_DISC;
-          hit.ang = chik;
-          hit.hx = (float)px;
-          hit.hy = (float)py;
-        }
-      }
-    }
-  }
-  return found;
-}
-
-__device__ __forceinline__ double next_event(double a, double chi0, float guard) {
-  return chi0 + (floor((a - (double)guard - chi0) * kInvPiD) + 1.0) * kPiD - (double)guard;
-}
-
-struct MarchOut {
-  HitRec hit;
-  int steps;     // steps taken
-  double a;      // swept angle after the last step taken
-  double du64;   // du of the ray
-  float d;       // last increment taken (for the escape direction)
-};
-
-// The march of one ray in precision T.  first_aa is the angle of the camera
-// in the (e1, e2) frame: 0 when Y0 > 0, pi when Y0 < 0 (quirk Q2: the
-// reference's phi0 = atan(b/Y0) then points away from the camera, and only
-// the first chord starts at the camera itself).
-//
-// Step n has u = u0 + kappa_n du and s = l - u = sigma_n du with
-//   inbound   n < nstep:  kappa = n,                 sigma = nstep - n
-//   approach  n = nstep:  kappa = nstep - 0.1,       sigma = 0.1
-//   outbound  n > nstep:  kappa = 2 nstep - 0.1 - n, sigma = 0.1 + (n - nstep)
-//
-// Every lane of a wave is at the same step, so the loops run wave-uniform
-// trip counts and keep no per-lane flag: a ray that has ended is disarmed by
-// setting its event thresholds to +inf (its results were recorded when it
-// ended; it keeps stepping harmlessly), and every four steps the wave leaves
-// once no lane is armed.  This keeps the exec-mask bookkeeping of the hot
-// loop to a handful of scalar instructions.
-template <typename T, bool ESCAPE>
-__device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, double first_aa,
-                             double du64, MarchOut& o) {
-  const int nstep = s.nstep;
-  T w = (T)0;
-  double a = 0.0;
-  T d = (T)0;
-  o.hit.cls = BH_HIT_NONE;
-  o.hit.ang = 0.0;
-  o.hit.hx = o.hit.hy = 0.f;
-  o.du64 = du64;
-  o.steps = 0;
-  o.a = 0.0;
-  o.d = 0.f;
-  const float kInf = __builtin_inff();
-  auto end_ray = [&](int n) {
-    o.steps = n;
-    o.a = a;
-    o.d = (float)d;
-    L.phi_safe = kInf;
-    L.a_ev = 1e308;
-  };
-  auto armed = [&]() { return L.phi_safe < kInf; };
-  auto step = [&](int n, T sv, T ka, T kb, bool force) {
-    const T G = fma_t<T>(fma_t<T>(fma_t<T>(L.c3, sv, L.c2), sv, L.c1), sv, L.c0);
-    const T wn = rsq<T>(G);
-    d = (w + wn) * L.h;
-    w = wn;
-    a += (double)d;
-    const bool eh = (float)d > L.phi_safe;
-    if (eh || a >= L.a_ev || force) {
-      const double aa = force ? first_aa : a - (double)d;
-      const T da = force ? (T)(a - first_aa) : d;
-      if (event_chord<T>(s, L, ka, aa, kb, a, da, eh || force, s.has_disc != 0, o.hit))
-        end_ray(n);
-      else if (L.chi0 != 1e300)
-        L.a_ev = next_event(a, L.chi0, L.guard);
-    }
-  };
-  // inbound (blackhole_solution_test.cc:217-239)
-  step(1, (T)(nstep - 1) * L.du, (T)0, (T)1, first_aa != 0.0);
-  int n = 2;
-  for (; n + 3 < nstep; n += 4) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const T kn = (T)(n + k);
-      step(n + k, ((T)nstep - kn) * L.du, kn - (T)1, kn, false);
-    }
-    if (!__any(armed())) goto done;
-  }
-  for (; n < nstep; ++n) {
-    const T kn = (T)n;
-    step(n, ((T)nstep - kn) * L.du, kn - (T)1, kn, false);
-  }
-  {
-    // final approach: u += 0.9 du, weighted du/2 (:240-262)
-    const T kt = (T)nstep - (T)0.1;
-    step(nstep, L.du01, kt - (T)0.9, kt, false);
-    // captured: chord to the centre; the horizon is always entered
-    // (r > 3M > 2M) and the disc test sees n.(B - B) = 0 (:264-272)
-    if (captured && armed()) {
-      o.hit.cls = BH_HIT_HORIZON;
-      o.hit.ang = a;
-      o.hit.hx = (float)s.Rh;
-      o.hit.hy = 0.f;
-      end_ray(nstep);
-    }
-    if (!__any(armed())) goto done;
-    // outbound: u -= du (:274-296)
-    n = nstep + 1;
-    for (; n + 3 < 2 * nstep; n += 4) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const T j = (T)(n + k - nstep);
-        step(n + k, fma_t<T>(j, L.du, L.du01), kt - j + (T)1, kt - j, false);
-      }
-      if (!__any(armed())) goto done;
-    }
-    for (; n < 2 * nstep; ++n) {
-      const T j = (T)(n - nstep);
-      step(n, fma_t<T>(j, L.du, L.du01), kt - j + (T)1, kt - j, false);
-    }
-    if (ESCAPE) {
-      // extension: keep stepping outward until the next point would lie at
-      // r >= r_escape (u <= 1/r_escape), or the step cap is reached
-      for (; n <= s.step_cap; ++n) {
-        if (!__any(armed())) goto done;
-        const T j = (T)(n - nstep);
-        const double un = __builtin_fma((double)kt - (double)j, du64, s.u0);
-        if (armed() && un * s.r_esc <= 1.0) end_ray(n - 1);
-        step(n, fma_t<T>(j, L.du, L.du01), kt - j + (T)1, kt - j, false);
-      }
-    }
-    if (armed()) end_ray(n - 1);
-  }
-done:;
-}
-
-// Disc texel of a world point q (Rectangle::color, vector_object.h:163-179)
-// with r cos(theta) = s1.v/|s1| and r sin(theta) = |s1 x v|/|s1|.
-__device__ inline uint32_t disc_color32(const DevScene& s, float qx, float qy, float qz) {
-  const DevTex& t = s.disc_tex;
-  const float vx = qx - s.tv1[0], vy = qy - s.tv1[1], vz = qz - s.tv1[2];
-  const float a = (s.ts1[0] * vx + s.ts1[1] * vy + s.ts1[2] * vz) * s.tka;
-  const float cx = s.ts1[1] * vz - s.ts1[2] * vy;
-  const float cy = s.ts1[2] * vx - s.ts1[0] * vz;
-  const float cz = s.ts1[0] * vy - s.ts1[1] * vx;
-  const float b = __builtin_amdgcn_sqrtf(cx * cx + cy * cy + cz * cz) * s.tkb;
-  return texel(t, tex_index((double)a, t.cols), tex_index((double)b, t.rows));
-}
-
-__device__ inline uint32_t equirect32(const DevTex& t, float dx, float dy, float dz) {
-  const float kPi = 3.14159265358979323846f;
-  const float z = fminf(1.f, fmaxf(-1.f, dz));
-  const float lon = atan2f(dy, dx);
-  const float lat = asinf(z);
-  const float c = (lon + kPi) * (1.0f / (2.0f * kPi)) * (float)t.cols;
-  const float r = (kPi / 2.0f - lat) * (1.0f / kPi) * (float)t.rows;
-  return texel(t, tex_index((double)floorf(c), t.cols), tex_index((double)floorf(r), t.rows));
-}
-
-// Lane set-up in precision T from the FP64 quantities of the ray.
-template <typename T>
-__device__ inline void setup_lane(const DevScene& s, double l, double ib2, double du, float e2x,
-                                  float e2y, float e2z, float sgn, Lane<T>& L) {
-  const double M = s.M;
-  const double dl = 1.0 - 3.0 * M * l;
-  L.c0 = (T)__builtin_fma(l * l, __builtin_fma(2.0 * M, l, -1.0), ib2);
-  L.c1 = (T)(2.0 * l * dl);
-  L.c2 = (T)(1.0 - 2.0 * dl);
-  L.c3 = (T)(-2.0 * M);
-  L.du = (T)du;
-  L.h = (T)(0.5 * du);
-  L.du01 = (T)(0.1 * du);
-  L.u0 = (T)s.u0;
-  // smallest radius on the march: u = u0 + (nstep - 0.1) du
-  const float x = (float)(s.Rh * __builtin_fma((double)s.nstep - 0.1, du, s.u0));
-  L.phi_safe = x < 1.f ? 3.14159265f * (1.f - x) * 0.9999f : 0.f;
-  L.chi0 = 0.0;
-  L.a_ev = 1e300;
-  L.guard = 0.f;
-  if (s.has_disc) {
-    const float al = sgn * s.alpha_f;
-    const float be = s.nf[0] * e2x + s.nf[1] * e2y + s.nf[2] * e2z;
-    const float amp2 = al * al + be * be;
-    if (amp2 > 0.f) {
-      // al cos a + be sin a vanishes at a = atan2(be, al) + pi/2 (mod pi)
-      L.chi0 = (double)atan2_fast(be, al) + kPiD / 2.0;
-      L.guard = 4e-6f + 4e-7f * __builtin_amdgcn_rsqf(amp2);
-      L.a_ev = next_event(0.0, L.chi0, L.guard);
-    }
-  }
-  if (L.a_ev == 1e300) L.chi0 = 1e300;  // no crossings: parity tests stay even
-}
-
-// Per-ray set-up shared by the FP32 march and the FP64 band kernel.
-struct RaySetup {
-  double cc, dd, dF, ib2, l, first_aa;
-  float e2x, e2y, e2z, sgn;
-  bool degenerate, captured, band;
-};
-
-__device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bisect,
-                                 RaySetup& r) {
-  const double sx = s.W / 2.0 - x, sy = s.H / 2.0 - y;
-  const double pvx = __builtin_fma(-s.vz[0], sy, __builtin_fma(-s.vy[0], sx, s.fvB[0]));
-  const double pvy = __builtin_fma(-s.vz[1], sy, __builtin_fma(-s.vy[1], sx, s.fvB[1]));
-  const double pvz = __builtin_fma(-s.vz[2], sy, __builtin_fma(-s.vy[2], sx, s.fvB[2]));
-  const double Fx = s.F[0], Fy = s.F[1], Fz = s.F[2];
-  const double dvx = Fx - pvx, dvy = Fy - pvy, dvz = Fz - pvz;  // along yv
-  const double cvx = pvy * Fz - pvz * Fy;                        // pv x F, |cv| = b |dv|
-  const double cvy = pvz * Fx - pvx * Fz;
-  const double cvz = pvx * Fy - pvy * Fx;
-  const double cc = __builtin_fma(cvz, cvz, __builtin_fma(cvy, cvy, cvx * cvx));
-  const double dd = __builtin_fma(dvz, dvz, __builtin_fma(dvy, dvy, dvx * dvx));
-  const double dF = __builtin_fma(dvz, Fz, __builtin_fma(dvy, Fy, dvx * Fx));
-  r.cc = cc;
-  r.dd = dd;
-  r.dF = dF;
-  r.degenerate = !(cc > 0.0) || !(dd > 0.0);
-  r.sgn = dF > 0.0 ? 1.f : -1.f;
-  r.first_aa = dF > 0.0 ? 0.0 : kPiD;
-  r.captured = cc < dd * s.bc2;  // b < b_c
-  r.band = cc > dd * s.lo2 && cc < dd * s.hi2;
-  r.e2x = r.e2y = r.e2z = 0.f;
-  r.ib2 = 0.0;
-  r.l = s.Uc;
-  if (r.degenerate) return;
-  r.ib2 = dd / cc;  // 1 / b^2
-  // e2 = sign(Y0) ((dv.F) F - r0^2 dv) / (r0 |cv|)
-  const double k = (double)r.sgn * s.u0 * rsq<double>(cc);
-  r.e2x = (float)(__builtin_fma(dF, Fx, -s.r02 * dvx) * k);
-  r.e2y = (float)(__builtin_fma(dF, Fy, -s.r02 * dvy) * k);
-  r.e2z = (float)(__builtin_fma(dF, Fz, -s.r02 * dvz) * k);
-  if (r.captured) return;
-  // Turning point: StaticBlackhole::SolveG's lower bisection end.  The root
-  // u* = (1/(3M)) (1/2 + sin(asin(2q - 1)/3)), q = (b_c/b)^2, is refined by
-  // two Newton steps in FP64; the bisection cell containing it is then exact
-  // unless u* sits within 1e-5 cells of a cell edge, where (as in the
-  // photon-sphere band) the reference's bisection is rerun.
-  bool bisect = force_bisect || r.band || s.root == BH_ROOT_EXACT;
-  if (!bisect) {
-    const float q = fminf(1.f, (float)(s.bc2 * r.ib2));
-    float sn, cs;
-    sincos_core(asinf(2.f * q - 1.f) * (1.f / 3.f), &sn, &cs);
-    double u = s.Uc * (0.5 + (double)sn);
-    const double M2 = 2.0 * s.M, M6 = 6.0 * s.M;
-#pragma unroll
-    for (int it = 0; it < 2; ++it) {
-      const double g = __builtin_fma(u * u, __builtin_fma(M2, u, -1.0), r.ib2);
-      const double gp = u * __builtin_fma(M6, u, -2.0);
-      double ig = __builtin_amdgcn_rcp(gp);
-      ig = ig * __builtin_fma(-gp, ig, 2.0);
-      u = __builtin_fma(-g, ig, u);
-    }
-    const double t = (u - s.l0) * s.inv_cell;
-    const double kc = floor(t);
-    const double fr = t - kc;
-    if (fr > 1e-5 && fr < 1.0 - 1e-5 && kc >= 0.0 && kc < 1048576.0)
-      r.l = __builtin_fma(kc, s.cell, s.l0);
-    else
-      bisect = true;
-  }
-  if (bisect) r.l = solve_g64(sqrt(cc / dd), s.M, s.l0, s.root);
-}
-
-// Hit point, colour and output planes of one ray.
-template <bool ESCAPE>
-__device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t idx,
-                                  const RaySetup& r, const MarchOut& mo) {
-  const float e1x = r.sgn * s.e1f[0], e1y = r.sgn * s.e1f[1], e1z = r.sgn * s.e1f[2];
-  uint32_t rgb = 0;
-  float qx = __builtin_nanf(""), qy = qx, qz = qx;
-  if (mo.hit.cls != BH_HIT_NONE) {
-    if (!r.degenerate) {
-      float sa, ca;
-      sincos_angle(mo.hit.ang, &sa, &ca);
-      const float X = mo.hit.hx * ca - mo.hit.hy * sa;
-      const float Y = mo.hit.hx * sa + mo.hit.hy * ca;
-      qx = __builtin_fmaf(Y, r.e2x, __builtin_fmaf(X, e1x, s.Bf[0]));
-      qy = __builtin_fmaf(Y, r.e2y, __builtin_fmaf(X, e1y, s.Bf[1]));
-      qz = __builtin_fmaf(Y, r.e2z, __builtin_fmaf(X, e1z, s.Bf[2]));
-      if (mo.hit.cls == BH_HIT_DISC) rgb = disc_color32(s, qx, qy, qz);
-    }
-  } else if (ESCAPE && s.has_bg) {
-    // direction of the last chord; outbound points are one du apart, the
-    // last one at sigma = 0.1 + steps - nstep
-    const double kb = (double)s.nstep - (0.1 + (double)(mo.steps - s.nstep));
-    const float rb = 1.0f / (float)__builtin_fma(kb, mo.du64, s.u0);
-    const float ra = 1.0f / (float)__builtin_fma(kb + 1.0, mo.du64, s.u0);
-    float sb, cb, sa, ca;
-    sincos_angle(mo.a, &sb, &cb);
-    sincos_angle(mo.a - (double)mo.d, &sa, &ca);
-    const float X = rb * cb - ra * ca, Y = rb * sb - ra * sa;
-    const float dx = X * e1x + Y * r.e2x, dy = X * e1y + Y * r.e2y, dz = X * e1z + Y * r.e2z;
-    const float inv = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
-    rgb = equirect32(s.bg, dx * inv, dy * inv, dz * inv);
-  }
-  if (out.rgb) {
-    out.rgb[3 * idx + 0] = (uint8_t)(rgb & 0xff);
-    out.rgb[3 * idx + 1] = (uint8_t)((rgb >> 8) & 0xff);
-    out.rgb[3 * idx + 2] = (uint8_t)((rgb >> 16) & 0xff);
-  }
-  if (out.hit) out.hit[idx] = (uint8_t)mo.hit.cls;
-  if (out.steps) out.steps[idx] = mo.steps;
-  if (out.hitpt) {
-    out.hitpt[3 * idx + 0] = (double)qx;
-    out.hitpt[3 * idx + 1] = (double)qy;
-    out.hitpt[3 * idx + 2] = (double)qz;
-  }
-  if (out.phi) out.phi[idx] = r.degenerate ? mo.a : atan(sqrt(r.cc) / r.dF) + mo.a;
-}
-
-__device__ inline void degenerate_out(MarchOut& mo) {
-  // the pixel line passes through B (quirk Q13): captured / black
-  mo.hit.cls = BH_HIT_HORIZON;
-  mo.hit.ang = 0.0;
-  mo.hit.hx = mo.hit.hy = 0.f;
-  mo.steps = 0;
-  mo.a = __builtin_nan("");
-  mo.du64 = 0.0;
-  mo.d = 0.f;
 }
 
 template <bool ESCAPE>

@@ -93,6 +93,57 @@ class Renderer:
                                             C.byref(m), C.byref(t), C.byref(n)))
         return m.value, t.value, n.value
 
+    # -- batches ------------------------------------------------------------
+    def render_batch(self, items, want=("rgb", "hit"), tuning: abi.BatchTuning | None = None,
+                     timing: bool = False):
+        """Render many (scene, options) pairs in one persistent launch
+        (bh_render_batch_device).  Returns a list of dicts of numpy planes;
+        with timing, also (scene_ms array, stats)."""
+        import torch
+
+        n = len(items)
+        scenes = (abi.Scene * n)(*[s for s, _ in items])
+        opts = (abi.Options * n)(*[o for _, o in items])
+        outs = (abi.Outputs * n)()
+        px = [s.camera.width * s.camera.height for s, _ in items]
+        off = np.concatenate([[0], np.cumsum(px)]).astype(np.int64)
+        total = int(off[-1])
+        dev = torch.device("cuda", 0)
+        bufs = {}
+        if "rgb" in want:
+            bufs["rgb"] = torch.empty(total * 3, dtype=torch.uint8, device=dev)
+        if "hit" in want:
+            bufs["hit"] = torch.empty(total, dtype=torch.uint8, device=dev)
+        if "steps" in want:
+            bufs["steps"] = torch.empty(total, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize(dev)
+        for i in range(n):
+            o = int(off[i])
+            outs[i].rgb = bufs["rgb"].data_ptr() + 3 * o if "rgb" in bufs else None
+            outs[i].hit = bufs["hit"].data_ptr() + o if "hit" in bufs else None
+            outs[i].steps = bufs["steps"].data_ptr() + 4 * o if "steps" in bufs else None
+        ms = (C.c_double * n)()
+        st = abi.BatchStats()
+        self._check(self.lib.bh_render_batch_device(
+            self.ctx, n, scenes, opts, outs, C.byref(tuning) if tuning is not None else None,
+            ms, C.byref(st)))
+        host = {k: v.cpu().numpy() for k, v in bufs.items()}
+        res = []
+        for i, (s, _) in enumerate(items):
+            w, h = s.camera.width, s.camera.height
+            d = {}
+            a, b = int(off[i]), int(off[i + 1])
+            if "rgb" in host:
+                d["rgb"] = host["rgb"][3 * a:3 * b].reshape(h, w, 3)
+            if "hit" in host:
+                d["hit"] = host["hit"][a:b].reshape(h, w)
+            if "steps" in host:
+                d["steps"] = host["steps"][a:b].reshape(h, w)
+            res.append(d)
+        if timing:
+            return res, np.array(ms[:], dtype=np.float64), st
+        return res
+
     def path(self, scene: abi.Scene, opts: abi.Options, cameras) -> "Path":
         return Path(self, scene, opts, cameras)
 

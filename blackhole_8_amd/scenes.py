@@ -217,6 +217,32 @@ def config2(width: int = 1280, height: int = 720, precision=abi.PREC_FP32,
     return Preset("config2", s, o, tex)
 
 
+def config3(n: int = 512, seed: int = SEED_BASE + 0x300):
+    """Batched uneven renders: n independent hole (+ disc with probability
+    1/2) scenes, square side log-uniform in [32, 1024], r_s ~ U[0.5, 2],
+    camera at U[6, 50] r_s, inclination U[0, 90] deg, azimuth U[0, 360) deg,
+    fov 60 deg, disc 3..12 r_s with the 1080p workload's texture, FP32.
+    Returns (list of (scene, options), textures)."""
+    rng = Rng(seed)
+    items = []
+    for _ in range(n):
+        side = int(round(math.exp(rng.uniform(math.log(32.0), math.log(1024.0)))))
+        r_s = rng.uniform(0.5, 2.0)
+        dist = rng.uniform(6.0, 50.0) * r_s
+        incl = max(1e-3, rng.uniform(0.0, 90.0))
+        azim = rng.uniform(0.0, 360.0)
+        disc = rng.uniform(0.0, 1.0) < 0.5
+        cam = orbit_camera(dist, incl, azim, side, side, math.pi / 3)
+        s = _scene(cam, mass=r_s / 2.0)
+        if disc:
+            r_out, r_in = 12.0 * r_s, 3.0 * r_s
+            s.has_disc = 1
+            s.disc = annulus((r_out, r_out, 0.0), (-r_out, r_out, 0.0), (-r_out, -r_out, 0.0),
+                             (r_out, -r_out, 0.0), r_out, r_in, SLOT_DISC)
+        items.append((s, options(precision=abi.PREC_FP32, escape=False, band=1e-3)))
+    return items, {SLOT_DISC: gen_disc_texture()}
+
+
 def config4_plain(theta, width: int = 512, height: int = 512, fov: float = math.radians(70.0),
                   r_outer: float = 12.0, azim: float = 0.0,
                   precision=abi.PREC_FP32) -> Preset:

@@ -254,6 +254,51 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                    int64_t* band_rays);
 
 /* ======================================================================== */
+/* Batches: many independent scenes in one persistent launch (no reference  */
+/* counterpart: the reference renders one scene at a time,                  */
+/* blackhole_solution_test.cc:161-298)                                      */
+/* ======================================================================== */
+/* Scheduling knobs of the persistent batch kernel (0 = default). */
+typedef struct bh_batch_tuning {
+  int32_t block;          /* threads per workgroup: 64, 128 or 256 (256)       */
+  int32_t waves_per_simd; /* persistent waves per SIMD, 1..8 (4)              */
+  int32_t refill_steps;   /* steps between refill checks, 1..64 (4)            */
+  int32_t refill_idle;    /* 64 (default): each wave marches whole 8x8 tiles;
+                             below 64: lanes refill individually once this many
+                             of a wave's 64 lanes are idle (ray compaction)    */
+  int32_t chunk;          /* least 8x8 tiles per queue pop, 1..1024 (4); pops
+                             take max(chunk, remaining / (4 waves))            */
+} bh_batch_tuning;
+
+typedef struct bh_batch_stats {
+  double total_ms;    /* upload of the batch tables to completion (HIP events) */
+  double march_ms;    /* persistent FP32 march                                 */
+  double band_ms;     /* FP64 photon-sphere band pass                          */
+  int64_t rays;       /* pixels rendered                                       */
+  int64_t band_rays;  /* rays marched by the band pass                         */
+  int32_t solo_scenes;/* scenes outside the FP32 march, rendered one by one    */
+  int32_t pad_;
+} bh_batch_stats;
+
+/* Render n scenes; scene i with options[i] into outs[i] (device pointers,
+ * planes as for bh_render_device).  Every scene's output equals its own
+ * bh_render_device output.  Scenes the FP32 march does not take (FP64,
+ * spheres, bilinear) are rendered one by one after the batch.  scene_ms
+ * (host, n doubles, optional) receives each scene's completion time from the
+ * start of the march; with scene_ms or stats the call waits for completion,
+ * otherwise it is asynchronous on the context stream. */
+int bh_render_batch_device(bh_ctx* ctx, int n, const bh_scene* scenes,
+                           const bh_options* options, const bh_outputs* outs,
+                           const bh_batch_tuning* tune, double* scene_ms,
+                           bh_batch_stats* stats);
+/* Time each of n_cand tunings on this batch (best of reps runs, march plus
+ * band pass) and return the fastest in *best (its time in *best_ms;
+ * cand_ms, optional, receives every candidate's time). */
+int bh_batch_autotune(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* options,
+                      const bh_outputs* outs, const bh_batch_tuning* cand, int n_cand, int reps,
+                      bh_batch_tuning* best, double* best_ms, double* cand_ms);
+
+/* ======================================================================== */
 /* Differentiable render and its adjoint (no reference counterpart: the     */
 /* reference renders only; these serve the inverse fit of the four          */
 /* parameters of bh_diff_params)                                            */

@@ -41,6 +41,7 @@ STRUCTS = {
     "bh_annulus": abi.Annulus, "bh_sphere": abi.Sphere, "bh_scene": abi.Scene,
     "bh_options": abi.Options, "bh_outputs": abi.Outputs, "bh_rect": abi.Rect,
     "bh_plane": abi.Plane, "bh_flat_scene": abi.FlatScene, "bh_diff_params": abi.DiffParams,
+    "bh_batch_tuning": abi.BatchTuning, "bh_batch_stats": abi.BatchStats,
 }
 
 
