@@ -238,16 +238,20 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
     d->tka = (float)(d->disc_tex.cols / (l1 * l1));
     d->tkb = (float)(d->disc_tex.rows / (l1 * l2));
   }
-  d->fast_ok = d->precision == BH_PREC_FP32 && d->n_spheres == 0 &&
-               d->filter == BH_FILTER_NEAREST && d->nstep <= 4096 &&
+  for (int i = 0; i < d->n_spheres; ++i) {
+    for (int k = 0; k < 3; ++k) d->sphf[i][k] = (float)(d->sc[i][k] - d->B[k]);
+    d->sphf[i][3] = (float)d->sr[i];
+  }
+  d->fast_ok = d->precision == BH_PREC_FP32 && d->filter == BH_FILTER_NEAREST &&
+               d->nstep <= 4096 &&
                (!d->has_disc || (d->dv0[0] == d->B[0] && d->dv0[1] == d->B[1] &&
                                  d->dv0[2] == d->B[2]));
   return BH_OK;
 }
 
 // The FP32 event march assumes the disc is centred on the hole (its plane
-// passes through B, so every crossing is an angle in the orbit plane) and
-// renders no spheres yet; other scenes take the FP64 march.
+// passes through B, so every crossing is an angle in the orbit plane); other
+// scenes take the FP64 march.
 bool fp32_eligible(const DevScene& d) { return d.fast_ok != 0; }
 
 // FP32 march then the FP64 band kernel, or the FP64 reference march.
@@ -256,8 +260,9 @@ int launch_frame(bh_ctx* ctx, const DevScene& host_scene, DevScene* d_scene, con
   const int W = host_scene.W, H = host_scene.H;
   if (fp32_eligible(host_scene)) {
     const bool esc = host_scene.escape != 0;
-    HIP_TRY(ctx, bh::launch_march_fp32(d_scene, out, W, H, q, st, esc));
-    HIP_TRY(ctx, bh::launch_band(d_scene, out, q, st, esc));
+    const bool sph = host_scene.n_spheres > 0;
+    HIP_TRY(ctx, bh::launch_march_fp32(d_scene, out, W, H, q, st, esc, sph));
+    HIP_TRY(ctx, bh::launch_band(d_scene, out, q, st, esc, sph));
   } else {
     HIP_TRY(ctx, bh::launch_march_fp64(d_scene, out, W, H, st));
   }
@@ -291,7 +296,7 @@ struct BatchPlan {
 };
 
 int plan_batch(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* options,
-               const bh_outputs* outs, BatchPlan* p) {
+               const bh_outputs* outs, bool lane_mode, BatchPlan* p) {
   if (n <= 0 || !scenes || !options || !outs)
     return fail(ctx, BH_ERR_INVALID_ARG, "batch needs n > 0 and scene, option and output arrays");
   p->scenes.resize(n);
@@ -306,7 +311,8 @@ int plan_batch(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* opt
     }
     const bh_outputs& o = outs[i];
     p->outs[i] = DevOut{o.rgb, o.hit, o.phi_end, o.hit_point, o.steps};
-    if (fp32_eligible(p->scenes[i])) {
+    // the lane-refill mode keeps no sphere table per lane: such scenes go solo
+    if (fp32_eligible(p->scenes[i]) && !(lane_mode && p->scenes[i].n_spheres > 0)) {
       const DevScene& d = p->scenes[i];
       rays += (uint64_t)((d.W + 7) / 8) * (uint64_t)((d.H + 7) / 8) * 64u;
       if (rays >= ((uint64_t)1 << 31)) return fail(ctx, BH_ERR_INVALID_ARG, "batch larger than 2^31 rays");
@@ -730,7 +736,7 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
   for (int it = 0; it < iters; ++it) {
     if (fast) {
-      HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0));
+      HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0, d.n_spheres > 0));
     } else {
       HIP_TRY(ctx, bh::launch_march_fp64(ds, o, d.W, d.H, st));
     }
@@ -811,10 +817,11 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
     const DevScene& f = frames[(size_t)i];
     if (fp32_eligible(f)) {
       const bool esc = f.escape != 0;
-      hipError_t le = bh::launch_march_fp32(p->d_scenes + i, o, p->W, p->H, q, st, esc);
+      const bool sph = f.n_spheres > 0;
+      hipError_t le = bh::launch_march_fp32(p->d_scenes + i, o, p->W, p->H, q, st, esc, sph);
       if (le == hipSuccess) le = hipEventRecord(p->events[(size_t)i], st);
       if (le == hipSuccess) le = hipStreamWaitEvent(side, p->events[(size_t)i], 0);
-      if (le == hipSuccess) le = bh::launch_band(p->d_scenes + i, o, q, side, esc);
+      if (le == hipSuccess) le = bh::launch_band(p->d_scenes + i, o, q, side, esc, sph);
       if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
       used_side = true;
     } else {
@@ -965,7 +972,7 @@ int bh_render_batch_device(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_
   int rc = check_tuning(ctx, t);
   if (rc) return rc;
   BatchPlan p;
-  if ((rc = plan_batch(ctx, n, scenes, options, outs, &p))) return rc;
+  if ((rc = plan_batch(ctx, n, scenes, options, outs, t.refill_idle < 64, &p))) return rc;
   (void)hipSetDevice(ctx->device);
   bh::BatchDev b;
   if ((rc = run_batch(ctx, p, t, &b))) return rc;
@@ -1009,15 +1016,22 @@ int bh_batch_autotune(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_optio
   if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
   if (!cand || n_cand <= 0 || !best) return fail(ctx, BH_ERR_INVALID_ARG, "need candidates and best");
   if (reps <= 0) reps = 1;
-  BatchPlan p;
-  int rc = plan_batch(ctx, n, scenes, options, outs, &p);
-  if (rc) return rc;
+  // one plan per mode (the lane-refill mode renders sphere scenes solo)
+  BatchPlan plans[2];
+  bool planned[2] = {false, false};
   (void)hipSetDevice(ctx->device);
   double bms = 1e300;
   int bi = -1;
+  int rc;
   for (int c = 0; c < n_cand; ++c) {
     const bh_batch_tuning t = normalized_tuning(&cand[c]);
     if ((rc = check_tuning(ctx, t))) return rc;
+    const int m = t.refill_idle < 64 ? 1 : 0;
+    if (!planned[m]) {
+      if ((rc = plan_batch(ctx, n, scenes, options, outs, m == 1, &plans[m]))) return rc;
+      planned[m] = true;
+    }
+    const BatchPlan& p = plans[m];
     double cms = 1e300;
     for (int r = 0; r < reps + 1; ++r) {  // first run warms up
       if ((rc = run_batch(ctx, p, t, nullptr))) return rc;

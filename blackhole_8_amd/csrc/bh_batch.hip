@@ -293,7 +293,7 @@ __device__ inline void run_lanes(const BatchDev& b, uint32_t waves) {
 // the per-frame kernel's wave-uniform loop (scene constants in scalar
 // registers).  Returns whether this lane's ray was finished here (not queued
 // for the band pass).
-template <bool ESCAPE>
+template <bool ESCAPE, bool SPH>
 __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t sc,
                                   uint32_t id) {
   int x, y;
@@ -313,7 +313,7 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
     Lane<float> L;
     const double du = (r.l - s.u0) * (1.0 / s.nstep);
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
-    march<float, ESCAPE>(s, L, r.captured, r.first_aa, du, mo);
+    march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
   }
   finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo);
   return true;
@@ -335,7 +335,12 @@ __device__ inline void run_tiles(const BatchDev& b, const DevScene* __restrict__
       const uint32_t sc = (uint32_t)__builtin_amdgcn_readfirstlane((int)scene_of(b, id0));
       const DevScene& s = scenes[sc];
       const uint32_t id = id0 + (uint32_t)lane_id();
-      const bool fin = s.escape ? march_tile<true>(b, s, sc, id) : march_tile<false>(b, s, sc, id);
+      bool fin;
+      if (s.n_spheres > 0)
+        fin = s.escape ? march_tile<true, true>(b, s, sc, id) : march_tile<false, true>(b, s, sc, id);
+      else
+        fin = s.escape ? march_tile<true, false>(b, s, sc, id)
+                       : march_tile<false, false>(b, s, sc, id);
       note_finish(b, f, fin, sc);
     }
   }
@@ -372,11 +377,18 @@ __global__ __launch_bounds__(64) void batch_band_kernel(BatchDev b) {
     Lane<double> L;
     const double du = (r.l - s.u0) * (1.0 / s.nstep);
     setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
+    const bool sph = s.n_spheres > 0;
     if (s.escape) {
-      march<double, true>(s, L, r.captured, r.first_aa, du, mo);
+      if (sph)
+        march<double, true, true>(s, L, r.captured, r.first_aa, du, mo);
+      else
+        march<double, true, false>(s, L, r.captured, r.first_aa, du, mo);
       finish_ray<true>(s, b.outs[sc], idx, r, mo);
     } else {
-      march<double, false>(s, L, r.captured, r.first_aa, du, mo);
+      if (sph)
+        march<double, false, true>(s, L, r.captured, r.first_aa, du, mo);
+      else
+        march<double, false, false>(s, L, r.captured, r.first_aa, du, mo);
       finish_ray<false>(s, b.outs[sc], idx, r, mo);
     }
   }

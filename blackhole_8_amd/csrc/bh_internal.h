@@ -60,6 +60,7 @@ struct DevScene {
   float Bf[3];
   float tv1[3], ts1[3];   // disc texture square: p1 and s1 = p2 - p1
   float tka, tkb;         // cols / |s1|^2 and rows / (|s1| |s2|)
+  float sphf[BH_MAX_SPHERES][4];  // sphere centre relative to B, radius
   int32_t fast_ok;        // the FP32 march applies to this scene
   int32_t pad1;
 };
@@ -86,9 +87,9 @@ struct DevQueue {
 hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, int H,
                              hipStream_t stream);
 hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, int H,
-                             const DevQueue& q, hipStream_t stream, bool escape);
+                             const DevQueue& q, hipStream_t stream, bool escape, bool spheres);
 hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,
-                       hipStream_t stream, bool escape);
+                       hipStream_t stream, bool escape, bool spheres);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
 
 // ---------------------------------------------------------------------------

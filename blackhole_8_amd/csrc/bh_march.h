@@ -44,10 +44,15 @@ struct Lane {
   float guard;        // disc crossing-angle guard
   double chi0;        // a disc crossing angle (crossings at chi0 + j pi)
   double a_ev;        // next disc event angle (crossing - guard)
+  // scenes with spheres only
+  float e2[3], nrm[3];  // in-plane basis vector e2 and the orbit-plane normal
+  float sgn;            // sign(Y0): e1 = sgn F/|F|
+  uint32_t smask;       // spheres that cut the orbit plane
 };
 
 struct HitRec {
   int cls;       // BH_HIT_*
+  int obj;       // sphere index of an object hit
   double ang;    // rotation of the frame the hit point is expressed in
   float hx, hy;  // hit point relative to B in that frame
 };
@@ -57,7 +62,7 @@ struct HitRec {
 // S is DevScene or any type with its fields Rh, r_in2 and r_out2.
 template <typename T, typename S>
 __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa, T kb,
-                                   double ab, T da, bool th, bool td, HitRec& hit) {
+                                   double ab, T da, bool th, bool td, HitRec& hit, T* d2out) {
   const T ra = rcp<T>(fma_t<T>(ka, L.du, L.u0));
   const T rb = rcp<T>(fma_t<T>(kb, L.du, L.u0));
   bool found = false;
@@ -107,12 +112,67 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
         const T d2 = dx * dx + dy * dy;
         if (!found || d2 < best) {
           found = true;
+          best = d2;
           hit.cls = BH_HIT_DISC;
           hit.ang = chik;
           hit.hx = (float)px;
           hit.hy = (float)py;
         }
       }
+    }
+  }
+  *d2out = best;
+  return found;
+}
+
+// Spheres (bit i of L.smask: sphere i cuts the orbit plane).  A sphere of
+// radius R whose centre lies at distance z from the plane cuts it in a circle
+// of radius sqrt(R^2 - z^2), and |P - c|^2 = |P - c'|^2 + z^2 for a point P
+// of the plane and the centre's projection c', so the reference's
+// segment/sphere test (blackhole_solution.h:65-88, the same test as the
+// horizon's) of an in-plane chord is exactly this circle test in the plane.
+// The chord from (ka, aa) to (kb, aa + da) is tested in the frame rotated by
+// aa; a hit nearer to the chord start than best (found) replaces hit.
+template <typename T>
+__device__ inline bool sphere_chord(const DevScene& s, const Lane<T>& L, T ka, double aa, T kb,
+                                    T da, bool found, T best, HitRec& hit) {
+  const T ra = rcp<T>(fma_t<T>(ka, L.du, L.u0));
+  const T rb = rcp<T>(fma_t<T>(kb, L.du, L.u0));
+  T sd, cd;
+  sincos_t<T>(da, &sd, &cd);
+  const T bx = rb * cd, by = rb * sd;
+  const T qx = bx - ra, qy = by;
+  const T qq = qx * qx + qy * qy;
+  float sa, ca;
+  sincos_angle(aa, &sa, &ca);
+  const float e1x = L.sgn * s.e1f[0], e1y = L.sgn * s.e1f[1], e1z = L.sgn * s.e1f[2];
+  for (int i = 0; i < s.n_spheres; ++i) {
+    if (!((L.smask >> i) & 1u)) continue;
+    const float cx = s.sphf[i][0], cy = s.sphf[i][1], cz = s.sphf[i][2], R = s.sphf[i][3];
+    const float X = cx * e1x + cy * e1y + cz * e1z;
+    const float Y = cx * L.e2[0] + cy * L.e2[1] + cz * L.e2[2];
+    const float Z = cx * L.nrm[0] + cy * L.nrm[1] + cz * L.nrm[2];
+    const T rho2 = (T)(R * R - Z * Z);
+    // centre in the rotated frame, then P_a - c' and P_b - c'
+    const T xc = (T)(X * ca + Y * sa), yc = (T)(Y * ca - X * sa);
+    const T ax = ra - xc, ay = -yc;
+    const T q1 = ax * ax + ay * ay;
+    const T q2 = (bx - xc) * (bx - xc) + (by - yc) * (by - yc);
+    if (q1 < rho2 && q2 < rho2) continue;
+    const T qd = qx * ax + qy * ay;
+    const T cc = qd * qd - qq * (q1 - rho2);
+    if (!(cc > (T)0)) continue;
+    const T t = (-qd - sqrt_t<T>(cc)) * rcp<T>(qq);
+    if (!(t > (T)0 && t < (T)1)) continue;
+    const T d2 = t * t * qq;
+    if (!found || d2 < best) {
+      found = true;
+      best = d2;
+      hit.cls = BH_HIT_OBJECT;
+      hit.obj = i;
+      hit.ang = aa;
+      hit.hx = (float)(ra + t * qx);
+      hit.hy = (float)(t * qy);
     }
   }
   return found;
@@ -182,7 +242,9 @@ template <typename T> struct MarchState {
 // One step: trapezoid update of the swept angle, then the exact chord tests
 // if the step is an event (a crossing angle passed, the horizon guard
 // exceeded, or force).  Returns true when the chord hits (hit is filled).
-template <typename T, typename S>
+// With SPH the chord is also tested against the spheres of L.smask, at every
+// step.
+template <typename T, typename S, bool SPH = false>
 __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>& st,
                                            double first_aa, T sv, T ka, T kb, bool force,
                                            HitRec& hit) {
@@ -192,14 +254,19 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
   st.w = wn;
   st.a += (double)st.d;
   const bool eh = (float)st.d > L.phi_safe;
+  const double aa = force ? first_aa : st.a - (double)st.d;
+  const T da = force ? (T)(st.a - first_aa) : st.d;
+  bool found = false;
+  T best = (T)0;
   if (eh || st.a >= L.a_ev || force) {
-    const double aa = force ? first_aa : st.a - (double)st.d;
-    const T da = force ? (T)(st.a - first_aa) : st.d;
-    if (event_chord<T, S>(s, L, ka, aa, kb, st.a, da, eh || force, s.has_disc != 0, hit))
-      return true;
-    if (L.chi0 != 1e300) L.a_ev = next_event(st.a, L.chi0, L.guard);
+    found = event_chord<T, S>(s, L, ka, aa, kb, st.a, da, eh || force, s.has_disc != 0, hit,
+                              &best);
+    if (!found && L.chi0 != 1e300) L.a_ev = next_event(st.a, L.chi0, L.guard);
   }
-  return false;
+  if constexpr (SPH) {
+    if (L.smask) found = sphere_chord<T>(s, L, ka, aa, kb, da, found, best, hit);
+  }
+  return found;
 }
 
 // The march of one ray in precision T.  first_aa is the angle of the camera
@@ -209,16 +276,17 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
 //
 // Every lane of a wave is at the same step, so the loops run wave-uniform
 // trip counts and keep no per-lane flag: a ray that has ended is disarmed by
-// setting its event thresholds to +inf (its results were recorded when it
-// ended; it keeps stepping harmlessly), and every four steps the wave leaves
-// once no lane is armed.  This keeps the exec-mask bookkeeping of the hot
+// setting its event thresholds to +inf and clearing its sphere mask (its
+// results were recorded when it ended; it keeps stepping harmlessly), and
+// every four steps the wave leaves once no lane is armed.  This keeps the exec-mask bookkeeping of the hot
 // loop to a handful of scalar instructions.
-template <typename T, bool ESCAPE>
+template <typename T, bool ESCAPE, bool SPH = false>
 __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, double first_aa,
                              double du64, MarchOut& o) {
   const int nstep = s.nstep;
   MarchState<T> st{(T)0, 0.0, (T)0};
   o.hit.cls = BH_HIT_NONE;
+  o.hit.obj = 0;
   o.hit.ang = 0.0;
   o.hit.hx = o.hit.hy = 0.f;
   o.du64 = du64;
@@ -232,10 +300,11 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     o.d = (float)st.d;
     L.phi_safe = kInf;
     L.a_ev = 1e308;
+    L.smask = 0u;  // no sphere tests either once ended
   };
   auto armed = [&]() { return L.phi_safe < kInf; };
   auto step = [&](int n, T sv, T ka, T kb, bool force) {
-    if (march_step<T, DevScene>(s, L, st, first_aa, sv, ka, kb, force, o.hit)) end_ray(n);
+    if (march_step<T, DevScene, SPH>(s, L, st, first_aa, sv, ka, kb, force, o.hit)) end_ray(n);
   };
   auto step_in = [&](int n, bool force) {
     T sv, ka, kb;
@@ -353,6 +422,22 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, doubl
     }
   }
   if (L.a_ev == 1e300) L.chi0 = 1e300;  // no crossings: parity tests stay even
+  // spheres: orbit-plane basis and the spheres that cut the plane
+  L.sgn = sgn;
+  L.e2[0] = e2x;
+  L.e2[1] = e2y;
+  L.e2[2] = e2z;
+  const float e1x = sgn * s.e1f[0], e1y = sgn * s.e1f[1], e1z = sgn * s.e1f[2];
+  L.nrm[0] = e1y * e2z - e1z * e2y;
+  L.nrm[1] = e1z * e2x - e1x * e2z;
+  L.nrm[2] = e1x * e2y - e1y * e2x;
+  L.smask = 0u;
+  for (int i = 0; i < s.n_spheres; ++i) {
+    const float Z = s.sphf[i][0] * L.nrm[0] + s.sphf[i][1] * L.nrm[1] + s.sphf[i][2] * L.nrm[2];
+    // margin for the FP32 normal: a sphere within 1e-5 R of the plane counts
+    const float R = s.sphf[i][3] * 1.00001f;
+    if (Z * Z < R * R) L.smask |= 1u << i;
+  }
 }
 
 // Per-ray set-up shared by the FP32 march and the FP64 band kernel.
@@ -442,7 +527,15 @@ __device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t i
       qx = __builtin_fmaf(Y, r.e2x, __builtin_fmaf(X, e1x, s.Bf[0]));
       qy = __builtin_fmaf(Y, r.e2y, __builtin_fmaf(X, e1y, s.Bf[1]));
       qz = __builtin_fmaf(Y, r.e2z, __builtin_fmaf(X, e1z, s.Bf[2]));
-      if (mo.hit.cls == BH_HIT_DISC) rgb = disc_color32(s, qx, qy, qz);
+      if (mo.hit.cls == BH_HIT_DISC) {
+        rgb = disc_color32(s, qx, qy, qz);
+      } else if (mo.hit.cls == BH_HIT_OBJECT) {
+        // sphere texel: equirectangular lookup of the outward normal
+        const float* c = s.sphf[mo.hit.obj];
+        const float iR = 1.0f / c[3];
+        rgb = equirect32(s.stex[mo.hit.obj], (qx - s.Bf[0] - c[0]) * iR,
+                         (qy - s.Bf[1] - c[1]) * iR, (qz - s.Bf[2] - c[2]) * iR);
+      }
     }
   } else if (ESCAPE && s.has_bg) {
     // direction of the last chord; outbound points are one du apart, the
@@ -476,6 +569,7 @@ __device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t i
 __device__ inline void degenerate_out(MarchOut& mo) {
   // the pixel line passes through B (quirk Q13): captured / black
   mo.hit.cls = BH_HIT_HORIZON;
+  mo.hit.obj = 0;
   mo.hit.ang = 0.0;
   mo.hit.hx = mo.hit.hy = 0.f;
   mo.steps = 0;

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void march64_kernel(const DevScene* __restrict
   store_pixel64(out, (size_t)y * s.W + x, px);
 }
 
-template <bool ESCAPE>
+template <bool ESCAPE, bool SPH>
 __global__ __launch_bounds__(256) void march_kernel(const DevScene* __restrict__ sp, DevOut out,
                                                     DevQueue q) {
   const DevScene& s = *sp;
@@ -50,14 +50,14 @@ __global__ __launch_bounds__(256) void march_kernel(const DevScene* __restrict__
     Lane<float> L;
     const double du = (r.l - s.u0) * (1.0 / s.nstep);
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
-    march<float, ESCAPE>(s, L, r.captured, r.first_aa, du, mo);
+    march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
   }
   finish_ray<ESCAPE>(s, out, idx, r, mo);
 }
 
 // FP64 march of the photon-sphere band rays queued by march_kernel.  The last
 // workgroup to finish clears the queue for the next frame.
-template <bool ESCAPE>
+template <bool ESCAPE, bool SPH>
 __global__ __launch_bounds__(64) void band_kernel(const DevScene* __restrict__ sp, DevOut out,
                                                   DevQueue q) {
   const DevScene& s = *sp;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(64) void band_kernel(const DevScene* __restrict__ s
     Lane<double> L;
     const double du = (r.l - s.u0) * (1.0 / s.nstep);
     setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
-    march<double, ESCAPE>(s, L, r.captured, r.first_aa, du, mo);
+    march<double, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
     finish_ray<ESCAPE>(s, out, idx, r, mo);
   }
   __syncthreads();
@@ -101,23 +101,37 @@ hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, in
 }
 
 hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, int H,
-                             const DevQueue& q, hipStream_t stream, bool escape) {
+                             const DevQueue& q, hipStream_t stream, bool escape, bool spheres) {
   dim3 grid((W + 15) / 16, (H + 15) / 16);
-  if (escape)
-    hipLaunchKernelGGL(march_kernel<true>, grid, dim3(256), 0, stream, scene, out, q);
-  else
-    hipLaunchKernelGGL(march_kernel<false>, grid, dim3(256), 0, stream, scene, out, q);
+  if (spheres) {
+    if (escape)
+      hipLaunchKernelGGL((march_kernel<true, true>), grid, dim3(256), 0, stream, scene, out, q);
+    else
+      hipLaunchKernelGGL((march_kernel<false, true>), grid, dim3(256), 0, stream, scene, out, q);
+  } else {
+    if (escape)
+      hipLaunchKernelGGL((march_kernel<true, false>), grid, dim3(256), 0, stream, scene, out, q);
+    else
+      hipLaunchKernelGGL((march_kernel<false, false>), grid, dim3(256), 0, stream, scene, out, q);
+  }
   return hipGetLastError();
 }
 
 // Band grid: 64 workgroups of one wave; every workgroup reaches the queue
 // reset, so the queue is clear when the launch completes.
 hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,
-                       hipStream_t stream, bool escape) {
-  if (escape)
-    hipLaunchKernelGGL(band_kernel<true>, dim3(64), dim3(64), 0, stream, scene, out, q);
-  else
-    hipLaunchKernelGGL(band_kernel<false>, dim3(64), dim3(64), 0, stream, scene, out, q);
+                       hipStream_t stream, bool escape, bool spheres) {
+  if (spheres) {
+    if (escape)
+      hipLaunchKernelGGL((band_kernel<true, true>), dim3(64), dim3(64), 0, stream, scene, out, q);
+    else
+      hipLaunchKernelGGL((band_kernel<false, true>), dim3(64), dim3(64), 0, stream, scene, out, q);
+  } else {
+    if (escape)
+      hipLaunchKernelGGL((band_kernel<true, false>), dim3(64), dim3(64), 0, stream, scene, out, q);
+    else
+      hipLaunchKernelGGL((band_kernel<false, false>), dim3(64), dim3(64), 0, stream, scene, out, q);
+  }
   return hipGetLastError();
 }
 

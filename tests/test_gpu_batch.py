@@ -102,6 +102,11 @@ def test_batch_with_fp64_and_sphere_scenes(renderer, c3):
     mixed = [items[1], (p2.scene, p2.options), (s64, o64), items[2]]
     print("[case] mixed batch: FP32, spheres, FP64", flush=True)
     res, ms, st = renderer.render_batch(mixed, timing=True)
+    assert st.solo_scenes == 1  # the FP64 scene; the sphere scene is in the FP32 march
+    _assert_same_as_solo(renderer, mixed, res)
+    # the lane-refill mode keeps no sphere table: there the sphere scene goes solo
+    res, ms, st = renderer.render_batch(mixed, timing=True,
+                                        tuning=abi.BatchTuning(refill_idle=16))
     assert st.solo_scenes == 2
     _assert_same_as_solo(renderer, mixed, res)
 

@@ -108,6 +108,40 @@ def test_fp32_config1_other_frames(renderer):
         _check_fp32(renderer, base.with_camera(base.cameras[k]), f"fp32 config1 640x360 frame {k}")
 
 
+def _check_config2(renderer, p, name):
+    g = renderer.render_preset(p, ALL)
+    r = oracle.render_preset(p, THREADS, ALL)
+    st = compare(g, r)
+    _log(name, st)
+    for cls in (abi.HIT_HORIZON, abi.HIT_OBJECT, abi.HIT_NONE):
+        assert (r["hit"] == cls).sum() > 100, cls
+    return st
+
+
+def test_fp32_config2_spheres_match_oracle(renderer):
+    print("[case] fp32 config2 640x360 (16 spheres, escape, background)", flush=True)
+    st = _check_config2(renderer, scenes.config2(640, 360), "fp32 config2 640x360")
+    assert st["class_mismatch_frac"] <= 1e-4
+    assert st["rgb_mismatch_frac"] <= 2e-3
+    assert st["phi_max_err"] <= 1e-5
+
+
+def test_fp32_config2_720p_matches_oracle(renderer):
+    print("[case] fp32 config2 1280x720", flush=True)
+    st = _check_config2(renderer, scenes.config2(), "fp32 config2 1280x720")
+    assert st["class_mismatch_frac"] <= 1e-4
+    assert st["rgb_mismatch_frac"] <= 2e-3
+
+
+def test_fp64_config2_matches_oracle(renderer):
+    print("[case] fp64 config2 320x180", flush=True)
+    st = _check_config2(renderer, scenes.config2(320, 180, precision=abi.PREC_FP64),
+                        "fp64 config2 320x180")
+    assert st["class_mismatch_frac"] <= 1e-4
+    assert st["rgb_mismatch_frac"] <= 1e-4
+    assert st["phi_max_err"] <= 1e-9
+
+
 def test_path_graph_matches_eager(renderer):
     p = scenes.config1(320, 180, frames=6)
     renderer.set_textures(p.textures)
