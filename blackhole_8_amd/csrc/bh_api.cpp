@@ -1051,11 +1051,70 @@ int bh_batch_autotune(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_optio
   return BH_OK;
 }
 
-int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* scene, const bh_outputs* out) {
+int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) {
   if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  (void)scene;
-  (void)out;
-  return fail(ctx, BH_ERR_UNSUPPORTED, "flat march not built yet");
+  if (!sc || !out) return fail(ctx, BH_ERR_INVALID_ARG, "scene and outputs are required");
+  const bh_camera& c = sc->camera;
+  if (c.width <= 0 || c.height <= 0 || c.width > 32768 || c.height > 32768)
+    return fail(ctx, BH_ERR_INVALID_ARG, "camera width/height must be in 1..32768");
+  if (!finite3(c.position) || !finite3(c.axis_x) || !finite3(c.axis_y) || !finite3(c.axis_z) ||
+      !std::isfinite(c.fov))
+    return fail(ctx, BH_ERR_INVALID_ARG, "camera pose/fov must be finite");
+  if (sc->n_rects < 0 || sc->n_rects > BH_MAX_RECTS || sc->n_planes < 0 ||
+      sc->n_planes > BH_MAX_PLANES)
+    return fail(ctx, BH_ERR_INVALID_ARG, "n_rects / n_planes out of range");
+  if (sc->steps < 1 || sc->steps > 1000000)
+    return fail(ctx, BH_ERR_INVALID_ARG, "steps must be in 1..1e6");
+  bh::DevFlat d;
+  std::memset(&d, 0, sizeof d);
+  copy3(d.C, c.position);
+  copy3(d.vy, c.axis_y);
+  copy3(d.vz, c.axis_z);
+  const double f = bh_focal_length(c.width, c.fov);
+  for (int i = 0; i < 3; ++i) d.fv[i] = c.axis_x[i] * f;  // Camera::focus_vector
+  d.W = c.width;
+  d.H = c.height;
+  d.n_rects = sc->n_rects;
+  d.n_planes = sc->n_planes;
+  d.steps = sc->steps;
+  for (int i = 0; i < sc->n_rects; ++i) {
+    for (int v = 0; v < 5; ++v) {
+      if (!finite3(sc->rects[i].vertex[v])) return fail(ctx, BH_ERR_INVALID_ARG, "rectangle vertices must be finite");
+      copy3(d.rv[i][v], sc->rects[i].vertex[v]);
+    }
+    d.rtex[i] = devtex(ctx, sc->rects[i].texture);
+    if (!d.rtex[i].data)
+      return fail(ctx, BH_ERR_EMPTY_TEXTURE, "rectangle texture slot is empty (Material::SetTexture)");
+  }
+  for (int i = 0; i < sc->n_planes; ++i) {
+    const bh_plane& p = sc->planes[i];
+    if (!finite3(p.position) || !finite3(p.axis_x) || !finite3(p.axis_y) || !finite3(p.axis_z) ||
+        !std::isfinite(p.pattern_size))
+      return fail(ctx, BH_ERR_INVALID_ARG, "plane must be finite");
+    copy3(d.pp[i], p.position);
+    copy3(d.px[i], p.axis_x);
+    copy3(d.py[i], p.axis_y);
+    copy3(d.pz[i], p.axis_z);
+    d.psize[i] = p.pattern_size;
+  }
+  (void)hipSetDevice(ctx->device);
+  const size_t npix = (size_t)d.W * d.H;
+  DevOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
+  int rc;
+  if (out->rgb) { if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc; o.rgb = (uint8_t*)ctx->rgb.p; }
+  if (out->hit) { if ((rc = ensure(ctx, ctx->hit, npix))) return rc; o.hit = (uint8_t*)ctx->hit.p; }
+  if (out->phi_end) { if ((rc = ensure(ctx, ctx->phi, npix * 8))) return rc; o.phi = (double*)ctx->phi.p; }
+  if (out->hit_point) { if ((rc = ensure(ctx, ctx->hitpt, npix * 24))) return rc; o.hitpt = (double*)ctx->hitpt.p; }
+  if (out->steps) { if ((rc = ensure(ctx, ctx->steps, npix * 4))) return rc; o.steps = (int32_t*)ctx->steps.p; }
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, bh::launch_flat(d, o, st));
+  if (out->rgb) HIP_TRY(ctx, hipMemcpyAsync(out->rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, st));
+  if (out->hit) HIP_TRY(ctx, hipMemcpyAsync(out->hit, o.hit, npix, hipMemcpyDeviceToHost, st));
+  if (out->phi_end) HIP_TRY(ctx, hipMemcpyAsync(out->phi_end, o.phi, npix * 8, hipMemcpyDeviceToHost, st));
+  if (out->hit_point) HIP_TRY(ctx, hipMemcpyAsync(out->hit_point, o.hitpt, npix * 24, hipMemcpyDeviceToHost, st));
+  if (out->steps) HIP_TRY(ctx, hipMemcpyAsync(out->steps, o.steps, npix * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return BH_OK;
 }
 
 }  // extern "C"

@@ -119,6 +119,22 @@ hipError_t launch_batch_band(const BatchDev& b, hipStream_t stream);
 hipError_t launch_batch_clear(const BatchDev& b, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
+// Flat-space scene (RayTracer::Prograde, ray_tracer.h:52-96): straight chords,
+// rectangles and chess-patterned infinite planes.  Passed by value.
+// ---------------------------------------------------------------------------
+struct DevFlat {
+  double C[3], fv[3], vy[3], vz[3];  // camera position, focus vector, axes y, z
+  int32_t W, H, n_rects, n_planes, steps, pad0;
+  double rv[BH_MAX_RECTS][5][3];     // rectangle vertices ([0] pivot, [1..4] p1..p4)
+  DevTex rtex[BH_MAX_RECTS];
+  double pp[BH_MAX_PLANES][3], px[BH_MAX_PLANES][3], py[BH_MAX_PLANES][3],
+      pz[BH_MAX_PLANES][3];
+  double psize[BH_MAX_PLANES];
+};
+
+hipError_t launch_flat(const DevFlat& s, const DevOut& out, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
 // Differentiable hole + disc scene (bh_diff_params), resolved on the host.
 // Passed by value as a kernel argument.  Derivatives are with respect to the
 // inclination (suffix _i) and the camera distance (suffix _d); the camera

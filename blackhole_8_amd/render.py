@@ -84,6 +84,13 @@ class Renderer:
         self.set_textures(preset.textures)
         return self.render(preset.scene, preset.options, want)
 
+    def render_flat(self, scene: abi.FlatScene, want=("rgb", "hit")) -> dict:
+        """Flat-space march (RayTracer::Prograde) of a FlatScene."""
+        w, h = scene.camera.width, scene.camera.height
+        arrs, o = alloc_outputs(w, h, want)
+        self._check(self.lib.bh_render_flat(self.ctx, C.byref(scene), C.byref(o)))
+        return arrs
+
     def check_scene(self, scene: abi.Scene, opts: abi.Options) -> int:
         return self.lib.bh_scene_check(self.ctx, C.byref(scene), C.byref(opts))
 

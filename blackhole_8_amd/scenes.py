@@ -28,6 +28,7 @@ from . import abi
 
 SEED_BASE = 0xB1AC0000
 SLOT_BG, SLOT_DISC, SLOT_SPHERE0 = 0, 1, 2
+SLOT_FLAT0 = 40  # rectangles of the flat-space scene
 
 
 # --------------------------------------------------------------------------
@@ -258,6 +259,42 @@ def config4_plain(theta, width: int = 512, height: int = 512, fov: float = math.
                      (r_outer, -r_outer, 0.0), r_outer, r_in, SLOT_DISC)
     o = options(precision=precision, escape=False, band=1e-3)
     return Preset("config4_plain", s, o, {SLOT_DISC: gen_disc_texture()})
+
+
+def flat_test(width: int = 800, height: int = 450):
+    """The flat-space scene of ray_tracer_test.cc:50-95: camera (fov pi/2)
+    moved to (-400, 120, 100), turned by RotateY(-pi/24) then RotateZ(pi/12);
+    three textured rectangles (the reference's jpeg photos replaced by seeded
+    noise images); the ground plane z = 0 with ChessPattern2D(10); 10 chords
+    per pixel.  Returns (FlatScene, textures)."""
+    lib = abi.load()
+    cam = camera(width, height, math.pi / 2, position=(-400.0, 120.0, 100.0))
+    axes = (C.c_double * 9)(*cam.axis_x, *cam.axis_y, *cam.axis_z)
+    lib.bh_object_rotate(axes, None, 0, 1, -math.pi / 24)
+    lib.bh_object_rotate(axes, None, 0, 2, math.pi / 12)
+    cam.axis_x[:], cam.axis_y[:], cam.axis_z[:] = axes[0:3], axes[3:6], axes[6:9]
+    s = abi.FlatScene()
+    s.camera = cam
+    rects = [((100, 120, 100), (100, 0, 100), (100, 0, 0), (100, 120, 0)),
+             ((100, 0, 150), (100, -100, 150), (100, -100, 0), (100, 0, 0)),
+             ((-100, 50, 100), (-100, -50, 100), (-100, -50, 0), (-100, 50, 0))]
+    sizes = [(360, 480), (400, 300), (320, 320)]
+    tex = {}
+    for i, (pts, (h, w)) in enumerate(zip(rects, sizes)):
+        r = s.rects[i]
+        r.vertex[0][:] = (0.0, 0.0, 0.0)  # Object's default pivot
+        for k in range(4):
+            r.vertex[k + 1][:] = pts[k]
+        r.texture = SLOT_FLAT0 + i
+        tex[SLOT_FLAT0 + i] = gen_noise((h, w, 3), SEED_BASE + 0x200 + i)
+    s.n_rects = 3
+    pl = s.planes[0]
+    pl.position[:] = (0.0, 0.0, 0.0)
+    pl.axis_x[:], pl.axis_y[:], pl.axis_z[:] = (1, 0, 0), (0, 1, 0), (0, 0, 1)
+    pl.pattern_size = 10.0
+    s.n_planes = 1
+    s.steps = 10
+    return s, tex
 
 
 def shipped(width: int = 960, height: int = 540, precision=abi.PREC_FP64) -> Preset:

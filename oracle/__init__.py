@@ -63,6 +63,9 @@ def load():
         "or_inbound_sweep": (D, [D, D, D, C.c_int, C.c_int]),
         "or_render_flat": (C.c_int, [C.POINTER(abi.FlatScene), C.POINTER(abi.Image), C.c_int,
                                      C.POINTER(abi.Outputs)]),
+        "or_rect_collide": (C.c_int, [C.POINTER(abi.Rect), dp, dp, dp]),
+        "or_plane_collide": (C.c_int, [C.POINTER(abi.Plane), dp, dp, dp]),
+        "or_plane_color": (None, [C.POINTER(abi.Plane), dp, P]),
         "or_render_diff": (C.c_int, [C.POINTER(abi.DiffParams), C.POINTER(abi.Image), C.c_int,
                                      C.c_int, P, P, P]),
         "or_diff_scene": (None, [C.POINTER(abi.DiffParams), C.POINTER(abi.Scene),

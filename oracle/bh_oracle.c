@@ -932,6 +932,24 @@ static void plane_color(const bh_plane* pl, v3 q, uint8_t rgb[3]) {
     rgb[0] = rgb[1] = rgb[2] = 0;
 }
 
+int or_rect_collide(const bh_rect* r, const double a[3], const double b[3], double out[3]) {
+  v3 q;
+  int h = rect_collide(r, ld(a), ld(b), &q);
+  if (h) memcpy(out, q.v, sizeof q.v);
+  return h;
+}
+
+int or_plane_collide(const bh_plane* pl, const double a[3], const double b[3], double out[3]) {
+  v3 q;
+  int h = plane_collide(pl, ld(a), ld(b), &q);
+  if (h) memcpy(out, q.v, sizeof q.v);
+  return h;
+}
+
+void or_plane_color(const bh_plane* pl, const double q[3], uint8_t rgb[3]) {
+  plane_color(pl, ld(q), rgb);
+}
+
 int or_render_flat(const bh_flat_scene* s, const bh_image* tex, int ntex,
                    const bh_outputs* out) {
   const bh_camera* cam = &s->camera;

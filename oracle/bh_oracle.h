@@ -79,7 +79,12 @@ int or_render_diff(const bh_diff_params* p, const bh_image* textures, int n_text
 /* The scene and options the differentiable mode renders (reference march). */
 void or_diff_scene(const bh_diff_params* p, bh_scene* s, bh_options* o);
 
-/* Flat march (ray_tracer.h:52-96). */
+/* Flat march (ray_tracer.h:52-96) and its object tests: Rectangle::Collide
+ * (vector_object.h:107-127), InfinitePlane::Collide / color with
+ * ChessPattern2D (vector_object.h:210-232, pattern.h:14-50). */
+int or_rect_collide(const bh_rect* r, const double a[3], const double b[3], double out[3]);
+int or_plane_collide(const bh_plane* pl, const double a[3], const double b[3], double out[3]);
+void or_plane_color(const bh_plane* pl, const double q[3], uint8_t rgb[3]);
 int or_render_flat(const bh_flat_scene* s, const bh_image* textures, int n_textures,
                    const bh_outputs* out);
 

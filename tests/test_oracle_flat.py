@@ -1,0 +1,198 @@
+"""Pins the oracle's flat-space march (or_render_flat in oracle/bh_oracle.c),
+the ray_tracer_test entry point (include/blackhole/ray_tracer.h:52-96), without
+running the reference:
+  - ChessPattern2D (pattern.h:14-50) against hand-worked known answers;
+  - Rectangle::Collide and InfinitePlane::Collide against a brute-force
+    search for the sign change along the segment and an independent solve of
+    the crossing's rectangle coordinates;
+  - whole frames: every pixel's first object and hit point against the exact
+    first intersection of its line with the objects, and the step count
+    against the chord the recurrence (first chord camera -> pixel point, then
+    fixed vector, rescaled by 100/|v|^2 when |v|^2 < 100) puts it in.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from blackhole_8_amd import abi, scenes
+import oracle
+
+L = oracle.load()
+
+
+def dp(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def plane(pos=(0, 0, 0), ax=(1, 0, 0), ay=(0, 1, 0), az=(0, 0, 1), size=10.0):
+    p = abi.Plane()
+    p.position[:], p.axis_x[:], p.axis_y[:], p.axis_z[:] = pos, ax, ay, az
+    p.pattern_size = size
+    return p
+
+
+def color_at(pl, q):
+    rgb = np.zeros(3, np.uint8)
+    L.or_plane_color(C.byref(pl), dp(q), rgb.ctypes.data)
+    return int(rgb[0])
+
+
+@pytest.mark.parametrize("x,y,white", [
+    (5, 5, True),      # both in the first half-period, x*y > 0
+    (15, 5, False),    # x in the second half, y in the first
+    (15, 15, True),    # both in the second half
+    (-5, 5, False),    # x*y < 0 flips the pattern
+    (-15, 5, True),
+    (25, 3, True),     # 25 mod 20 = 5
+    (-25, -3, True),   # |x| mod 20, x*y > 0
+    (10, 10, True),    # boundary: <= on both sides
+    (0, 5, False),     # x*y = 0 takes the flipped branch
+    (30, 30, True),    # 30 mod 20 = 10 -> boundary, in
+])
+def test_chess_pattern_known_answers(x, y, white):
+    assert color_at(plane(), (x, y, 0.0)) == (255 if white else 0)
+
+
+def test_chess_pattern_zero_size_is_black():
+    assert color_at(plane(size=0.0), (5, 5, 0)) == 0
+
+
+def test_plane_coordinates_follow_axes():
+    # axes rotated by 90 deg about z: a = y, b = -x
+    pl = plane(ax=(math.sqrt(0.5), math.sqrt(0.5), 0), ay=(-math.sqrt(0.5), math.sqrt(0.5), 0))
+    q = np.array([3.0, 11.0, 0.0])
+    a = (q[0] + q[1]) / math.sqrt(2)
+    b = (q[1] - q[0]) / math.sqrt(2)
+    assert color_at(pl, q) == color_at(plane(), (a, b, 0.0))
+
+
+def _crossing_brute(p1, p2, f, n=2001):
+    """Parameter of the sign change of f along p1 -> p2 by dense sampling
+    and bisection, or None."""
+    ts = np.linspace(0.0, 1.0, n)
+    v = np.array([f(p1 + t * (p2 - p1)) for t in ts])
+    idx = np.nonzero(np.sign(v[:-1]) * np.sign(v[1:]) < 0)[0]
+    if idx.size == 0:
+        return None
+    lo, hi = ts[idx[0]], ts[idx[0] + 1]
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if np.sign(f(p1 + mid * (p2 - p1))) == np.sign(f(p1 + lo * (p2 - p1))):
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def test_rect_collide_against_brute_force():
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        p1 = rng.uniform(-5, 5, 3)
+        s1 = rng.uniform(-3, 3, 3)
+        s3 = np.cross(s1, rng.uniform(-1, 1, 3))
+        s3 *= rng.uniform(0.5, 3) / np.linalg.norm(s3)
+        r = abi.Rect()
+        r.vertex[0][:] = (0, 0, 0)
+        r.vertex[1][:] = p1
+        r.vertex[2][:] = p1 + s1
+        r.vertex[3][:] = p1 + s1 + s3
+        r.vertex[4][:] = p1 + s3
+        a, b = rng.uniform(-8, 8, 3), rng.uniform(-8, 8, 3)
+        out = np.zeros(3)
+        hit = L.or_rect_collide(C.byref(r), dp(a), dp(b), dp(out))
+        nrm = np.cross(s1, s3)
+        t = _crossing_brute(a, b, lambda P: float(np.dot(nrm, P - p1)))
+        expect = False
+        if t is not None:
+            q = a + t * (b - a) - p1
+            uv = np.linalg.lstsq(np.stack([s1, s3], 1), q, rcond=None)[0]
+            expect = bool(0 < uv[0] < 1 and 0 < uv[1] < 1)
+            if min(abs(uv[0]), abs(uv[0] - 1), abs(uv[1]), abs(uv[1] - 1)) < 1e-9:
+                continue  # on an edge: either answer
+        assert bool(hit) == expect
+        if hit:
+            np.testing.assert_allclose(out, a + t * (b - a), atol=1e-9)
+
+
+def test_plane_collide_closed_form():
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        o = rng.uniform(-3, 3, 3)
+        z = rng.normal(size=3)
+        z /= np.linalg.norm(z)
+        pl = plane(pos=o, az=z)
+        a, b = rng.uniform(-6, 6, 3), rng.uniform(-6, 6, 3)
+        out = np.zeros(3)
+        hit = L.or_plane_collide(C.byref(pl), dp(a), dp(b), dp(out))
+        t1, t2 = np.dot(z, a - o), np.dot(z, b - o)
+        assert bool(hit) == (t1 * t2 <= 0)
+        if hit:
+            np.testing.assert_allclose(out, a + t1 / (t1 - t2) * (b - a), atol=1e-9)
+    # a chord touching the plane at its start hits there
+    pl = plane()
+    out = np.zeros(3)
+    assert L.or_plane_collide(C.byref(pl), dp([1, 2, 0]), dp([1, 2, 5]), dp(out)) == 1
+    np.testing.assert_allclose(out, [1, 2, 0])
+
+
+def _line_hits(C0, P, s):
+    """First intersection along the line C0 -> P (parameter along the unit
+    direction) with the scene's rectangles and plane: (dist, kind, index)."""
+    d = P - C0
+    d /= np.linalg.norm(d)
+    best = (np.inf, None, -1)
+    for i in range(s.n_rects):
+        v = np.array(s.rects[i].vertex)
+        p1, s1, s3 = v[1], v[2] - v[1], v[4] - v[1]
+        n = np.cross(s1, s3)
+        den = np.dot(n, d)
+        if abs(den) < 1e-14:
+            continue
+        lam = np.dot(n, p1 - C0) / den
+        if lam <= 0:
+            continue
+        q = C0 + lam * d - p1
+        u, w = np.dot(q, s1) / np.dot(s1, s1), np.dot(q, s3) / np.dot(s3, s3)
+        if 0 < u < 1 and 0 < w < 1 and lam < best[0]:
+            best = (lam, "rect", i)
+    for i in range(s.n_planes):
+        pl = s.planes[i]
+        z, o = np.array(pl.axis_z[:]), np.array(pl.position[:])
+        den = np.dot(z, d)
+        if abs(den) < 1e-14:
+            continue
+        lam = np.dot(z, o - C0) / den
+        if lam > 0 and lam < best[0]:
+            best = (lam, "plane", i)
+    return best, d
+
+
+def test_flat_frame_against_exact_line_intersections():
+    s, tex = scenes.flat_test(160, 90)
+    r = oracle.render_flat(s, tex, ("rgb", "hit", "hitpt", "steps"))
+    cam = s.camera
+    C0 = np.array(cam.position[:])
+    f = L.or_focal_length(cam.width, cam.fov)
+    ax, ay, az = (np.array(v[:]) for v in (cam.axis_x, cam.axis_y, cam.axis_z))
+    checked = 0
+    for y in range(0, cam.height, 3):
+        for x in range(0, cam.width, 3):
+            P = f * ax - ay * (cam.width / 2.0 - x) - az * (cam.height / 2.0 - y)
+            (lam, kind, i), d = _line_hits(C0, P, s)
+            L0 = np.linalg.norm(P - C0)
+            vlen = L0 if L0 * L0 >= 100 else 100.0 / L0
+            reach = L0 + (s.steps - 1) * vlen  # marched length
+            if kind is None or lam > reach:
+                assert r["hit"][y, x] == abi.HIT_NONE and r["steps"][y, x] == s.steps
+                continue
+            # the chord that holds the hit
+            k = 1 if lam <= L0 else 2 + int((lam - L0) // vlen)
+            if abs(lam - L0 - (k - 2) * vlen) < 1e-6 or abs(lam - reach) < 1e-6:
+                continue  # on a chord end: either chord
+            assert r["hit"][y, x] == abi.HIT_OBJECT
+            assert r["steps"][y, x] == k, (x, y)
+            np.testing.assert_allclose(r["hitpt"][y, x], C0 + lam * d, rtol=0, atol=1e-8)
+            checked += 1
+    assert checked > 1000
