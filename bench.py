@@ -49,6 +49,46 @@ def host_threads() -> int:
     return max(1, min(16, n))
 
 
+def batch_timing(reps: int = 3) -> dict:
+    """The 512-scene batch of uneven renders in one persistent launch: best of
+    reps runs with the default knobs and with the autotuned ones."""
+    import numpy as np
+
+    from blackhole_8_amd import Renderer, scenes, tune
+
+    r = Renderer(0)
+    try:
+        items, tex = scenes.config3()
+        r.set_textures(tex)
+        rays = sum(s.camera.width * s.camera.height for s, _ in items)
+
+        def best_of(tuning):
+            runs = [r.render_batch(items, timing=True, tuning=tuning) for _ in range(reps)]
+            _, ms, st = min(runs, key=lambda x: x[2].total_ms)
+            return ms, st
+
+        ms0, st0 = best_of(None)
+        best, rec = tune.autotune(r, items, reps=2)
+        ms1, st1 = best_of(best)
+    finally:
+        r.close()
+
+    def line(ms, st):
+        return {"total_ms": round(st.total_ms, 4), "march_ms": round(st.march_ms, 4),
+                "band_ms": round(st.band_ms, 4), "band_rays": int(st.band_rays),
+                "mrays_per_s": round(rays / st.total_ms / 1e3, 1),
+                "scene_ms_median": round(float(np.median(ms)), 4),
+                "scene_ms_p90": round(float(np.quantile(ms, 0.9)), 4),
+                "scene_ms_max": round(float(ms.max()), 4)}
+
+    return {"workload": "config3: 512 scenes, side log-uniform 32..1024, r_s U[0.5,2], camera "
+                        "U[6,50] r_s, inclination U[0,90] deg, disc with p 0.5, FP32, one "
+                        "batch call (grid, persistent-tile or lane-refill mode, autotuned)",
+            "scenes": len(items), "rays": int(rays), "default": line(ms0, st0),
+            "tuned": dict(line(ms1, st1), tuning={f: getattr(best, f)
+                                                   for f, _ in best._fields_})}
+
+
 def adjoint_timing(iters: int = 50) -> dict:
     """Forward+adjoint against the forward on the 512x512 inverse-fit scene
     (FP32): the differentiable forward, the fused forward+adjoint launch, and
@@ -82,8 +122,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the 512x512 forward / forward+adjoint and exit")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only time the 512-scene batch and exit")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times), for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -103,6 +146,9 @@ def main() -> int:
 
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_timing()}))
+        return 0
+    if args.batch_only:
+        print(json.dumps({"batch": batch_timing()}))
         return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -203,6 +249,7 @@ def main() -> int:
     r.close()
 
     adjoint = None if args.no_adjoint else adjoint_timing()
+    batch = None if args.no_batch else batch_timing()
 
     line = {
         "metric": "Mrays/s at 1080p BH+disc; fwd+adjoint ms/frame vs plain fwd; % FP32 peak",
@@ -227,6 +274,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "accuracy_frame0": accuracy,
         "adjoint": adjoint,
+        "batch": batch,
     }
     print(json.dumps(line))
     return 0

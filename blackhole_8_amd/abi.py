@@ -31,6 +31,7 @@ BH_ERR_BAD_HANDLE = -7
 HIT_NONE, HIT_HORIZON, HIT_DISC, HIT_OBJECT = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
 FILTER_NEAREST, FILTER_BILINEAR = 0, 1
+BATCH_GRID, BATCH_TILES, BATCH_LANES = 1, 2, 3
 ROOT_BISECT20, ROOT_EXACT = 0, 1
 
 d3 = C.c_double * 3
@@ -79,8 +80,8 @@ class Outputs(C.Structure):
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("block", C.c_int32), ("waves_per_simd", C.c_int32), ("refill_steps", C.c_int32),
-                ("refill_idle", C.c_int32), ("chunk", C.c_int32)]
+    _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("waves_per_simd", C.c_int32),
+                ("refill_steps", C.c_int32), ("refill_idle", C.c_int32), ("chunk", C.c_int32)]
 
 
 class BatchStats(C.Structure):

@@ -38,7 +38,7 @@ struct bh_ctx {
   DeviceTexture tex[BH_MAX_TEXTURES];
   DevBuf scene, rgb, hit, phi, hitpt, steps, qitems, qcount;
   DevBuf dpart, dgrad, drgb, dhit, dgin;  // differentiable render
-  DevBuf bscenes, bouts, boff, bctl, bband, bend;  // batches
+  DevBuf bscenes, bouts, boff, bctl, bband, bend, btile;  // batches
   int wall_khz = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   char name[256] = {0};
@@ -259,6 +259,9 @@ int launch_frame(bh_ctx* ctx, const DevScene& host_scene, DevScene* d_scene, con
                  const DevQueue& q, hipStream_t st) {
   const int W = host_scene.W, H = host_scene.H;
   if (fp32_eligible(host_scene)) {
+    // the FP32 march appends its band rays to q
+    if (!q.items || !q.count || q.capacity == 0)
+      return fail(ctx, BH_ERR_INVALID_ARG, "FP32 march launched without a band queue");
     const bool esc = host_scene.escape != 0;
     const bool sph = host_scene.n_spheres > 0;
     HIP_TRY(ctx, bh::launch_march_fp32(d_scene, out, W, H, q, st, esc, sph));
@@ -290,10 +293,14 @@ struct BatchPlan {
   std::vector<DevScene> scenes;
   std::vector<bh::DevOut> outs;
   std::vector<uint32_t> off;   // ray offsets of the FP32 scenes (n_fast + 1)
-  std::vector<int> fast;       // indices of the scenes in the persistent march
+  std::vector<int> fast;       // indices of the scenes in the persistent march, by class
   std::vector<int> solo;       // indices rendered one by one
+  uint32_t cls_tile[5] = {0, 0, 0, 0, 0};  // tiles of class c: [cls_tile[c], cls_tile[c+1])
   uint32_t n_rays = 0;
 };
+
+// Scene class of the tile-mode kernel instances: escape + 2 spheres.
+int scene_class(const DevScene& d) { return (d.escape ? 1 : 0) + (d.n_spheres > 0 ? 2 : 0); }
 
 int plan_batch(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* options,
                const bh_outputs* outs, bool lane_mode, BatchPlan* p) {
@@ -301,8 +308,7 @@ int plan_batch(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* opt
     return fail(ctx, BH_ERR_INVALID_ARG, "batch needs n > 0 and scene, option and output arrays");
   p->scenes.resize(n);
   p->outs.resize(n);
-  uint64_t rays = 0;
-  p->off.push_back(0);
+  std::vector<int> by_cls[4];
   for (int i = 0; i < n; ++i) {
     int rc = build_scene(ctx, &scenes[i], &options[i], &p->scenes[i]);
     if (rc) {
@@ -312,23 +318,34 @@ int plan_batch(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* opt
     const bh_outputs& o = outs[i];
     p->outs[i] = DevOut{o.rgb, o.hit, o.phi_end, o.hit_point, o.steps};
     // the lane-refill mode keeps no sphere table per lane: such scenes go solo
-    if (fp32_eligible(p->scenes[i]) && !(lane_mode && p->scenes[i].n_spheres > 0)) {
+    if (fp32_eligible(p->scenes[i]) && !(lane_mode && p->scenes[i].n_spheres > 0))
+      by_cls[scene_class(p->scenes[i])].push_back(i);
+    else
+      p->solo.push_back(i);
+  }
+  // scenes grouped by class, in the given order within a class
+  uint64_t rays = 0;
+  p->off.push_back(0);
+  for (int c = 0; c < 4; ++c) {
+    p->cls_tile[c] = (uint32_t)(rays >> 6);
+    for (int i : by_cls[c]) {
       const DevScene& d = p->scenes[i];
       rays += (uint64_t)((d.W + 7) / 8) * (uint64_t)((d.H + 7) / 8) * 64u;
-      if (rays >= ((uint64_t)1 << 31)) return fail(ctx, BH_ERR_INVALID_ARG, "batch larger than 2^31 rays");
+      if (rays >= ((uint64_t)1 << 31))
+        return fail(ctx, BH_ERR_INVALID_ARG, "batch larger than 2^31 rays");
       p->fast.push_back(i);
       p->off.push_back((uint32_t)rays);
-    } else {
-      p->solo.push_back(i);
     }
   }
+  p->cls_tile[4] = (uint32_t)(rays >> 6);
   p->n_rays = (uint32_t)rays;
   return BH_OK;
 }
 
 bh_batch_tuning normalized_tuning(const bh_batch_tuning* t) {
-  bh_batch_tuning r{256, 4, 4, 64, 4};
+  bh_batch_tuning r{BH_BATCH_GRID, 256, 4, 4, 16, 4};
   if (t) {
+    if (t->mode) r.mode = t->mode;
     if (t->block) r.block = t->block;
     if (t->waves_per_simd) r.waves_per_simd = t->waves_per_simd;
     if (t->refill_steps) r.refill_steps = t->refill_steps;
@@ -339,6 +356,8 @@ bh_batch_tuning normalized_tuning(const bh_batch_tuning* t) {
 }
 
 int check_tuning(bh_ctx* ctx, const bh_batch_tuning& t) {
+  if (t.mode != BH_BATCH_GRID && t.mode != BH_BATCH_TILES && t.mode != BH_BATCH_LANES)
+    return fail(ctx, BH_ERR_INVALID_ARG, "tuning.mode must be BH_BATCH_GRID, _TILES or _LANES");
   if (t.block != 64 && t.block != 128 && t.block != 256)
     return fail(ctx, BH_ERR_INVALID_ARG, "tuning.block must be 64, 128 or 256");
   if (t.waves_per_simd < 1 || t.waves_per_simd > 8)
@@ -363,6 +382,9 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   if ((rc = ensure(ctx, ctx->boff, (p.off.size() + 1) * sizeof(uint32_t)))) return rc;
   if ((rc = ensure(ctx, ctx->bctl, 64))) return rc;
   if ((rc = ensure(ctx, ctx->bend, (n + 1) * sizeof(unsigned long long)))) return rc;
+  if (t.mode == BH_BATCH_GRID &&
+      (rc = ensure(ctx, ctx->btile, ((size_t)(p.n_rays >> 6) + 1) * sizeof(unsigned long long))))
+    return rc;
   const uint32_t cap = std::max<uint32_t>(p.n_rays / 16u, 4096u);
   if ((rc = ensure(ctx, ctx->bband, (size_t)cap * sizeof(uint32_t)))) return rc;
   hipStream_t st = ctx->stream;
@@ -390,13 +412,18 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   b.ray_off = (const uint32_t*)ctx->boff.p;
   b.n_scenes = (uint32_t)p.fast.size();
   b.n_rays = p.n_rays;
+  // control block: four queue heads, the band count, the start time
   b.head = (uint32_t*)ctx->bctl.p;
-  b.band_count = (uint32_t*)ctx->bctl.p + 1;
-  b.t_start = (unsigned long long*)((char*)ctx->bctl.p + 8);
+  b.band_count = (uint32_t*)ctx->bctl.p + 4;
+  b.t_start = (unsigned long long*)((char*)ctx->bctl.p + 24);
+  b.tile_lo = 0;
+  b.tile_hi = p.n_rays >> 6;
   b.band_items = (uint32_t*)ctx->bband.p;
   b.band_capacity = cap;
   b.refill_steps = t.refill_steps;
   b.refill_idle = t.refill_idle;
+  b.mode = t.mode;
+  b.tile_end = (unsigned long long*)ctx->btile.p;
   b.chunk = t.chunk;
   b.scene_end = (unsigned long long*)ctx->bend.p;
   HIP_TRY(ctx, bh::launch_batch_clear(b, st));
@@ -404,14 +431,36 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   if (b.n_scenes > 0) {
     const int waves = std::max(1, ctx->n_cu) * 4 * t.waves_per_simd;
     const int grid = std::max(1, waves / (t.block / 64));
-    HIP_TRY(ctx, bh::launch_batch(b, t.block, grid, st));
+    if (t.mode == BH_BATCH_LANES) {
+      HIP_TRY(ctx, bh::launch_batch(b, 0, t.block, grid, st));
+    } else {
+      // one persistent launch per scene class present, each with its own head
+      bh::BatchDev bc = b;
+      for (int c = 0; c < 4; ++c) {
+        if (p.cls_tile[c + 1] == p.cls_tile[c]) continue;
+        bc.tile_lo = p.cls_tile[c];
+        bc.tile_hi = p.cls_tile[c + 1];
+        bc.head = b.head + c;
+        HIP_TRY(ctx, bh::launch_batch(bc, c, t.block, grid, st));
+      }
+    }
   }
+  if (b.n_scenes > 0 && t.mode == BH_BATCH_GRID) HIP_TRY(ctx, bh::launch_batch_grid_finish(b, st));
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
   if (b.n_scenes > 0) HIP_TRY(ctx, bh::launch_batch_band(b, st));
-  for (size_t k = 0; k < p.solo.size(); ++k) {
-    const int i = p.solo[k];
-    DevQueue q{nullptr, nullptr, 0};
-    if ((rc = launch_frame(ctx, p.scenes[i], d_all + p.fast.size() + k, p.outs[i], q, st))) return rc;
+  // solo scenes: the per-frame kernels, with the context's band queue (the
+  // band kernel clears it after each scene)
+  if (!p.solo.empty()) {
+    size_t maxpix = 0;
+    for (int i : p.solo) maxpix = std::max(maxpix, (size_t)p.scenes[i].W * p.scenes[i].H);
+    uint32_t *qi, *qc;
+    if ((rc = queue_buffers(ctx, &qi, &qc, maxpix))) return rc;
+    for (size_t k = 0; k < p.solo.size(); ++k) {
+      const int i = p.solo[k];
+      const DevScene& d = p.scenes[i];
+      DevQueue q{qi, qc, (uint32_t)((size_t)d.W * d.H)};
+      if ((rc = launch_frame(ctx, d, d_all + p.fast.size() + k, p.outs[i], q, st))) return rc;
+    }
   }
   HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
   if (bd_out) *bd_out = b;
@@ -565,7 +614,7 @@ void bh_ctx_destroy(bh_ctx* ctx) {
   for (DevBuf* b : {&ctx->scene, &ctx->rgb, &ctx->hit, &ctx->phi, &ctx->hitpt, &ctx->steps,
                     &ctx->qitems, &ctx->qcount, &ctx->dpart, &ctx->dgrad, &ctx->drgb, &ctx->dhit,
                     &ctx->dgin, &ctx->bscenes, &ctx->bouts, &ctx->boff, &ctx->bctl, &ctx->bband,
-                    &ctx->bend})
+                    &ctx->bend, &ctx->btile})
     if (b->p) (void)hipFree(b->p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -972,7 +1021,7 @@ int bh_render_batch_device(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_
   int rc = check_tuning(ctx, t);
   if (rc) return rc;
   BatchPlan p;
-  if ((rc = plan_batch(ctx, n, scenes, options, outs, t.refill_idle < 64, &p))) return rc;
+  if ((rc = plan_batch(ctx, n, scenes, options, outs, t.mode == BH_BATCH_LANES, &p))) return rc;
   (void)hipSetDevice(ctx->device);
   bh::BatchDev b;
   if ((rc = run_batch(ctx, p, t, &b))) return rc;
@@ -1026,7 +1075,7 @@ int bh_batch_autotune(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_optio
   for (int c = 0; c < n_cand; ++c) {
     const bh_batch_tuning t = normalized_tuning(&cand[c]);
     if ((rc = check_tuning(ctx, t))) return rc;
-    const int m = t.refill_idle < 64 ? 1 : 0;
+    const int m = t.mode == BH_BATCH_LANES ? 1 : 0;
     if (!planned[m]) {
       if ((rc = plan_batch(ctx, n, scenes, options, outs, m == 1, &plans[m]))) return rc;
       planned[m] = true;

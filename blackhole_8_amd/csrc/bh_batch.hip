@@ -6,12 +6,16 @@
 // kernel runs the same set-up, step and chord tests as the per-frame FP32
 // kernel (bh_march.h), so every scene of a batch equals its solo render.
 //
-// Scheduling.  A fixed grid of persistent waves fills the GPU and takes work
-// from a device queue of ray ids, in one of two modes:
-//  tile mode (refill_idle = 64): a wave takes a whole 8x8 tile of one scene
-//    and marches it with the per-frame kernel's wave-uniform loop, the scene
-//    constants in scalar registers;
-//  lane-refill mode (refill_idle < 64): each lane owns one ray at a time and
+// Scheduling, in one of three modes (scenes grouped by class: escape,
+// spheres; one kernel instance per class present):
+//  grid mode: one ordinary launch with a wave per 8x8 tile of every scene;
+//    the hardware dispatcher balances the load, each wave marches its tile
+//    with the per-frame kernel's wave-uniform loop (scene constants in
+//    scalar registers) and stores its finish time for the per-scene maximum;
+//  tile mode: a fixed grid of persistent waves takes ranges of tiles from a
+//    device queue and marches them the same way;
+//  lane-refill mode: a fixed grid of persistent waves; each lane owns one
+//    ray at a time and
 //    advances it by its own step counter (a state machine over the reference
 //    rule's steps), so lanes of one wave may be at different steps, in
 //    different scenes.  After every `refill_steps` steps the wave ballots its
@@ -319,9 +323,12 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
   return true;
 }
 
+// Tiles [b.tile_lo, b.tile_hi) hold scenes of one class (escape, spheres),
+// marched by the kernel instance of that class.
+template <bool ESCAPE, bool SPH>
 __device__ inline void run_tiles(const BatchDev& b, const DevScene* __restrict__ scenes,
                                  uint32_t waves) {
-  const uint32_t n_tiles = b.n_rays >> 6;
+  const uint32_t n_tiles = b.tile_hi - b.tile_lo;
   Finish f{~0u, 0ull};
   uint32_t seen = 0;
   uint32_t t0, cnt;
@@ -331,28 +338,56 @@ __device__ inline void run_tiles(const BatchDev& b, const DevScene* __restrict__
        ++pops) {
     const uint32_t t_end = min(t0 + cnt, n_tiles);
     for (uint32_t t = t0; t < t_end; ++t) {
-      const uint32_t id0 = t << 6;
+      const uint32_t id0 = (b.tile_lo + t) << 6;
       const uint32_t sc = (uint32_t)__builtin_amdgcn_readfirstlane((int)scene_of(b, id0));
       const DevScene& s = scenes[sc];
       const uint32_t id = id0 + (uint32_t)lane_id();
-      bool fin;
-      if (s.n_spheres > 0)
-        fin = s.escape ? march_tile<true, true>(b, s, sc, id) : march_tile<false, true>(b, s, sc, id);
-      else
-        fin = s.escape ? march_tile<true, false>(b, s, sc, id)
-                       : march_tile<false, false>(b, s, sc, id);
+      const bool fin = march_tile<ESCAPE, SPH>(b, s, sc, id);
       note_finish(b, f, fin, sc);
     }
   }
   flush(b, f);
 }
 
+// Grid mode: one wave per tile of [b.tile_lo, b.tile_hi), no queue; the
+// wave's finish time goes to tile_end[tile] (reduced per scene afterwards).
+template <bool ESCAPE, bool SPH>
+__global__ __launch_bounds__(256) void batch_grid_kernel(BatchDev b,
+                                                         const DevScene* __restrict__ scenes) {
+  // the first workgroup is dispatched first: its start is the batch's
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    atomicMin(b.t_start, (unsigned long long)wall_clock64());
+  const uint32_t tile = b.tile_lo + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (tile >= b.tile_hi) return;
+  const uint32_t id0 = tile << 6;
+  const uint32_t sc = (uint32_t)__builtin_amdgcn_readfirstlane((int)scene_of(b, id0));
+  march_tile<ESCAPE, SPH>(b, scenes[sc], sc, id0 + (uint32_t)lane_id());
+  if (lane_id() == 0) b.tile_end[tile] = (unsigned long long)wall_clock64();
+}
+
+// Latest tile finish time of each scene (grid mode), and the earliest start.
+__global__ __launch_bounds__(256) void batch_grid_finish_kernel(BatchDev b) {
+  const uint32_t sc = blockIdx.x;
+  const uint32_t t0 = b.ray_off[sc] >> 6, t1 = b.ray_off[sc + 1] >> 6;
+  unsigned long long m = 0ull;
+  for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) m = max(m, b.tile_end[t]);
+  __shared__ unsigned long long red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) b.scene_end[sc] = red[0];
+}
+
 // The scene table is also passed as a restrict-qualified read-only argument
 // so that the tile mode's uniform scene loads go through the scalar cache.
+template <bool ESCAPE, bool SPH>
 __global__ __launch_bounds__(256) void batch_tile_kernel(BatchDev b,
                                                          const DevScene* __restrict__ scenes) {
   if (lane_id() == 0) atomicMin(b.t_start, (unsigned long long)wall_clock64());
-  run_tiles(b, scenes, (gridDim.x * blockDim.x) >> 6);
+  run_tiles<ESCAPE, SPH>(b, scenes, (gridDim.x * blockDim.x) >> 6);
 }
 
 __global__ __launch_bounds__(256) void batch_lane_kernel(BatchDev b) {
@@ -397,8 +432,8 @@ __global__ __launch_bounds__(64) void batch_band_kernel(BatchDev b) {
 __global__ void batch_clear_kernel(BatchDev b) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < b.n_scenes) b.scene_end[i] = 0ull;
+  if (i < 4) b.head[i] = 0u;
   if (i == 0) {
-    b.head[0] = 0u;
     b.band_count[0] = 0u;
     b.t_start[0] = ~0ull;
   }
@@ -412,11 +447,54 @@ hipError_t launch_batch_clear(const BatchDev& b, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_batch(const BatchDev& b, int block, int grid, hipStream_t stream) {
-  if (b.refill_idle >= 64)
-    hipLaunchKernelGGL(batch_tile_kernel, dim3(grid), dim3(block), 0, stream, b, b.scenes);
-  else
+hipError_t launch_batch(const BatchDev& b, int cls, int block, int grid, hipStream_t stream) {
+  if (b.mode == BH_BATCH_GRID) {
+    const uint32_t tiles = b.tile_hi - b.tile_lo;
+    const uint32_t per = (uint32_t)block / 64;
+    const dim3 g((tiles + per - 1) / per);
+    switch (cls) {
+      case 0:
+        hipLaunchKernelGGL((batch_grid_kernel<false, false>), g, dim3(block), 0, stream, b, b.scenes);
+        break;
+      case 1:
+        hipLaunchKernelGGL((batch_grid_kernel<true, false>), g, dim3(block), 0, stream, b, b.scenes);
+        break;
+      case 2:
+        hipLaunchKernelGGL((batch_grid_kernel<false, true>), g, dim3(block), 0, stream, b, b.scenes);
+        break;
+      default:
+        hipLaunchKernelGGL((batch_grid_kernel<true, true>), g, dim3(block), 0, stream, b, b.scenes);
+        break;
+    }
+    return hipGetLastError();
+  }
+  if (b.mode == BH_BATCH_LANES) {
     hipLaunchKernelGGL(batch_lane_kernel, dim3(grid), dim3(block), 0, stream, b);
+  } else {
+    switch (cls) {
+      case 0:
+        hipLaunchKernelGGL((batch_tile_kernel<false, false>), dim3(grid), dim3(block), 0, stream,
+                           b, b.scenes);
+        break;
+      case 1:
+        hipLaunchKernelGGL((batch_tile_kernel<true, false>), dim3(grid), dim3(block), 0, stream,
+                           b, b.scenes);
+        break;
+      case 2:
+        hipLaunchKernelGGL((batch_tile_kernel<false, true>), dim3(grid), dim3(block), 0, stream,
+                           b, b.scenes);
+        break;
+      default:
+        hipLaunchKernelGGL((batch_tile_kernel<true, true>), dim3(grid), dim3(block), 0, stream,
+                           b, b.scenes);
+        break;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_grid_finish(const BatchDev& b, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_grid_finish_kernel, dim3(b.n_scenes), dim3(256), 0, stream, b);
   return hipGetLastError();
 }
 

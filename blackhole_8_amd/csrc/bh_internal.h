@@ -103,19 +103,24 @@ struct BatchDev {
   const DevOut* outs;
   const uint32_t* ray_off;   // n_scenes + 1 entries
   uint32_t n_scenes, n_rays;
-  uint32_t* head;            // next ray id of the queue
+  uint32_t tile_lo, tile_hi; // tile-mode launch: the tiles of one scene class
+  uint32_t* head;            // queue head of this launch (four slots, cleared together)
   uint32_t* band_items;      // band rays (ray ids) for the FP64 pass
   uint32_t* band_count;      // [0] queued band rays
   uint32_t band_capacity;
+  int32_t mode;              // BH_BATCH_GRID, _TILES or _LANES
   int32_t refill_steps;      // steps between refill checks
   int32_t refill_idle;       // refill once at least this many lanes of 64 idle
   int32_t chunk;             // least tiles per queue pop
+  unsigned long long* tile_end;   // grid mode: finish time of each tile
   unsigned long long* scene_end;  // per scene: latest finish time (wall clock)
   unsigned long long* t_start;    // earliest wave start (wall clock)
 };
 
-hipError_t launch_batch(const BatchDev& b, int block, int grid, hipStream_t stream);
+// cls: scene class of a tile-mode launch, escape + 2 spheres
+hipError_t launch_batch(const BatchDev& b, int cls, int block, int grid, hipStream_t stream);
 hipError_t launch_batch_band(const BatchDev& b, hipStream_t stream);
+hipError_t launch_batch_grid_finish(const BatchDev& b, hipStream_t stream);
 hipError_t launch_batch_clear(const BatchDev& b, hipStream_t stream);
 
 // ---------------------------------------------------------------------------

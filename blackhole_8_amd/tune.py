@@ -18,11 +18,11 @@ from . import abi
 CACHE_ENV = "BH8_TUNE_CACHE"
 DEFAULT_CACHE = Path.home() / ".cache" / "blackhole_8_amd" / "batch_tuning.json"
 
-# refill_idle = 64 is the tile mode (refill_steps unused there); smaller
-# values are the lane-refill mode
-GRID = ([dict(block=b, waves_per_simd=w, refill_steps=4, refill_idle=64, chunk=c)
+# the three modes: grid (block size only), persistent tiles, lane refill
+GRID = ([dict(mode=abi.BATCH_GRID, block=b) for b in (64, 128, 256)] +
+        [dict(mode=abi.BATCH_TILES, block=b, waves_per_simd=w, chunk=c)
          for b in (64, 256) for w in (2, 4, 8) for c in (1, 4, 16)] +
-        [dict(block=b, waves_per_simd=w, refill_steps=k, refill_idle=i, chunk=4)
+        [dict(mode=abi.BATCH_LANES, block=b, waves_per_simd=w, refill_steps=k, refill_idle=i)
          for b in (64, 256) for w in (2, 4) for k in (4, 8, 16) for i in (16, 32, 48)])
 
 

@@ -258,16 +258,23 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
 /* counterpart: the reference renders one scene at a time,                  */
 /* blackhole_solution_test.cc:161-298)                                      */
 /* ======================================================================== */
-/* Scheduling knobs of the persistent batch kernel (0 = default). */
+/* Scheduling of a batch (0 = default).  Modes:
+ *  BH_BATCH_GRID   one launch, a wave per 8x8 tile of every scene (default);
+ *  BH_BATCH_TILES  persistent waves take ranges of tiles from a device queue;
+ *  BH_BATCH_LANES  persistent waves whose lanes each take rays from the queue
+ *                  and refill individually once refill_idle of a wave's 64
+ *                  lanes are idle (ray compaction; sphere scenes go solo). */
+#define BH_BATCH_GRID 1
+#define BH_BATCH_TILES 2
+#define BH_BATCH_LANES 3
 typedef struct bh_batch_tuning {
+  int32_t mode;           /* BH_BATCH_* (BH_BATCH_GRID)                         */
   int32_t block;          /* threads per workgroup: 64, 128 or 256 (256)       */
-  int32_t waves_per_simd; /* persistent waves per SIMD, 1..8 (4)              */
-  int32_t refill_steps;   /* steps between refill checks, 1..64 (4)            */
-  int32_t refill_idle;    /* 64 (default): each wave marches whole 8x8 tiles;
-                             below 64: lanes refill individually once this many
-                             of a wave's 64 lanes are idle (ray compaction)    */
-  int32_t chunk;          /* least 8x8 tiles per queue pop, 1..1024 (4); pops
-                             take max(chunk, remaining / (4 waves))            */
+  int32_t waves_per_simd; /* persistent modes: waves per SIMD, 1..8 (4)       */
+  int32_t refill_steps;   /* lanes mode: steps between refill checks (4)       */
+  int32_t refill_idle;    /* lanes mode: refill once this many lanes idle (16) */
+  int32_t chunk;          /* persistent modes: least 8x8 tiles per queue pop
+                             (4); pops take max(chunk, remaining / (4 waves))  */
 } bh_batch_tuning;
 
 typedef struct bh_batch_stats {

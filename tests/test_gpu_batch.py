@@ -49,10 +49,13 @@ def test_batch_equals_solo_renders(renderer, c3):
     assert sum(int((r["hit"] == abi.HIT_DISC).sum()) for r in res) > 1000
 
 
-@pytest.mark.parametrize("tuning", [dict(block=64, waves_per_simd=1, refill_steps=1, refill_idle=1),
-                                    dict(block=256, waves_per_simd=8, refill_steps=8, refill_idle=64),
-                                    dict(block=64, waves_per_simd=2, refill_steps=16, refill_idle=48),
-                                    dict(block=128, waves_per_simd=4, refill_steps=3, refill_idle=32)])
+@pytest.mark.parametrize("tuning", [
+    dict(mode=abi.BATCH_GRID, block=64),
+    dict(mode=abi.BATCH_TILES, block=256, waves_per_simd=8, chunk=1),
+    dict(mode=abi.BATCH_TILES, block=64, waves_per_simd=2, chunk=16),
+    dict(mode=abi.BATCH_LANES, block=64, waves_per_simd=1, refill_steps=1, refill_idle=1),
+    dict(mode=abi.BATCH_LANES, block=64, waves_per_simd=2, refill_steps=16, refill_idle=48),
+    dict(mode=abi.BATCH_LANES, block=128, waves_per_simd=4, refill_steps=3, refill_idle=32)])
 def test_scheduling_knobs_do_not_change_pixels(renderer, c3, tuning):
     items, tex = c3
     renderer.set_textures(tex)
@@ -106,7 +109,7 @@ def test_batch_with_fp64_and_sphere_scenes(renderer, c3):
     _assert_same_as_solo(renderer, mixed, res)
     # the lane-refill mode keeps no sphere table: there the sphere scene goes solo
     res, ms, st = renderer.render_batch(mixed, timing=True,
-                                        tuning=abi.BatchTuning(refill_idle=16))
+                                        tuning=abi.BatchTuning(mode=abi.BATCH_LANES))
     assert st.solo_scenes == 2
     _assert_same_as_solo(renderer, mixed, res)
 
@@ -115,9 +118,9 @@ def test_autotune_picks_and_caches(renderer, c3, tmp_path):
     items, tex = c3
     renderer.set_textures(tex)
     sub = _small(items, 256, 64)
-    grid = [dict(block=64, waves_per_simd=2, refill_steps=4, refill_idle=16),
-            dict(block=256, waves_per_simd=8, refill_steps=4, refill_idle=16),
-            dict(block=256, waves_per_simd=4, refill_steps=8, refill_idle=32)]
+    grid = [dict(mode=abi.BATCH_GRID, block=256),
+            dict(mode=abi.BATCH_TILES, block=256, waves_per_simd=4, chunk=4),
+            dict(mode=abi.BATCH_LANES, block=256, waves_per_simd=4, refill_steps=8, refill_idle=32)]
     cache = tmp_path / "tune.json"
     print("[case] autotune 3 candidates", flush=True)
     best, rec = tune.autotune(renderer, sub, grid=grid, reps=1, cache=cache)
@@ -125,4 +128,4 @@ def test_autotune_picks_and_caches(renderer, c3, tmp_path):
     assert rec["best_ms"] == min(g["ms"] for g in rec["grid"])
     best2, rec2 = tune.autotune(renderer, sub, grid=grid, reps=1, cache=cache)
     assert rec2 == rec
-    assert (best2.block, best2.refill_idle) == (best.block, best.refill_idle)
+    assert (best2.mode, best2.block) == (best.mode, best.block)
