@@ -147,6 +147,19 @@ SIGNATURES = {
     "bh_diff_time": (C.c_int, [P, C.POINTER(DiffParams), C.c_int, C.c_int,
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bh_render_flat": (C.c_int, [P, C.POINTER(FlatScene), C.POINTER(Outputs)]),
+    "bh_render_flat_device": (C.c_int, [P, C.POINTER(FlatScene), C.POINTER(Outputs)]),
+    "bh_ring_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
+    "bh_ring_render": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options), C.POINTER(C.c_int)]),
+    "bh_ring_render_flat": (C.c_int, [P, C.POINTER(FlatScene), C.POINTER(C.c_int)]),
+    "bh_ring_wait": (C.c_int, [P, C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(C.c_float)]),
+    "bh_ring_release": (C.c_int, [P, C.c_int]),
+    "bh_ring_destroy": (None, [P]),
+    "bh_png_decode": (C.c_int, [P, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)]),
+    "bh_image_load": (C.c_int, [C.c_char_p, P, C.c_size_t, C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)]),
+    "bh_image_write": (C.c_int, [C.c_char_p, P, C.c_int, C.c_int]),
+    "bh_io_last_error": (C.c_char_p, []),
     "bh_focal_length": (C.c_double, [C.c_int, C.c_double]),
     "bh_object_rotate": (None, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                 C.c_int, C.c_double]),

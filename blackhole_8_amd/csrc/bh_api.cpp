@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1100,9 +1102,8 @@ int bh_batch_autotune(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_optio
   return BH_OK;
 }
 
-int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!sc || !out) return fail(ctx, BH_ERR_INVALID_ARG, "scene and outputs are required");
+static int build_flat(bh_ctx* ctx, const bh_flat_scene* sc, bh::DevFlat* dp) {
+  if (!sc) return fail(ctx, BH_ERR_INVALID_ARG, "scene is NULL");
   const bh_camera& c = sc->camera;
   if (c.width <= 0 || c.height <= 0 || c.width > 32768 || c.height > 32768)
     return fail(ctx, BH_ERR_INVALID_ARG, "camera width/height must be in 1..32768");
@@ -1114,7 +1115,7 @@ int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) 
     return fail(ctx, BH_ERR_INVALID_ARG, "n_rects / n_planes out of range");
   if (sc->steps < 1 || sc->steps > 1000000)
     return fail(ctx, BH_ERR_INVALID_ARG, "steps must be in 1..1e6");
-  bh::DevFlat d;
+  bh::DevFlat& d = *dp;
   std::memset(&d, 0, sizeof d);
   copy3(d.C, c.position);
   copy3(d.vy, c.axis_y);
@@ -1146,10 +1147,30 @@ int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) 
     copy3(d.pz[i], p.axis_z);
     d.psize[i] = p.pattern_size;
   }
+  return BH_OK;
+}
+
+int bh_render_flat_device(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
+  bh::DevFlat d;
+  int rc = build_flat(ctx, sc, &d);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  DevOut o{out->rgb, out->hit, out->phi_end, out->hit_point, out->steps};
+  HIP_TRY(ctx, bh::launch_flat(d, o, ctx->stream));
+  return BH_OK;
+}
+
+int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (!sc || !out) return fail(ctx, BH_ERR_INVALID_ARG, "scene and outputs are required");
+  bh::DevFlat d;
+  int rc = build_flat(ctx, sc, &d);
+  if (rc) return rc;
   (void)hipSetDevice(ctx->device);
   const size_t npix = (size_t)d.W * d.H;
   DevOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
-  int rc;
   if (out->rgb) { if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc; o.rgb = (uint8_t*)ctx->rgb.p; }
   if (out->hit) { if ((rc = ensure(ctx, ctx->hit, npix))) return rc; o.hit = (uint8_t*)ctx->hit.p; }
   if (out->phi_end) { if ((rc = ensure(ctx, ctx->phi, npix * 8))) return rc; o.phi = (double*)ctx->phi.p; }
@@ -1164,6 +1185,159 @@ int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) 
   if (out->steps) HIP_TRY(ctx, hipMemcpyAsync(out->steps, o.steps, npix * 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
   return BH_OK;
+}
+
+// ---- frame ring ------------------------------------------------------------
+struct bh_ring {
+  struct Slot {
+    uint8_t* d_rgb = nullptr;
+    uint8_t* d_hit = nullptr;
+    uint8_t* h_rgb = nullptr;  // pinned
+    uint8_t* h_hit = nullptr;  // pinned
+    hipEvent_t start = nullptr, done = nullptr;
+    bool busy = false;  // queued or held by the host
+  };
+  bh_ctx* ctx = nullptr;
+  int W = 0, H = 0, next = 0;
+  std::vector<Slot> slots;
+  std::mutex m;
+  std::condition_variable cv;
+};
+
+static void ring_free(bh_ring* r) {
+  for (auto& s : r->slots) {
+    if (s.d_rgb) (void)hipFree(s.d_rgb);
+    if (s.d_hit) (void)hipFree(s.d_hit);
+    if (s.h_rgb) (void)hipHostFree(s.h_rgb);
+    if (s.h_hit) (void)hipHostFree(s.h_hit);
+    if (s.start) (void)hipEventDestroy(s.start);
+    if (s.done) (void)hipEventDestroy(s.done);
+  }
+  delete r;
+}
+
+int bh_ring_create(bh_ctx* ctx, int width, int height, int depth, bh_ring** out) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (width <= 0 || height <= 0 || width > 32768 || height > 32768)
+    return fail(ctx, BH_ERR_INVALID_ARG, "width/height must be in 1..32768");
+  if (depth < 2 || depth > 8) return fail(ctx, BH_ERR_INVALID_ARG, "depth must be in 2..8");
+  (void)hipSetDevice(ctx->device);
+  bh_ring* r = new bh_ring();
+  r->ctx = ctx;
+  r->W = width;
+  r->H = height;
+  r->slots.resize(depth);
+  const size_t npix = (size_t)width * height;
+  for (auto& s : r->slots) {
+    hipError_t e = hipMalloc(&s.d_rgb, npix * 3);
+    if (e == hipSuccess) e = hipMalloc(&s.d_hit, npix);
+    if (e == hipSuccess) e = hipHostMalloc(&s.h_rgb, npix * 3, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&s.h_hit, npix, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreate(&s.start);
+    if (e == hipSuccess) e = hipEventCreate(&s.done);
+    if (e != hipSuccess) {
+      ring_free(r);
+      return hip_fail(ctx, e, "frame ring allocation");
+    }
+  }
+  *out = r;
+  return BH_OK;
+}
+
+// Take the next slot, waiting while the host still holds it.
+static bh_ring::Slot* ring_acquire(bh_ring* r, int* idx) {
+  std::unique_lock<std::mutex> lk(r->m);
+  bh_ring::Slot* s = &r->slots[r->next];
+  r->cv.wait(lk, [s] { return !s->busy; });
+  *idx = r->next;
+  r->next = (r->next + 1) % (int)r->slots.size();
+  return s;
+}
+
+static int ring_finish(bh_ring* r, bh_ring::Slot* s, int idx, int rc, int* slot) {
+  bh_ctx* ctx = r->ctx;
+  if (rc) return rc;
+  const size_t npix = (size_t)r->W * r->H;
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipMemcpyAsync(s->h_rgb, s->d_rgb, npix * 3, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(s->h_hit, s->d_hit, npix, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipEventRecord(s->done, st));
+  {
+    std::lock_guard<std::mutex> lk(r->m);
+    s->busy = true;
+  }
+  *slot = idx;
+  return BH_OK;
+}
+
+int bh_ring_render(bh_ring* r, const bh_scene* scene, const bh_options* opt, int* slot) {
+  if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
+  bh_ctx* ctx = r->ctx;
+  if (!scene || !slot) return fail(ctx, BH_ERR_INVALID_ARG, "scene/slot are NULL");
+  if (scene->camera.width != r->W || scene->camera.height != r->H)
+    return fail(ctx, BH_ERR_INVALID_ARG, "camera size differs from the ring's");
+  int idx;
+  bh_ring::Slot* s = ring_acquire(r, &idx);
+  (void)hipSetDevice(ctx->device);
+  HIP_TRY(ctx, hipEventRecord(s->start, ctx->stream));
+  bh_outputs o{s->d_rgb, s->d_hit, nullptr, nullptr, nullptr};
+  return ring_finish(r, s, idx, bh_render_device(ctx, scene, opt, &o), slot);
+}
+
+int bh_ring_render_flat(bh_ring* r, const bh_flat_scene* scene, int* slot) {
+  if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
+  bh_ctx* ctx = r->ctx;
+  if (!scene || !slot) return fail(ctx, BH_ERR_INVALID_ARG, "scene/slot are NULL");
+  if (scene->camera.width != r->W || scene->camera.height != r->H)
+    return fail(ctx, BH_ERR_INVALID_ARG, "camera size differs from the ring's");
+  int idx;
+  bh_ring::Slot* s = ring_acquire(r, &idx);
+  (void)hipSetDevice(ctx->device);
+  HIP_TRY(ctx, hipEventRecord(s->start, ctx->stream));
+  bh_outputs o{s->d_rgb, s->d_hit, nullptr, nullptr, nullptr};
+  return ring_finish(r, s, idx, bh_render_flat_device(ctx, scene, &o), slot);
+}
+
+int bh_ring_wait(bh_ring* r, int slot, const uint8_t** rgb, const uint8_t** hit, float* gpu_ms) {
+  if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
+  if (slot < 0 || slot >= (int)r->slots.size())
+    return fail(nullptr, BH_ERR_INVALID_ARG, "slot out of range");
+  bh_ring::Slot& s = r->slots[slot];
+  {
+    std::lock_guard<std::mutex> lk(r->m);
+    if (!s.busy) return fail(nullptr, BH_ERR_INVALID_ARG, "slot holds no frame");
+  }
+  hipError_t e = hipEventSynchronize(s.done);
+  if (e != hipSuccess) return fail(nullptr, BH_ERR_HIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+  if (gpu_ms) {
+    e = hipEventElapsedTime(gpu_ms, s.start, s.done);
+    if (e != hipSuccess) *gpu_ms = -1.0f;
+  }
+  if (rgb) *rgb = s.h_rgb;
+  if (hit) *hit = s.h_hit;
+  return BH_OK;
+}
+
+int bh_ring_release(bh_ring* r, int slot) {
+  if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
+  if (slot < 0 || slot >= (int)r->slots.size())
+    return fail(nullptr, BH_ERR_INVALID_ARG, "slot out of range");
+  {
+    std::lock_guard<std::mutex> lk(r->m);
+    r->slots[slot].busy = false;
+  }
+  r->cv.notify_all();
+  return BH_OK;
+}
+
+void bh_ring_destroy(bh_ring* r) {
+  if (!r) return;
+  (void)hipSetDevice(r->ctx->device);
+  for (auto& s : r->slots)
+    if (s.done) (void)hipEventSynchronize(s.done);
+  ring_free(r);
 }
 
 }  // extern "C"

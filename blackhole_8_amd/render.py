@@ -182,3 +182,56 @@ class Path:
         if self.handle:
             self.r.lib.bh_path_destroy(self.handle)
             self.handle = C.c_void_p()
+
+
+class Ring:
+    """Frame ring (bh_ring_*): frames render into device slots and are copied
+    to pinned host memory while the host consumes earlier slots."""
+
+    def __init__(self, r: Renderer, width: int, height: int, depth: int = 2):
+        self.r = r
+        self.w, self.h = width, height
+        self.handle = C.c_void_p()
+        r._check(r.lib.bh_ring_create(r.ctx, width, height, depth, C.byref(self.handle)))
+
+    def render(self, scene) -> int:
+        slot = C.c_int()
+        if isinstance(scene, abi.FlatScene):
+            rc = self.r.lib.bh_ring_render_flat(self.handle, C.byref(scene), C.byref(slot))
+        else:
+            s, o = scene
+            rc = self.r.lib.bh_ring_render(self.handle, C.byref(s), C.byref(o), C.byref(slot))
+        self.r._check(rc)
+        return slot.value
+
+    def wait(self, slot: int, copy: bool = True):
+        """(rgb, hit, gpu_ms) of slot's frame; copies out of the pinned slot
+        unless copy is False (then the arrays alias it until release)."""
+        rgb, hit, ms = C.c_void_p(), C.c_void_p(), C.c_float()
+        rc = self.r.lib.bh_ring_wait(self.handle, slot, C.byref(rgb), C.byref(hit), C.byref(ms))
+        if rc != abi.BH_OK:
+            raise BHError(rc, self.r.lib.bh_last_error(None).decode())
+        n = self.w * self.h
+        a = np.ctypeslib.as_array(C.cast(rgb, C.POINTER(C.c_uint8)), (n * 3,))
+        b = np.ctypeslib.as_array(C.cast(hit, C.POINTER(C.c_uint8)), (n,))
+        a = a.reshape(self.h, self.w, 3)
+        b = b.reshape(self.h, self.w)
+        if copy:
+            a, b = a.copy(), b.copy()
+        return a, b, ms.value
+
+    def release(self, slot: int):
+        rc = self.r.lib.bh_ring_release(self.handle, slot)
+        if rc != abi.BH_OK:
+            raise BHError(rc, self.r.lib.bh_last_error(None).decode())
+
+    def close(self):
+        if self.handle:
+            self.r.lib.bh_ring_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -334,6 +334,61 @@ int bh_diff_time(bh_ctx* ctx, const bh_diff_params* p, int precision, int iters,
 /* Flat-space march — RayTracer::Prograde (ray_tracer.h:52-96)              */
 /* ======================================================================== */
 int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* scene, const bh_outputs* out);
+/* The same into device buffers on the context stream (asynchronous). */
+int bh_render_flat_device(bh_ctx* ctx, const bh_flat_scene* scene, const bh_outputs* out_device);
+
+/* ======================================================================== */
+/* Frame ring — the double-buffered screen hand-off of                       */
+/* blackhole_solution_test.cc:117-145,327-334 and ray_tracer_test.cc:97-181  */
+/* (two cv::Mat screens swapped and cleared by a thread, the finished one   */
+/* written out while the next frame renders)                                 */
+/* ======================================================================== */
+/* depth (2..8) slots, each a device rgb/hit frame and a pinned host copy.
+ * A render call queues the march of one frame into the next slot and the
+ * copy of its planes to the slot's pinned memory, and returns at once
+ * (waiting first while that slot is still held by the host).  wait gives the
+ * host planes once the frame is there; release hands the slot back.  Every
+ * pixel is written by the march, so no clearing pass runs between frames.
+ * bh_ring_wait and bh_ring_release may run on another thread than the render
+ * calls (the writer thread); their error message is then read with
+ * bh_last_error(NULL) on that thread. */
+typedef struct bh_ring bh_ring;
+int bh_ring_create(bh_ctx* ctx, int width, int height, int depth, bh_ring** out);
+/* Frame of scene into the next slot; *slot receives its index.  The scene's
+ * camera must have the ring's width and height. */
+int bh_ring_render(bh_ring* ring, const bh_scene* scene, const bh_options* opt, int* slot);
+int bh_ring_render_flat(bh_ring* ring, const bh_flat_scene* scene, int* slot);
+/* Block until slot's frame is on the host; rgb (H*W*3) and hit (H*W) point
+ * into the slot's pinned memory until it is released; gpu_ms (optional) =
+ * march plus copy time from HIP events. */
+int bh_ring_wait(bh_ring* ring, int slot, const uint8_t** rgb, const uint8_t** hit,
+                 float* gpu_ms);
+int bh_ring_release(bh_ring* ring, int slot);
+/* Waits for every queued frame, then frees the ring. */
+void bh_ring_destroy(bh_ring* ring);
+
+/* ======================================================================== */
+/* Image files (host only, no GPU needed) — texture input in place of       */
+/* cv::imread(path, IMREAD_COLOR) (config.h:35-37) and frame output in      */
+/* place of cv::VideoWriter (blackhole_solution_test.cc:69-72,330)          */
+/* ======================================================================== */
+/* Decode a PNG held in memory to rows x cols x 3 bytes in B, G, R order with
+ * IMREAD_COLOR semantics: alpha and tRNS dropped (no compositing), grey
+ * replicated, grey below 8 bits scaled to 0..255, 16-bit samples keep the
+ * high byte, palette looked up, gamma chunks not applied; interlaced files
+ * are read.  dst NULL: only *rows / *cols are set.  Errors:
+ * BH_ERR_INVALID_ARG (malformed, CRC, truncated, dst too small),
+ * BH_ERR_UNSUPPORTED (unknown critical chunk, more than 2^28 pixels). */
+int bh_png_decode(const uint8_t* data, size_t size, uint8_t* dst, size_t dst_size, int* rows,
+                  int* cols);
+/* The same from a file holding a PNG or a binary PPM (P6, maxval 255). */
+int bh_image_load(const char* path, uint8_t* dst, size_t dst_size, int* rows, int* cols);
+/* Write rows x cols x 3 bytes in B, G, R order (the framebuffer's order for
+ * textures loaded as above) to path: ".ppm" binary PPM or ".png" 8-bit RGB
+ * PNG; both files store R, G, B. */
+int bh_image_write(const char* path, const uint8_t* bgr, int rows, int cols);
+/* Message of the last failing image-file call on this thread. */
+const char* bh_io_last_error(void);
 
 /* ======================================================================== */
 /* Host-side helpers shared by every caller (no GPU needed)                  */

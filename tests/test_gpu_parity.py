@@ -163,7 +163,8 @@ def test_error_paths(renderer):
     p = scenes.config1(64, 36, frames=1)
     renderer.set_textures(p.textures)
     s = abi.Scene.from_buffer_copy(p.scene)
-    s.disc.texture = 40  # empty slot -> Material::SetTexture failure
+    assert renderer.lib.bh_texture_clear(renderer.ctx, 63) == abi.BH_OK
+    s.disc.texture = 63  # empty slot -> Material::SetTexture failure
     assert renderer.check_scene(s, p.options) == abi.BH_ERR_EMPTY_TEXTURE
     s = abi.Scene.from_buffer_copy(p.scene)
     s.hole.mass = 0.0
