@@ -102,7 +102,7 @@ def adjoint_timing(iters: int = 50) -> dict:
         fwd, fwd_adj = dr.time(diff.FIT_TRUE, iters)
         pl = scenes.config4_plain(diff.FIT_TRUE, dr.cfg.width, dr.cfg.height, dr.cfg.fov,
                                   dr.cfg.r_outer)
-        plain, _, _ = r.time_render(pl.scene, pl.options, iters)
+        plain, plain_frame, _ = r.time_render(pl.scene, pl.options, iters)
     finally:
         r.close()
     return {"workload": "config4: 512x512 hole + disc, r_s 1.2, inclination 72 deg, camera at "
@@ -110,7 +110,32 @@ def adjoint_timing(iters: int = 50) -> dict:
             "fwd_ms": round(fwd, 5), "fwd_adj_ms": round(fwd_adj, 5),
             "ratio_vs_diff_fwd": round(fwd_adj / fwd, 3),
             "plain_fwd_ms": round(plain, 5), "ratio_vs_plain_fwd": round(fwd_adj / plain, 3),
+            "plain_frame_ms": round(plain_frame, 5),
+            "ratio_vs_plain_frame": round(fwd_adj / plain_frame, 3),
+            "note": "plain_fwd_ms is the plain march kernel alone; plain_frame_ms adds its "
+                    "FP64 photon-sphere band pass (eager, serial)",
             "iters": iters}
+
+
+def flat_timing(iters: int = 50) -> dict:
+    """The flat-space scene of the ray_tracer_test program (FP64, the
+    reference's operation order): kernel time per frame at the reference's
+    800x450 and at 1920x1080."""
+    from blackhole_8_amd import Renderer, scenes
+
+    r = Renderer(0)
+    out = {"workload": "ray_tracer_test scene: 3 textured rectangles + chess plane, "
+                       "RayTracer::Prograde 10 chords per pixel, FP64"}
+    try:
+        for w, h in ((800, 450), (1920, 1080)):
+            s, tex = scenes.flat_test(w, h)
+            r.set_textures(tex)
+            ms = r.time_flat(s, iters)
+            out[f"{w}x{h}"] = {"kernel_ms": round(ms, 5),
+                               "mrays_per_s": round(w * h / ms / 1e3, 1)}
+    finally:
+        r.close()
+    return out
 
 
 def main() -> int:
@@ -127,6 +152,9 @@ def main() -> int:
                     help="only time the 512x512 forward / forward+adjoint and exit")
     ap.add_argument("--batch-only", action="store_true",
                     help="only time the 512-scene batch and exit")
+    ap.add_argument("--no-flat", action="store_true")
+    ap.add_argument("--flat-only", action="store_true",
+                    help="only time the flat-space (ray_tracer_test) scene and exit")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 times), for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters, and exit")
@@ -149,6 +177,9 @@ def main() -> int:
         return 0
     if args.batch_only:
         print(json.dumps({"batch": batch_timing()}))
+        return 0
+    if args.flat_only:
+        print(json.dumps({"flat": flat_timing()}))
         return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -250,6 +281,7 @@ def main() -> int:
 
     adjoint = None if args.no_adjoint else adjoint_timing()
     batch = None if args.no_batch else batch_timing()
+    flat = None if args.no_flat else flat_timing()
 
     line = {
         "metric": "Mrays/s at 1080p BH+disc; fwd+adjoint ms/frame vs plain fwd; % FP32 peak",
@@ -275,6 +307,7 @@ def main() -> int:
         "accuracy_frame0": accuracy,
         "adjoint": adjoint,
         "batch": batch,
+        "flat": flat,
     }
     print(json.dumps(line))
     return 0

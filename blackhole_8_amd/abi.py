@@ -148,6 +148,7 @@ SIGNATURES = {
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bh_render_flat": (C.c_int, [P, C.POINTER(FlatScene), C.POINTER(Outputs)]),
     "bh_render_flat_device": (C.c_int, [P, C.POINTER(FlatScene), C.POINTER(Outputs)]),
+    "bh_time_flat": (C.c_int, [P, C.POINTER(FlatScene), C.c_int, C.POINTER(C.c_double)]),
     "bh_ring_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
     "bh_ring_render": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options), C.POINTER(C.c_int)]),
     "bh_ring_render_flat": (C.c_int, [P, C.POINTER(FlatScene), C.POINTER(C.c_int)]),

@@ -40,8 +40,8 @@ struct Args {
   }
 };
 
-// The key bindings of the reference's loop (blackhole_solution_test.cc:351-391,
-// ray_tracer_test.cc:190-226), one key applied after each frame.  Upper case
+// The key bindings of the reference's loop (blackhole_solution_test.cc:351-406,
+// ray_tracer_test.cc:196-235), one key applied after each frame.  Upper case
 // moves or turns `faster` times as far.  '-' and '+' change the field of view
 // by pi/54.  Returns false for a key the camera does not handle.
 inline bool camera_key(bh::Camera& cam, char key, double move_d, double rotate_d, double faster) {

@@ -7,7 +7,7 @@
 // clean thread, and PPM/PNG files in place of the MJPG video.
 //
 // The reference's window, HUD text and live keyboard are out of scope; a key
-// script (--keys) replays its bindings, one key after each frame (:351-400):
+// script (--keys) replays its bindings, one key after each frame (:351-406):
 // w/s a/d z/x move the camera along its x/y/z axes by 10, q/e i/k j/l turn it
 // about them by pi/1800, upper case 10 times as far, -/+ change the field of
 // view by pi/54, and ^ v < > , . move the hole by 5 along z, -z, y, -y, x, -x

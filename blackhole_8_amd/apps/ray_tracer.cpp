@@ -4,14 +4,14 @@
 // RotateZ(pi/12); three textured rectangles; the ground plane z = 0 with
 // ChessPattern2D(10); RayTracer::Prograde with 10 chords per pixel, :50-95,
 // :145) rendered on the GPU through the C ABI, with the reference's per-frame
-// "movement test" (:229-261): the second rectangle slides along x by 3 and
+// "movement test" (:237-261): the second rectangle slides along x by 3 and
 // turns back outside 0..100, the first turns by pi/180 about an axis shifted
 // by (60, 50) along its y/z axes, the third by pi/120 about an axis shifted by
 // (-100, 50) along its x/z axes.
 //
 // Frames go through a frame ring to PPM/PNG files in place of the window and
 // MJPG video.  A key script (--keys) replays the reference's bindings, one key
-// after each frame (:190-226): moves by 0.9, turns by pi/180, upper case 10
+// after each frame (:196-235): moves by 0.9, turns by pi/180, upper case 10
 // times as far, -/+ change the field of view by pi/54.
 #include <cstdio>
 #include <memory>

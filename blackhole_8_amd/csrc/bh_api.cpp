@@ -1187,6 +1187,29 @@ int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) 
   return BH_OK;
 }
 
+int bh_time_flat(bh_ctx* ctx, const bh_flat_scene* sc, int iters, double* avg_ms) {
+  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+  if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
+  bh::DevFlat d;
+  int rc = build_flat(ctx, sc, &d);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const size_t npix = (size_t)d.W * d.H;
+  if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc;
+  if ((rc = ensure(ctx, ctx->hit, npix))) return rc;
+  DevOut o{(uint8_t*)ctx->rgb.p, (uint8_t*)ctx->hit.p, nullptr, nullptr, nullptr};
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, bh::launch_flat(d, o, st));  // warm-up
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+  for (int it = 0; it < iters; ++it) HIP_TRY(ctx, bh::launch_flat(d, o, st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (avg_ms) *avg_ms = ms / iters;
+  return BH_OK;
+}
+
 // ---- frame ring ------------------------------------------------------------
 struct bh_ring {
   struct Slot {

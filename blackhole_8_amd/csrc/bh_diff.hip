@@ -14,14 +14,20 @@
 // Adjoint (reverse mode, fused with the forward in one kernel):
 //   pass 1   forward march; each disc crossing records (psi_{j-1}, psi_j, T_k)
 //            in LDS and sets bit j of a chord mask;
-//   pass 2   reverse over chords j = j_max .. 1.  The trajectory is not
-//            stored: the ray set-up is the one checkpoint, u_j and w_j are
-//            recomputed in closed form, and psi at a crossing comes from its
-//            record.  Compositing adjoint with the behind colour B built back
-//            to front; crossing point, coverage and bilinear texel adjoints
-//            give P_bar, hence (psi_bar, u_bar, e1_bar, e2_bar); suffix sums
-//            A_j of psi_bar give w_bar_j = h (A_j + A_{j+1}) and
-//            G_bar = -w^3 w_bar / 2, hence (u, M, 1/b^2);
+//   pass 2   reverse over the crossing records only.  The angle psi_m =
+//            h sum_{i <= m} (w_{i-1} + w_i) makes w_i enter every later angle,
+//            so the adjoint of w_i is h (A_i + A_{i+1}) with A_i the sum of
+//            the angle adjoints injected at m >= i.  Angle adjoints arise only
+//            at the two ends of a crossing chord, so A is piecewise constant
+//            and sum_i (A_i + A_{i+1}) X_i = sum_m g_m (S(m) + S(m - 1)) for
+//            the prefix sums S of each per-step factor X_i of dw_i
+//            (w^3 G', kappa w^3 G', w^3 u^3, w^3; w itself gives psi_m / h).
+//            Pass 1 carries the four prefix sums and records them at each
+//            crossing; S(m - 1), S(m - 2) follow by recomputing X_m, X_{m-1}
+//            in closed form.  No step of the march is revisited.  Compositing
+//            adjoint with the behind colour B built back to front; crossing
+//            point, coverage and bilinear texel adjoints give P_bar, hence
+//            (psi_bar, u_bar, e1_bar, e2_bar);
 //   set-up   turning point by implicit differentiation of G(l) = 0, then the
 //            orbit-plane frame in forward mode (tangents: inclination,
 //            distance) contracted with the set-up adjoints.
@@ -397,7 +403,25 @@ template <typename T> struct Acc {
 template <typename T> struct Rec {
   double* psi;  // psi_{j-1} at psi[2k kThreads], psi_j at psi[(2k + 1) kThreads]
   T* tr;        // T_k at tr[k kThreads]
+  T* ps;        // prefix sums through step j at ps[(4k + q) kThreads], q < 4
 };
+
+// Per-step factors of dw_j (w = G^(-1/2), dw = -w^3 dG / 2 with
+// dG = G'(u) du + 2 u^3 dM + d(1/b^2) and du_j = du0 + kappa_j d(du)):
+// w^3 G', kappa w^3 G', w^3 u^3 and w^3, in the order of Rec::ps.
+template <typename T> struct StepX {
+  T w, x[4];
+};
+template <typename T>
+__device__ __forceinline__ StepX<T> step_x(const DRay<T>& d, int j, int n, T c3x3, T c2x2) {
+  const T sj = -sigma<T>(j, n) * d.du;
+  const T w = rsq(g_at(d, sj));
+  const T kj = kappa<T>(j, n);
+  const T u = fma_t(kj, d.du, d.u0);
+  const T w3 = w * w * w;
+  const T t = w3 * fma_t(sj, fma_t(c3x3, sj, c2x2), d.c1);
+  return {w, {t, kj * t, w3 * (u * u * u), w3}};
+}
 
 // Pass 1: the forward march.  With RECORD, crossing k stores
 // (psi_{j-1}, psi_j, T_k) in rec.  Chord j is examined only when psi_j has
@@ -428,6 +452,8 @@ __device__ inline void diff_march(const DiffScene& s, const DRay<T>& d, Rec<T> r
   double psi = 0.0;
   double a_ev = d.a0 - d.guard;
   T wp = (T)0, u_prev = d.u0;
+  T sx[4] = {(T)0, (T)0, (T)0, (T)0};  // RECORD: prefix sums of the StepX factors
+  const T c3x3 = (T)3 * d.c3, c2x2 = (T)2 * d.c2;
   for (int j = 1; j <= 2 * n - 1; ++j) {
     if (j > n && d.captured) {
       // chord to the centre: the horizon is entered, the disc plane is not
@@ -442,7 +468,16 @@ __device__ inline void diff_march(const DiffScene& s, const DRay<T>& d, Rec<T> r
     const double psi_prev = psi;
     psi += (double)dpsi;
     wp = w;
-    const T u = fma_t(kappa<T>(j, n), d.du, d.u0);
+    const T kj = kappa<T>(j, n);
+    const T u = fma_t(kj, d.du, d.u0);
+    if (RECORD) {
+      const T w3 = w * w * w;
+      const T t = w3 * fma_t(sj, fma_t(c3x3, sj, c2x2), d.c1);
+      sx[0] += t;
+      sx[1] = fma_t(kj, t, sx[1]);
+      sx[2] = fma_t(w3, u * u * u, sx[2]);
+      sx[3] += w3;
+    }
     const T umax = u > u_prev ? u : u_prev;
     const bool need_h = !(dpsi < (T)3) || fma_t(-dpsi * dpsi, (T)0.125, (T)1) <= Rh_hi * umax;
     const bool need_d = psi >= a_ev;
@@ -462,6 +497,8 @@ __device__ inline void diff_march(const DiffScene& s, const DRay<T>& d, Rec<T> r
             rec.psi[(2 * a.n + 0) * kThreads] = psi_prev;
             rec.psi[(2 * a.n + 1) * kThreads] = psi;
             rec.tr[a.n * kThreads] = a.tr;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) rec.ps[(4 * a.n + q) * kThreads] = sx[q];
             a.mask |= 1ull << j;
           } else {
             a.overflow = true;
@@ -520,6 +557,7 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
                                                         double* __restrict__ partial) {
   __shared__ double rec_psi[ADJ ? 2 * kMaxRec * kThreads : 1];
   __shared__ T rec_tr[ADJ ? kMaxRec * kThreads : 1];
+  __shared__ T rec_ps[ADJ ? 4 * kMaxRec * kThreads : 1];
   __shared__ double red[ADJ ? kPartial * (kThreads / 64) : 1];
   const int tid = threadIdx.x;
   const int px = blockIdx.x * kTile + (tid & (kTile - 1));
@@ -533,7 +571,7 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
     DRay<T> d;
     to_ray(s, su, d);
     Acc<T> a;
-    const Rec<T> rec{rec_psi + tid, rec_tr + tid};
+    const Rec<T> rec{rec_psi + tid, rec_tr + tid, rec_ps + tid};
     diff_march<T, ADJ>(s, d, rec, a);
     if (rgb_out) {
       rgb_out[3 * idx + 0] = (float)a.rgb[0];
@@ -547,97 +585,103 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
       const int n = s.nstep;
       T B[3] = {(T)0, (T)0, (T)0};
       T e1b[3] = {(T)0, (T)0, (T)0}, e2b[3] = {(T)0, (T)0, (T)0}, Cb[3] = {(T)0, (T)0, (T)0};
-      T A_next = (T)0, pend = (T)0, hb = (T)0, u0b = (T)0, dub = (T)0, Mb = (T)0, ib2b = (T)0,
-        rinb = (T)0;
-      int k = a.n < kMaxRec ? a.n - 1 : kMaxRec - 1;
-      const int jmax = 63 - __builtin_clzll(a.mask);
-      for (int j = jmax; j >= 1; --j) {
-        const T kj = kappa<T>(j, n);
-        const T uj = fma_t(kj, d.du, d.u0);
-        T Aj = A_next + pend;
-        pend = (T)0;
-        T ubj = (T)0;
-        if ((a.mask >> j) & 1) {
-          const double ps_a = rec.psi[(2 * k + 0) * kThreads];
-          const double ps_b = rec.psi[(2 * k + 1) * kThreads];
-          const T Tk = rec.tr[k * kThreads];
-          --k;
-          T P1[3], P2[3], ca = (T)0, sa = (T)0, ra = (T)0, cb, sb;
-          if (j == 1) {
+      T hb = (T)0, u0b = (T)0, dub = (T)0, Mb = (T)0, ib2b = (T)0, rinb = (T)0;
+      // sum over angle injections m of g_m (S(m) + S(m - 1)), per factor
+      T wx[4] = {(T)0, (T)0, (T)0, (T)0};
+      const T c3x3 = (T)3 * d.c3, c2x2 = (T)2 * d.c2;
+      const T ih = (T)1 / d.h;
+      int k = (a.n < kMaxRec ? a.n : kMaxRec) - 1;
+      uint64_t mask = a.mask;
+      while (mask) {
+        const int j = 63 - __builtin_clzll(mask);
+        mask &= ~(1ull << j);
+        const double ps_a = rec.psi[(2 * k + 0) * kThreads];
+        const double ps_b = rec.psi[(2 * k + 1) * kThreads];
+        const T Tk = rec.tr[k * kThreads];
+        T S0[4];
 #pragma unroll
-            for (int m = 0; m < 3; ++m) P1[m] = d.C[m];
-          } else {
-            ra = rcp(fma_t(kappa<T>(j - 1, n), d.du, d.u0));
-            orbit_point(d, ps_a, ra, P1, &ca, &sa);
-          }
-          const T rb = rcp(uj);
-          orbit_point(d, ps_b, rb, P2, &cb, &sb);
-          T q[3], sf, dd;
-          const T z1 = P1[2], z2 = P2[2];
-          disc_cross(s, P1, P2, q, &sf, &dd);
-          const T rho = sqrt_t(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-          T da_rho, da_rin;
-          const T al = coverage<T, true>(s, rho, &da_rho, &da_rin);
-          T c[3], cw[3], qb[3];
+        for (int q = 0; q < 4; ++q) S0[q] = rec.ps[(4 * k + q) * kThreads];
+        --k;
+        const T uj = fma_t(kappa<T>(j, n), d.du, d.u0);
+        T P1[3], P2[3], ca = (T)0, sa = (T)0, ra = (T)0, cb, sb;
+        if (j == 1) {
 #pragma unroll
-          for (int m = 0; m < 3; ++m) cw[m] = gb[m] * Tk * al;
-          disc_texel<T, true>(s, q, c, cw, qb);
-          // compositing: rgb = F + T_k al c + T_k (1 - al) B
-          T alb = (T)0;
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            alb = fma_t(gb[m], Tk * (c[m] - B[m]), alb);
-            B[m] = fma_t(al, c[m] - B[m], B[m]);
-          }
-          rinb = fma_t(alb, da_rin, rinb);
-          const T rhob = alb * da_rho;
-          if (rho > (T)0) {
-            const T kr = rhob / rho;
-#pragma unroll
-            for (int m = 0; m < 3; ++m) qb[m] = fma_t(kr, q[m], qb[m]);
-          }
-          // q = P1 + s (P2 - P1), s = z1 / (z1 - z2)
-          T sb_ = (T)0, P1b[3], P2b[3];
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            sb_ = fma_t(qb[m], P2[m] - P1[m], sb_);
-            P1b[m] = ((T)1 - sf) * qb[m];
-            P2b[m] = sf * qb[m];
-          }
-          const T den = z1 - z2;
-          const T kd = sb_ / (den * den);
-          P1b[2] = fma_t(-kd, z2, P1b[2]);
-          P2b[2] = fma_t(kd, z1, P2b[2]);
-          T phb, ub;
-          point_adjoint(d, P2b, cb, sb, rb, e1b, e2b, &phb, &ub);
-          Aj += phb;
-          ubj += ub;
-          if (j == 1) {
-#pragma unroll
-            for (int m = 0; m < 3; ++m) Cb[m] += P1b[m];
-          } else {
-            point_adjoint(d, P1b, ca, sa, ra, e1b, e2b, &phb, &ub);
-            pend = phb;
-            u0b += ub;
-            dub = fma_t(kappa<T>(j - 1, n), ub, dub);
-          }
+          for (int m = 0; m < 3; ++m) P1[m] = d.C[m];
+        } else {
+          ra = rcp(fma_t(kappa<T>(j - 1, n), d.du, d.u0));
+          orbit_point(d, ps_a, ra, P1, &ca, &sa);
         }
-        // psi_m = h sum_{i <= m} (w_{i-1} + w_i): w_j enters psi_m for
-        // m >= j twice over (increments j and j + 1)
-        const T sj = -sigma<T>(j, n) * d.du;
-        const T G = g_at(d, sj);
-        const T w = rsq(G);
-        const T A2 = Aj + A_next;
-        hb = fma_t(w, A2, hb);
-        const T Gb = (T)-0.5 * w * w * w * (d.h * A2);
-        const T Gu = fma_t(sj, fma_t((T)3 * d.c3, sj, (T)2 * d.c2), d.c1);
-        ubj = fma_t(Gb, Gu, ubj);
-        Mb = fma_t(Gb, (T)2 * uj * uj * uj, Mb);
-        ib2b += Gb;
-        u0b += ubj;
-        dub = fma_t(kj, ubj, dub);
-        A_next = Aj;
+        const T rb = rcp(uj);
+        orbit_point(d, ps_b, rb, P2, &cb, &sb);
+        T q[3], sf, dd;
+        const T z1 = P1[2], z2 = P2[2];
+        disc_cross(s, P1, P2, q, &sf, &dd);
+        const T rho = sqrt_t(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+        T da_rho, da_rin;
+        const T al = coverage<T, true>(s, rho, &da_rho, &da_rin);
+        T c[3], cw[3], qb[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) cw[m] = gb[m] * Tk * al;
+        disc_texel<T, true>(s, q, c, cw, qb);
+        // compositing: rgb = F + T_k al c + T_k (1 - al) B
+        T alb = (T)0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          alb = fma_t(gb[m], Tk * (c[m] - B[m]), alb);
+          B[m] = fma_t(al, c[m] - B[m], B[m]);
+        }
+        rinb = fma_t(alb, da_rin, rinb);
+        const T rhob = alb * da_rho;
+        if (rho > (T)0) {
+          const T kr = rhob / rho;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) qb[m] = fma_t(kr, q[m], qb[m]);
+        }
+        // q = P1 + s (P2 - P1), s = z1 / (z1 - z2)
+        T sb_ = (T)0, P1b[3], P2b[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          sb_ = fma_t(qb[m], P2[m] - P1[m], sb_);
+          P1b[m] = ((T)1 - sf) * qb[m];
+          P2b[m] = sf * qb[m];
+        }
+        const T den = z1 - z2;
+        const T kd = sb_ / (den * den);
+        P1b[2] = fma_t(-kd, z2, P1b[2]);
+        P2b[2] = fma_t(kd, z1, P2b[2]);
+        // the chord's end point: angle psi_j (injection m = j), radius 1/u_j
+        T phb, ub;
+        point_adjoint(d, P2b, cb, sb, rb, e1b, e2b, &phb, &ub);
+        u0b += ub;
+        dub = fma_t(kappa<T>(j, n), ub, dub);
+        const StepX<T> xj = step_x(d, j, n, c3x3, c2x2);
+        T S1[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          S1[m] = S0[m] - xj.x[m];
+          wx[m] = fma_t(phb, S0[m] + S1[m], wx[m]);
+        }
+        hb = fma_t(phb, (T)ps_b * ih, hb);
+        if (j == 1) {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) Cb[m] += P1b[m];
+        } else {
+          // the start point: angle psi_{j-1} (injection m = j - 1)
+          point_adjoint(d, P1b, ca, sa, ra, e1b, e2b, &phb, &ub);
+          u0b += ub;
+          dub = fma_t(kappa<T>(j - 1, n), ub, dub);
+          const StepX<T> xi = step_x(d, j - 1, n, c3x3, c2x2);
+#pragma unroll
+          for (int m = 0; m < 4; ++m) wx[m] = fma_t(phb, (T)2 * S1[m] - xi.x[m], wx[m]);
+          hb = fma_t(phb, (T)ps_a * ih, hb);
+        }
       }
+      // G_bar_i = -w_i^3 h (A_i + A_{i+1}) / 2 times the factors of dG
+      const T gh = (T)-0.5 * d.h;
+      u0b = fma_t(gh, wx[0], u0b);
+      dub = fma_t(gh, wx[1], dub);
+      Mb = (T)2 * gh * wx[2];
+      ib2b = gh * wx[3];
       // epilogue in FP64: h = du/2, du = (l - u0)/n, l(M, 1/b^2), set-up
       // (psi starts at 0 whatever the parameters, so A_1 goes nowhere)
       double dub64 = (double)dub + 0.5 * (double)hb;

@@ -142,25 +142,39 @@ def render(theta: torch.Tensor, dr: DiffRenderer) -> torch.Tensor:
 FIT_SCALE = (0.025, math.radians(1.0), 0.25, 0.1)
 
 
+def _pool(img: torch.Tensor, k: int) -> torch.Tensor:
+    """k x k box average of an (H, W, 3) image."""
+    if k == 1:
+        return img
+    return torch.nn.functional.avg_pool2d(img.permute(2, 0, 1).unsqueeze(0), k).squeeze(0)
+
+
 def fit(dr: DiffRenderer, target: torch.Tensor, theta0, iters: int = 200, lr: float = 0.2,
-        log=None):
-    """Adam on the mean squared error to target, learning rate annealed on a
-    cosine to 5% over the iterations.  theta = theta0 + scale * x, x from 0.
-    Returns (theta, history of (loss, theta))."""
+        pyramid=(16, 8, 4, 2, 1), log=None):
+    """Adam on the mean squared error to target, coarse to fine: the
+    iterations are split evenly over the box-filter sizes in `pyramid`, and
+    each stage compares k x k averages of the render and the target (the
+    noisy disc texture makes the full-resolution loss rough in r_s and the
+    distance, so the coarse stages find the basin and the last stage the
+    minimum).  The learning rate is annealed on a cosine to 5% over the
+    iterations; theta = theta0 + scale * x, x from 0.  Returns (theta,
+    history of (full-resolution mse, theta))."""
     scale = torch.tensor(FIT_SCALE, dtype=torch.float64)
     base = torch.tensor([float(v) for v in theta0], dtype=torch.float64)
     x = torch.zeros(4, dtype=torch.float64, requires_grad=True)
     opt = torch.optim.Adam([x], lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=iters, eta_min=0.05 * lr)
+    targets = {k: _pool(target, k) for k in pyramid}
     hist = []
-    n = float(target.numel())
     for it in range(iters):
+        k = pyramid[min(len(pyramid) - 1, it * len(pyramid) // iters)]
         opt.zero_grad()
         theta = base + scale * x
         img = render(theta, dr)
-        loss = ((img - target) ** 2).sum() / n
+        loss = ((_pool(img, k) - targets[k]) ** 2).mean()
         loss.backward()
-        hist.append((float(loss.item()), theta.detach().tolist()))
+        mse = float(((img.detach() - target) ** 2).mean())
+        hist.append((mse, theta.detach().tolist()))
         if log is not None and (it % 20 == 0 or it == iters - 1):
             log(it, hist[-1])
         opt.step()

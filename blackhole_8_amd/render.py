@@ -91,6 +91,12 @@ class Renderer:
         self._check(self.lib.bh_render_flat(self.ctx, C.byref(scene), C.byref(o)))
         return arrs
 
+    def time_flat(self, scene: abi.FlatScene, iters: int = 20) -> float:
+        """Average ms of the flat march kernel (bh_time_flat)."""
+        ms = C.c_double()
+        self._check(self.lib.bh_time_flat(self.ctx, C.byref(scene), iters, C.byref(ms)))
+        return ms.value
+
     def check_scene(self, scene: abi.Scene, opts: abi.Options) -> int:
         return self.lib.bh_scene_check(self.ctx, C.byref(scene), C.byref(opts))
 

@@ -336,6 +336,9 @@ int bh_diff_time(bh_ctx* ctx, const bh_diff_params* p, int precision, int iters,
 int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* scene, const bh_outputs* out);
 /* The same into device buffers on the context stream (asynchronous). */
 int bh_render_flat_device(bh_ctx* ctx, const bh_flat_scene* scene, const bh_outputs* out_device);
+/* Average ms of the flat march kernel over iters back-to-back launches (HIP
+ * events on the context stream; rgb and hit into buffers owned by ctx). */
+int bh_time_flat(bh_ctx* ctx, const bh_flat_scene* scene, int iters, double* avg_ms);
 
 /* ======================================================================== */
 /* Frame ring — the double-buffered screen hand-off of                       */

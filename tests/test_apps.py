@@ -7,8 +7,8 @@ CPU: the programs' scene sequences (--dry-run) against an independent Python
 replay of the reference's loops: the disc turned by pi/180 about its axis
 after each frame with its construction-time normal kept
 (blackhole_solution_test.cc:407, vector_object.h:320-326), the key bindings
-(:351-400; ray_tracer_test.cc:190-226) and the flat scene's movement test
-(ray_tracer_test.cc:229-261); frame 0 equals the library presets.
+(:351-406; ray_tracer_test.cc:196-235) and the flat scene's movement test
+(ray_tracer_test.cc:237-261); frame 0 equals the library presets.
 GPU: the files the programs write equal the library's own render of the same
 scenes byte for byte, and the frame ring returns what bh_render returns.
 """

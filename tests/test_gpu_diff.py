@@ -12,7 +12,7 @@ Bars:
             at theta +- h (the render is only piecewise smooth: texel cells,
             crossing counts and classes change discretely).  FP64 at 64x64
             (the gradcheck), FP32 at 64x64 and 256x256.
-  fit       200 Adam iterations at 512x512 FP32 from the perturbed start
+  fit       200 Adam iterations (coarse-to-fine loss) at 512x512 FP32 from the perturbed start
             reduce the loss twenty-fold and bring every parameter to within a
             tenth of its starting error.
 """
