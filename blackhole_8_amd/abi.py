@@ -131,6 +131,7 @@ SIGNATURES = {
                                  C.c_int, C.POINTER(P)]),
     "bh_path_launch": (C.c_int, [P, C.POINTER(C.c_float)]),
     "bh_path_read": (C.c_int, [P, C.c_int, P, P]),
+    "bh_path_time": (C.c_int, [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bh_path_destroy": (None, [P]),
     "bh_time_render": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options), C.c_int,
                                  C.POINTER(C.c_double), C.POINTER(C.c_double),

@@ -55,6 +55,9 @@ struct bh_path {
   uint8_t* d_hit = nullptr;
   uint32_t* d_qitems = nullptr;
   uint32_t* d_qcount = nullptr;
+  uint32_t qcap = 0;                 // queue slice per frame
+  bool fused = false;                // all frames FP32: one march + one band launch
+  bool esc = false, sph = false;     // scene class of the fused launches
   std::vector<hipEvent_t> events;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -224,6 +227,13 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
   d->r02 = r0 * r0;
   d->cell = (d->Uc - d->l0) * (1.0 / 1048576.0);
   d->inv_cell = 1048576.0 / (d->Uc - d->l0);
+  d->inv_nstep = 1.0 / d->nstep;
+  d->M2 = 2.0 * d->M;
+  d->M3 = 3.0 * d->M;
+  d->M6 = 6.0 * d->M;
+  d->Rhf = (float)d->Rh;
+  d->u0f = (float)d->u0;
+  d->ktf = (float)d->nstep - 0.1f;
   d->alpha_f = d->nf[0] * d->e1f[0] + d->nf[1] * d->e1f[1] + d->nf[2] * d->e1f[2];
   d->r_in2 = d->r_in > 0.0 ? (float)(d->r_in * d->r_in) : -1.0f;
   d->r_out2 = d->r_out >= 0.0 ? (float)(d->r_out * d->r_out) : -1.0f;
@@ -565,6 +575,34 @@ int build_diff(bh_ctx* ctx, const bh_diff_params* p, int precision, bh::DiffScen
   return BH_OK;
 }
 
+// Fused path launches: every frame's march in one launch (up to 1024 frames
+// each), then every frame's FP64 band pass in one launch.
+constexpr int kFramesPerLaunch = 1024;
+
+hipError_t path_march(const bh_path* p, hipStream_t st) {
+  const DevOut o{p->d_rgb, p->d_hit, nullptr, nullptr, nullptr};
+  const DevQueue q{p->d_qitems, p->d_qcount, p->qcap};
+  for (int f0 = 0; f0 < p->n_frames; f0 += kFramesPerLaunch) {
+    const int nf = std::min(kFramesPerLaunch, p->n_frames - f0);
+    const hipError_t e = bh::launch_march_frames(p->d_scenes, o, p->W, p->H, f0, nf, q, p->qcap,
+                                                 st, p->esc, p->sph);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t path_band(const bh_path* p, hipStream_t st) {
+  const DevOut o{p->d_rgb, p->d_hit, nullptr, nullptr, nullptr};
+  const DevQueue q{p->d_qitems, p->d_qcount, p->qcap};
+  for (int f0 = 0; f0 < p->n_frames; f0 += kFramesPerLaunch) {
+    const int nf = std::min(kFramesPerLaunch, p->n_frames - f0);
+    const hipError_t e = bh::launch_band_frames(p->d_scenes, o, p->W, p->H, f0, nf, q, p->qcap,
+                                                st, p->esc, p->sph);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -850,6 +888,11 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
   // one queue slice and count pair per frame, so that the FP64 band kernel of
   // frame i (side stream) overlaps the FP32 march of frame i+1
   const size_t qcap = npix / 16 > 1024 ? npix / 16 : 1024;
+  p->qcap = (uint32_t)qcap;
+  p->fused = true;
+  for (const DevScene& f : frames) p->fused = p->fused && fp32_eligible(f);
+  p->esc = frames[0].escape != 0;
+  p->sph = frames[0].n_spheres > 0;
   if ((e = hipMalloc(&p->d_qitems, qcap * n_frames * sizeof(uint32_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
   if ((e = hipMalloc(&p->d_qcount, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMalloc queue count");
   if ((e = hipMemset(p->d_qcount, 0, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMemset queue count");
@@ -862,7 +905,12 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
   if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return cleanup(e, "capture");
   int rc = BH_OK;
   bool used_side = false;
-  for (int i = 0; i < n_frames && rc == BH_OK; ++i) {
+  if (p->fused) {
+    hipError_t le = path_march(p, st);
+    if (le == hipSuccess) le = path_band(p, st);
+    if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
+  }
+  for (int i = 0; i < n_frames && rc == BH_OK && !p->fused; ++i) {
     DevOut o{p->d_rgb + npix * 3 * i, p->d_hit + npix * i, nullptr, nullptr, nullptr};
     DevQueue q{p->d_qitems + qcap * i, p->d_qcount + 16 * i, (uint32_t)qcap};
     const DevScene& f = frames[(size_t)i];
@@ -911,6 +959,32 @@ int bh_path_launch(bh_path* p, float* ms) {
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
   HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
   if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  return BH_OK;
+}
+
+int bh_path_time(bh_path* p, int iters, double* march_ms, double* band_ms) {
+  if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
+  bh_ctx* ctx = p->ctx;
+  if (!p->fused) return fail(ctx, BH_ERR_UNSUPPORTED, "path frames are not all in the FP32 march");
+  if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  double tm = 0, tb = 0;
+  for (int it = 0; it < iters; ++it) {
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    HIP_TRY(ctx, path_march(p, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    HIP_TRY(ctx, path_band(p, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+    float a = 0, b = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&b, ctx->ev1, ctx->ev2));
+    tm += a;
+    tb += b;
+  }
+  if (march_ms) *march_ms = tm / iters;
+  if (band_ms) *band_ms = tb / iters;
   return BH_OK;
 }
 

@@ -163,9 +163,10 @@ __device__ inline bool advance(RayRun& R) {
     const double un = __builtin_fma((double)kt - (double)j, R.o.du64, s.u0);
     if (un * s.r_esc <= 1.0) return end_ray(n - 1);
   }
-  float sv, ka, kb;
-  step_coeffs<float>(n, nstep, R.L, &sv, &ka, &kb);
-  if (march_step<float, StepScene>(s, R.L, R.st, R.r.first_aa, sv, ka, kb,
+  float G, ka, kb;
+  if (n == nstep) to_outbound<float>(R.L);
+  step_coeffs<float>(n, nstep, R.L, &G, &ka, &kb);
+  if (march_step<float, StepScene>(s, R.L, R.st, R.r.first_aa, G, ka, kb,
                                    n == 1 && R.r.first_aa != 0.0, R.o.hit))
     return end_ray(n);
   if (n == nstep && R.r.captured) {
@@ -179,7 +180,7 @@ __device__ inline bool advance(RayRun& R) {
 }
 
 __device__ inline void arm(const DevScene& s, RayRun& R) {
-  const double du = (R.r.l - s.u0) * (1.0 / s.nstep);
+  const double du = (R.r.l - s.u0) * s.inv_nstep;
   setup_lane<float>(s, R.r.l, R.r.ib2, du, R.r.e2x, R.r.e2y, R.r.e2z, R.r.sgn, R.L);
   R.st = MarchState<float>{0.f, 0.0, 0.f};
   R.o.hit.cls = BH_HIT_NONE;
@@ -315,7 +316,7 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
     degenerate_out(mo);
   } else {
     Lane<float> L;
-    const double du = (r.l - s.u0) * (1.0 / s.nstep);
+    const double du = (r.l - s.u0) * s.inv_nstep;
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
     march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
   }
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(64) void batch_band_kernel(BatchDev b) {
     setup_ray(s, x, y, true, r);
     MarchOut mo;
     Lane<double> L;
-    const double du = (r.l - s.u0) * (1.0 / s.nstep);
+    const double du = (r.l - s.u0) * s.inv_nstep;
     setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
     const bool sph = s.n_spheres > 0;
     if (s.escape) {

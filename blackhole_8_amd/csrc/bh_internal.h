@@ -53,6 +53,10 @@ struct DevScene {
   double Uc;              // 1/(3M)
   double r02;             // |C - B|^2
   double cell, inv_cell;  // SolveG grid: (1/(3M) - l0) / 2^20 and its inverse
+  double inv_nstep;       // 1 / nstep
+  double M2, M3, M6;      // 2M, 3M, 6M
+  float Rhf, u0f;         // R_h and u0 in FP32
+  float ktf;              // nstep - 0.1 in FP32
   float e1f[3];           // F / |F|
   float nf[3];            // disc normal
   float alpha_f;          // n . e1
@@ -90,6 +94,12 @@ hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, in
                              const DevQueue& q, hipStream_t stream, bool escape, bool spheres);
 hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,
                        hipStream_t stream, bool escape, bool spheres);
+hipError_t launch_march_frames(const DevScene* scenes, const DevOut& out, int W, int H,
+                               int frame0, int nf, const DevQueue& q, uint32_t qstride,
+                               hipStream_t stream, bool escape, bool spheres);
+hipError_t launch_band_frames(const DevScene* scenes, const DevOut& out, int W, int H,
+                              int frame0, int nf, const DevQueue& q, uint32_t qstride,
+                              hipStream_t stream, bool escape, bool spheres);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
 
 // ---------------------------------------------------------------------------

@@ -36,9 +36,15 @@ namespace bh {
 // tested exactly too.  Every other step provably hits nothing.
 template <typename T>
 struct Lane {
-  T c0, c1, c2, c3;   // G(s) cubic
+  // G at the steps as a cubic c0 + x (c1 + x (c2 + x c3)) in a uniform x,
+  // with s = l - u = sigma du:
+  //   inbound  x = sigma = nstep - n
+  //   approach G = c0 after to_outbound                  (sigma = 0.1)
+  //   outbound x = j = n - nstep                         (sigma = 0.1 + j)
+  // to_outbound re-expands the cubic about sigma = 0.1 once the inbound steps
+  // are done, so a step needs no per-lane s.
+  T c0, c1, c2, c3;
   T du, h;            // du and du/2
-  T du01;             // 0.1 du
   T u0;               // 1/|C - B|
   float phi_safe;     // horizon guard angle
   float guard;        // disc crossing-angle guard
@@ -195,41 +201,74 @@ struct MarchOut {
 //   inbound   n < nstep:  kappa = n,                 sigma = nstep - n
 //   approach  n = nstep:  kappa = nstep - 0.1,       sigma = 0.1
 //   outbound  n > nstep:  kappa = 2 nstep - 0.1 - n, sigma = 0.1 + (n - nstep)
-// The coeffs_* helpers give sv = sigma_n du and the kappa of the chord's two
-// ends for each phase; step_coeffs picks the phase from n.
+// The coeffs_* helpers give G at step n and the kappa of the chord's two
+// ends for each phase (the approach and outbound ones once to_outbound has
+// re-expanded the cubic); step_coeffs picks the phase from n.
+// x = (T)k for the wave-uniform step index 1 <= k < 2^24.  In FP32 the float
+// is assembled from the integer with integer operations only (exponent from
+// the leading bit, mantissa by shifting), so it stays in scalar registers and
+// costs no vector instruction.
+template <typename T> __device__ __forceinline__ T step_x(int k);
+template <> __device__ __forceinline__ float step_x<float>(int k) {
+#ifdef STEPX_SALU
+  const uint32_t u = (uint32_t)k;
+  const uint32_t e = 31u - (uint32_t)__builtin_clz(u);
+  const uint32_t m = (u << (23u - e)) & 0x7FFFFFu;
+  return __builtin_bit_cast(float, ((e + 127u) << 23) | m);
+#else
+  return (float)k;
+#endif
+}
+template <> __device__ __forceinline__ double step_x<double>(int k) { return (double)k; }
+
 template <typename T>
-__device__ __forceinline__ void coeffs_in(int n, int nstep, const Lane<T>& L, T* sv, T* ka,
+__device__ __forceinline__ T step_g(const Lane<T>& L, T x) {
+  return fma_t<T>(fma_t<T>(fma_t<T>(L.c3, x, L.c2), x, L.c1), x, L.c0);
+}
+// The step cubic in sigma re-expanded about sigma = 0.1 in powers of
+// j = sigma - 0.1 (the approach and outbound steps).
+template <typename T> __device__ __forceinline__ void to_outbound(Lane<T>& L) {
+  const T a = (T)0.1, a3 = (T)0.3;
+  const T c0 = fma_t<T>(a, fma_t<T>(a, fma_t<T>(a, L.c3, L.c2), L.c1), L.c0);
+  const T c1 = fma_t<T>(a, fma_t<T>(a3, L.c3, (T)2 * L.c2), L.c1);
+  L.c2 = fma_t<T>(a3, L.c3, L.c2);
+  L.c1 = c1;
+  L.c0 = c0;
+}
+template <typename T>
+__device__ __forceinline__ void coeffs_in(int n, int nstep, const Lane<T>& L, T* G, T* ka,
                                           T* kb) {
+  *G = step_g<T>(L, step_x<T>(nstep - n));
   const T kn = (T)n;
-  *sv = ((T)nstep - kn) * L.du;
   *ka = kn - (T)1;
   *kb = kn;
 }
+// (after to_outbound)
 template <typename T>
-__device__ __forceinline__ void coeffs_final(int nstep, const Lane<T>& L, T* sv, T* ka, T* kb) {
+__device__ __forceinline__ void coeffs_final(int nstep, const Lane<T>& L, T* G, T* ka, T* kb) {
   const T kt = (T)nstep - (T)0.1;
-  *sv = L.du01;
+  *G = L.c0;
   *ka = kt - (T)0.9;
   *kb = kt;
 }
 template <typename T>
-__device__ __forceinline__ void coeffs_out(int n, int nstep, const Lane<T>& L, T* sv, T* ka,
+__device__ __forceinline__ void coeffs_out(int n, int nstep, const Lane<T>& L, T* G, T* ka,
                                            T* kb) {
   const T kt = (T)nstep - (T)0.1;
   const T j = (T)(n - nstep);
-  *sv = fma_t<T>(j, L.du, L.du01);
+  *G = step_g<T>(L, step_x<T>(n - nstep));
   *ka = kt - j + (T)1;
   *kb = kt - j;
 }
 template <typename T>
-__device__ __forceinline__ void step_coeffs(int n, int nstep, const Lane<T>& L, T* sv, T* ka,
+__device__ __forceinline__ void step_coeffs(int n, int nstep, const Lane<T>& L, T* G, T* ka,
                                             T* kb) {
   if (n < nstep)
-    coeffs_in<T>(n, nstep, L, sv, ka, kb);
+    coeffs_in<T>(n, nstep, L, G, ka, kb);
   else if (n == nstep)
-    coeffs_final<T>(nstep, L, sv, ka, kb);
+    coeffs_final<T>(nstep, L, G, ka, kb);
   else
-    coeffs_out<T>(n, nstep, L, sv, ka, kb);
+    coeffs_out<T>(n, nstep, L, G, ka, kb);
 }
 
 // Running state of one ray's march.
@@ -239,16 +278,16 @@ template <typename T> struct MarchState {
   T d;       // last increment
 };
 
-// One step: trapezoid update of the swept angle, then the exact chord tests
+// One step (G = G(u) at the step): trapezoid update of the swept angle, then
+// the exact chord tests
 // if the step is an event (a crossing angle passed, the horizon guard
 // exceeded, or force).  Returns true when the chord hits (hit is filled).
 // With SPH the chord is also tested against the spheres of L.smask, at every
 // step.
 template <typename T, typename S, bool SPH = false>
 __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>& st,
-                                           double first_aa, T sv, T ka, T kb, bool force,
+                                           double first_aa, T G, T ka, T kb, bool force,
                                            HitRec& hit) {
-  const T G = fma_t<T>(fma_t<T>(fma_t<T>(L.c3, sv, L.c2), sv, L.c1), sv, L.c0);
   const T wn = rsq<T>(G);
   st.d = (st.w + wn) * L.h;
   st.w = wn;
@@ -303,18 +342,18 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     L.smask = 0u;  // no sphere tests either once ended
   };
   auto armed = [&]() { return L.phi_safe < kInf; };
-  auto step = [&](int n, T sv, T ka, T kb, bool force) {
-    if (march_step<T, DevScene, SPH>(s, L, st, first_aa, sv, ka, kb, force, o.hit)) end_ray(n);
+  auto step = [&](int n, T G, T ka, T kb, bool force) {
+    if (march_step<T, DevScene, SPH>(s, L, st, first_aa, G, ka, kb, force, o.hit)) end_ray(n);
   };
   auto step_in = [&](int n, bool force) {
-    T sv, ka, kb;
-    coeffs_in<T>(n, nstep, L, &sv, &ka, &kb);
-    step(n, sv, ka, kb, force);
+    T G, ka, kb;
+    coeffs_in<T>(n, nstep, L, &G, &ka, &kb);
+    step(n, G, ka, kb, force);
   };
   auto step_out = [&](int n) {
-    T sv, ka, kb;
-    coeffs_out<T>(n, nstep, L, &sv, &ka, &kb);
-    step(n, sv, ka, kb, false);
+    T G, ka, kb;
+    coeffs_out<T>(n, nstep, L, &G, &ka, &kb);
+    step(n, G, ka, kb, false);
   };
   // inbound (blackhole_solution_test.cc:217-239)
   step_in(1, first_aa != 0.0);
@@ -328,16 +367,17 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
   {
     // final approach: u += 0.9 du, weighted du/2 (:240-262)
     {
-      T sv, ka, kb;
-      coeffs_final<T>(nstep, L, &sv, &ka, &kb);
-      step(nstep, sv, ka, kb, false);
+      T G, ka, kb;
+      to_outbound<T>(L);
+      coeffs_final<T>(nstep, L, &G, &ka, &kb);
+      step(nstep, G, ka, kb, false);
     }
     // captured: chord to the centre; the horizon is always entered
     // (r > 3M > 2M) and the disc test sees n.(B - B) = 0 (:264-272)
     if (captured && armed()) {
       o.hit.cls = BH_HIT_HORIZON;
       o.hit.ang = st.a;
-      o.hit.hx = (float)s.Rh;
+      o.hit.hx = s.Rhf;
       o.hit.hy = 0.f;
       end_ray(nstep);
     }
@@ -394,18 +434,22 @@ __device__ inline uint32_t equirect32(const DevTex& t, float dx, float dy, float
 template <typename T>
 __device__ inline void setup_lane(const DevScene& s, double l, double ib2, double du, float e2x,
                                   float e2y, float e2z, float sgn, Lane<T>& L) {
-  const double M = s.M;
-  const double dl = 1.0 - 3.0 * M * l;
-  L.c0 = (T)__builtin_fma(l * l, __builtin_fma(2.0 * M, l, -1.0), ib2);
-  L.c1 = (T)(2.0 * l * dl);
-  L.c2 = (T)(1.0 - 2.0 * dl);
-  L.c3 = (T)(-2.0 * M);
+  // G(l - s) = c0 + s (c1 + s (c2 + s c3)), no cancellation near the
+  // photon sphere; the step cubics of Lane in units of du
+  const double dl = __builtin_fma(-s.M3, l, 1.0);
+  const double c0 = __builtin_fma(l * l, __builtin_fma(s.M2, l, -1.0), ib2);
+  const double c1 = 2.0 * l * dl, c2 = __builtin_fma(-2.0, dl, 1.0), c3 = -s.M2;
+  const double du2 = du * du;
+  L.c0 = (T)c0;
+  L.c1 = (T)(c1 * du);
+  L.c2 = (T)(c2 * du2);
+  L.c3 = (T)(c3 * du2 * du);
   L.du = (T)du;
   L.h = (T)(0.5 * du);
-  L.du01 = (T)(0.1 * du);
   L.u0 = (T)s.u0;
-  // smallest radius on the march: u = u0 + (nstep - 0.1) du
-  const float x = (float)(s.Rh * __builtin_fma((double)s.nstep - 0.1, du, s.u0));
+  // smallest radius on the march: u = u0 + (nstep - 0.1) du (FP32: the guard
+  // keeps a 1e-4 relative margin)
+  const float x = s.Rhf * __builtin_fmaf(s.ktf, (float)du, s.u0f);
   L.phi_safe = x < 1.f ? 3.14159265f * (1.f - x) * 0.9999f : 0.f;
   L.chi0 = 0.0;
   L.a_ev = 1e300;
@@ -416,9 +460,13 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, doubl
     const float amp2 = al * al + be * be;
     if (amp2 > 0.f) {
       // al cos a + be sin a vanishes at a = atan2(be, al) + pi/2 (mod pi)
-      L.chi0 = (double)atan2_fast(be, al) + kPiD / 2.0;
+      const float at = atan2_fast(be, al);
+      const float chi = at + 1.57079633f;
+      L.chi0 = (double)at + kPiD / 2.0;
       L.guard = 4e-6f + 4e-7f * __builtin_amdgcn_rsqf(amp2);
-      L.a_ev = next_event(0.0, L.chi0, L.guard);
+      // first crossing threshold at or after 0 (FP32: far inside the guard)
+      L.a_ev = (double)(chi + (__builtin_floorf((-L.guard - chi) * 0.318309886f) + 1.f) *
+                                  3.14159265f - L.guard);
     }
   }
   if (L.a_ev == 1e300) L.chi0 = 1e300;  // no crossings: parity tests stay even
@@ -473,9 +521,14 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
   r.ib2 = 0.0;
   r.l = s.Uc;
   if (r.degenerate) return;
-  r.ib2 = dd / cc;  // 1 / b^2
-  // e2 = sign(Y0) ((dv.F) F - r0^2 dv) / (r0 |cv|)
-  const double k = (double)r.sgn * s.u0 * rsq<double>(cc);
+  // 1 / b^2 = dd / cc: hardware reciprocal and two Newton steps (<= 2 ulp)
+  double rc = __builtin_amdgcn_rcp(cc);
+  rc = rc * __builtin_fma(-cc, rc, 2.0);
+  rc = rc * __builtin_fma(-cc, rc, 2.0);
+  r.ib2 = dd * rc;
+  // e2 = sign(Y0) ((dv.F) F - r0^2 dv) / (r0 |cv|), kept in FP32: the
+  // FP32 1/|cv| scales it by 1 + O(1e-7)
+  const double k = (double)(r.sgn * __builtin_amdgcn_rsqf((float)cc)) * s.u0;
   r.e2x = (float)(__builtin_fma(dF, Fx, -s.r02 * dvx) * k);
   r.e2y = (float)(__builtin_fma(dF, Fy, -s.r02 * dvy) * k);
   r.e2z = (float)(__builtin_fma(dF, Fz, -s.r02 * dvz) * k);
@@ -491,13 +544,13 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
     float sn, cs;
     sincos_core(asinf(2.f * q - 1.f) * (1.f / 3.f), &sn, &cs);
     double u = s.Uc * (0.5 + (double)sn);
-    const double M2 = 2.0 * s.M, M6 = 6.0 * s.M;
+    const double M2 = s.M2, M6 = s.M6;
+    // (the FP32 1/G' leaves the quadratic convergence a 1e-7 linear term)
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
       const double g = __builtin_fma(u * u, __builtin_fma(M2, u, -1.0), r.ib2);
       const double gp = u * __builtin_fma(M6, u, -2.0);
-      double ig = __builtin_amdgcn_rcp(gp);
-      ig = ig * __builtin_fma(-gp, ig, 2.0);
+      const double ig = (double)__builtin_amdgcn_rcpf((float)gp);
       u = __builtin_fma(-g, ig, u);
     }
     const double t = (u - s.l0) * s.inv_cell;

@@ -178,6 +178,13 @@ class Path:
         self.r._check(self.r.lib.bh_path_launch(self.handle, C.byref(ms)))
         return ms.value
 
+    def time(self, iters: int = 3):
+        """(march_ms, band_ms): the fused launches of a path whose frames
+        all take the FP32 march (bh_path_time)."""
+        m, b = C.c_double(), C.c_double()
+        self.r._check(self.r.lib.bh_path_time(self.handle, iters, C.byref(m), C.byref(b)))
+        return m.value, b.value
+
     def read(self, frame: int):
         rgb = np.zeros((self.h, self.w, 3), np.uint8)
         hit = np.zeros((self.h, self.w), np.uint8)

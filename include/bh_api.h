@@ -237,8 +237,13 @@ typedef struct bh_path bh_path;
  * the path.  All cameras must share width/height. */
 int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                    const bh_camera* cameras, int n_frames, bh_path** out);
-/* Launch the captured graph once; ms (optional) = GPU time from HIP events. */
+/* Launch the captured graph once; ms (optional) = GPU time from HIP events.
+ * When every frame takes the FP32 march, the graph is one march launch over
+ * all frames followed by one FP64 band launch over all frames' queues. */
 int bh_path_launch(bh_path* path, float* ms);
+/* Average ms of those two launches over iters eager runs (HIP events on the
+ * context stream); BH_ERR_UNSUPPORTED for paths with FP64 frames. */
+int bh_path_time(bh_path* path, int iters, double* march_ms, double* band_ms);
 /* Copy frame i back to host buffers (rgb H*W*3, hit H*W; either may be NULL). */
 int bh_path_read(bh_path* path, int frame, uint8_t* rgb, uint8_t* hit);
 void bh_path_destroy(bh_path* path);
