@@ -115,9 +115,13 @@ __device__ __forceinline__ void band_rays(const DevScene& s, const DevOut& out, 
     march<double, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
     finish_ray<ESCAPE>(s, out, idx, r, mo);
   }
+  // Every lane's read of count[0] has returned before the barrier, so the
+  // reset cannot race with it; the queue is read again only by later
+  // launches.  (No device-scope fence here: on this multi-XCD part it writes
+  // back and invalidates the XCD's L2, full of framebuffer lines, in every
+  // workgroup.)
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
     const uint32_t done = atomicAdd(q.count + 1, 1u);
     if (done == nblk - 1) {
       atomicExch(q.count, 0u);
