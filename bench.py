@@ -245,7 +245,7 @@ def main() -> int:
         "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
         "frac_nonpacked": round(achieved / FP32_NONPACKED_TFLOPS, 4),
         "traffic": traffic,
-        "traffic_per": "frame",
+        "traffic_per": "launch (all frames of the path)",
         "launch_ms": round(march_ms, 4),
         "avg_kernel_ms": round(march_ms / F, 5),
         "band_launch_ms": round(band_ms, 4),
