@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from blackhole_8_amd import abi, scenes
+from blackhole_8_amd.render import BHError
 import oracle
 from tests.parity import compare, disc_texel_coords, oracle_disc_texel, straddles, summary
 
@@ -142,19 +143,34 @@ def test_fp64_config2_matches_oracle(renderer):
     assert st["phi_max_err"] <= 1e-9
 
 
-def test_path_graph_matches_eager(renderer):
+@pytest.mark.parametrize("precision", [abi.PREC_FP32, abi.PREC_FP64])
+def test_path_graph_matches_eager(renderer, precision):
+    # FP32: one march launch over all frames and one band launch (the frames'
+    # band queues must be drained and cleared between replays); FP64: a launch
+    # per frame.
     p = scenes.config1(320, 180, frames=6)
+    o = scenes.options(precision=precision, band=1e-3)
     renderer.set_textures(p.textures)
-    print("[case] path graph 6 frames 320x180", flush=True)
-    path = renderer.path(p.scene, p.options, p.cameras)
+    print(f"[case] path graph 6 frames 320x180 precision {precision}", flush=True)
+    path = renderer.path(p.scene, o, p.cameras)
     try:
-        path.launch()
-        path.launch()  # replay twice; every replay must equal the eager render
-        for k in range(len(p.cameras)):
-            rgb, hit = path.read(k)
-            e = renderer.render(p.with_camera(p.cameras[k]).scene, p.options, ("rgb", "hit"))
-            assert np.array_equal(rgb, e["rgb"])
-            assert np.array_equal(hit, e["hit"])
+        for replay in range(3):  # every replay must equal the eager render
+            path.launch()
+            for k in range(len(p.cameras)):
+                rgb, hit = path.read(k)
+                e = renderer.render(p.with_camera(p.cameras[k]).scene, o, ("rgb", "hit"))
+                assert np.array_equal(rgb, e["rgb"]), (replay, k)
+                assert np.array_equal(hit, e["hit"]), (replay, k)
+        if precision == abi.PREC_FP32:
+            march_ms, band_ms = path.time(iters=2)
+            assert march_ms > 0 and band_ms > 0
+            rgb, hit = path.read(3)  # timing launches leave the same frames behind
+            e = renderer.render(p.with_camera(p.cameras[3]).scene, o, ("rgb", "hit"))
+            assert np.array_equal(rgb, e["rgb"]) and np.array_equal(hit, e["hit"])
+        else:
+            with pytest.raises(BHError) as err:
+                path.time(iters=1)
+            assert err.value.code == abi.BH_ERR_UNSUPPORTED
     finally:
         path.close()
 
