@@ -162,6 +162,9 @@ SIGNATURES = {
                                 C.POINTER(C.c_int)]),
     "bh_image_write": (C.c_int, [C.c_char_p, P, C.c_int, C.c_int]),
     "bh_io_last_error": (C.c_char_p, []),
+    "bh_jpeg_decode": (C.c_int, [P, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int)]),
+    "bh_jpeg_last_error": (C.c_char_p, []),
     "bh_focal_length": (C.c_double, [C.c_int, C.c_double]),
     "bh_object_rotate": (None, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                 C.c_int, C.c_double]),

@@ -31,7 +31,7 @@ const char kUsage[] =
     "  --width/--height    frame size (default 960x540, the reference's)\n"
     "  --fp32              FP32 march with the FP64 photon-sphere band\n"
     "                      (default: FP64, the reference's operation order)\n"
-    "  --disc-texture FILE PNG or binary PPM disc texture, e.g. the reference's\n"
+    "  --disc-texture FILE PNG, JPEG or binary PPM disc texture, e.g. the reference's\n"
     "                      resource/acc_disc.png (default: a seeded 598x578\n"
     "                      banded texture)\n"
     "  --keys STRING       key script, one key after each frame\n"

@@ -29,8 +29,10 @@ const char kUsage[] =
     "  --frames N        frames to render (default 3)\n"
     "  --out DIR         write DIR/frame_NNNNN.<format> (default: no files)\n"
     "  --width/--height  frame size (default 800x450, the reference's)\n"
-    "  --textures A,B,C  PNG or binary PPM images for the three rectangles\n"
-    "                    (default: seeded noise of 480x360, 300x400, 320x320)\n"
+    "  --textures A,B,C  PNG, JPEG or binary PPM images for the three rectangles,\n"
+    "                    e.g. the reference's resource/mooni.jpeg,karina.jpeg,\n"
+    "                    winter.jpg (default: seeded noise of 480x360, 300x400,\n"
+    "                    320x320)\n"
     "  --keys STRING     key script, one key after each frame\n"
     "  --depth D         frame ring slots, 2..8 (default 2: double buffering)\n"
     "  --dry-run         print each frame's scene as a JSON line; no GPU\n";
@@ -62,8 +64,9 @@ int main(int argc, char** argv) {
       if (!a.flag("--quiet")) std::printf("device: %s\n", ctx->device_name().c_str());
     }
 
-    // Textures: the reference's photos (mooni.jpeg, karina.jpeg, winter.jpg)
-    // or, by default, seeded noise of comparable sizes.
+    // Textures: the reference's photos (mooni.jpeg, karina.jpeg, winter.jpg,
+    // decoded as imread decodes them) or, by default, seeded noise of
+    // comparable sizes.
     const int sizes[3][2] = {{360, 480}, {400, 300}, {320, 320}};
     std::vector<std::string> paths;
     {

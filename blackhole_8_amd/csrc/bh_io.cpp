@@ -363,6 +363,10 @@ void png_chunk(std::vector<uint8_t>& out, const char* type, const uint8_t* data,
 
 extern "C" {
 
+int bh_jpeg_decode(const uint8_t* data, size_t size, uint8_t* dst, size_t dst_size, int* rows,
+                   int* cols);
+const char* bh_jpeg_last_error(void);
+
 const char* bh_io_last_error(void) { return t_err.c_str(); }
 
 int bh_png_decode(const uint8_t* data, size_t size, uint8_t* dst, size_t dst_size, int* rows,
@@ -387,7 +391,11 @@ int bh_image_load(const char* path, uint8_t* dst, size_t dst_size, int* rows, in
   if (f.size() >= 8 && std::memcmp(f.data(), kSig, 8) == 0)
     return bh_png_decode(f.data(), f.size(), dst, dst_size, rows, cols);
   if (f.size() >= 2 && f[0] == 'P' && f[1] == '6') return decode_ppm(f, dst, dst_size, rows, cols);
-  return io_fail(BH_ERR_UNSUPPORTED, "only PNG and binary PPM files are read");
+  if (f.size() >= 3 && f[0] == 0xFF && f[1] == 0xD8 && f[2] == 0xFF) {
+    const int rc = bh_jpeg_decode(f.data(), f.size(), dst, dst_size, rows, cols);
+    return rc ? io_fail(rc, bh_jpeg_last_error()) : BH_OK;
+  }
+  return io_fail(BH_ERR_UNSUPPORTED, "only PNG, JPEG and binary PPM files are read");
 }
 
 int bh_image_write(const char* path, const uint8_t* bgr, int rows, int cols) {

@@ -1,6 +1,6 @@
 """Image files through the library's host-side codec (no GPU needed).
 
-`load` / `decode_png` stand in for the reference's texture input,
+`load` / `decode_png` / `decode_jpeg` stand in for the reference's texture input,
 cv::imread(path, IMREAD_COLOR) (config.h:35-37): the result is rows x cols x 3
 bytes in B, G, R order with alpha dropped, the layout Material::SetTexture
 takes (material.h:29-35).  `write` saves a frame (B, G, R bytes, as the
@@ -39,6 +39,22 @@ def decode_png(data: bytes) -> np.ndarray:
                            C.byref(rows), C.byref(cols))
     if rc:
         _err(rc)
+    return out
+
+
+def decode_jpeg(data: bytes) -> np.ndarray:
+    lib = abi.load()
+    buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+    rows, cols = C.c_int(), C.c_int()
+    rc = lib.bh_jpeg_decode(C.cast(buf, C.c_void_p), len(data), None, 0, C.byref(rows),
+                            C.byref(cols))
+    if rc:
+        raise ImageError(rc, lib.bh_jpeg_last_error().decode())
+    out = np.empty((rows.value, cols.value, 3), np.uint8)
+    rc = lib.bh_jpeg_decode(C.cast(buf, C.c_void_p), len(data), abi.ptr(out), out.size,
+                            C.byref(rows), C.byref(cols))
+    if rc:
+        raise ImageError(rc, lib.bh_jpeg_last_error().decode())
     return out
 
 

@@ -389,7 +389,17 @@ void bh_ring_destroy(bh_ring* ring);
  * BH_ERR_UNSUPPORTED (unknown critical chunk, more than 2^28 pixels). */
 int bh_png_decode(const uint8_t* data, size_t size, uint8_t* dst, size_t dst_size, int* rows,
                   int* cols);
-/* The same from a file holding a PNG or a binary PPM (P6, maxval 255). */
+/* Decode a baseline (or extended sequential, Huffman, 8-bit) JPEG held in
+ * memory to rows x cols x 3 bytes in B, G, R order as libjpeg's default
+ * decompression gives it to imread (integer IDCT, triangle upsampling of
+ * 2:1 chroma, JFIF YCbCr -> RGB); grey replicated.  dst NULL: only *rows /
+ * *cols.  BH_ERR_UNSUPPORTED for progressive, arithmetic, 12-bit and
+ * 4-component files; message via bh_jpeg_last_error. */
+int bh_jpeg_decode(const uint8_t* data, size_t size, uint8_t* dst, size_t dst_size, int* rows,
+                   int* cols);
+const char* bh_jpeg_last_error(void);
+/* The same from a file holding a PNG, a JPEG or a binary PPM (P6, maxval
+ * 255), told apart by their signatures. */
 int bh_image_load(const char* path, uint8_t* dst, size_t dst_size, int* rows, int* cols);
 /* Write rows x cols x 3 bytes in B, G, R order (the framebuffer's order for
  * textures loaded as above) to path: ".ppm" binary PPM or ".png" 8-bit RGB

@@ -1,5 +1,12 @@
-"""Image files: the library's PNG decoder (bh_png_decode) with cv::imread
-IMREAD_COLOR semantics (config.h:35-37), and the PPM / PNG frame writer.
+"""Image files: the library's PNG and JPEG decoders (bh_png_decode,
+bh_jpeg_decode) with cv::imread IMREAD_COLOR semantics (config.h:35-37), and
+the PPM / PNG frame writer.
+
+The JPEG decoder is pinned byte for byte against Pillow, whose JPEG decoder is
+libjpeg(-turbo), the library imread uses: on the reference's own
+resource/*.jpeg textures when the reference tree is present, and on files
+Pillow writes over 4:4:4 / 4:2:2 / 4:2:0, greyscale, two qualities, odd sizes,
+restart intervals and optimized Huffman tables; progressive files are refused.
 
 The decoder is pinned by
   - Pillow (an independent PNG decoder, where installed) on 8-bit files it
@@ -239,3 +246,99 @@ def test_write_rejects_unknown_extension(tmp_path):
     with pytest.raises(images.ImageError) as e:
         images.write(tmp_path / "x.jpg", np.zeros((2, 2, 3), np.uint8))
     assert e.value.code == abi.BH_ERR_UNSUPPORTED
+
+
+# ---------------------------------------------------------------------------
+# JPEG (baseline): against Pillow, whose decoder is libjpeg(-turbo), the
+# library imread decodes JPEG with; the decoder is meant to be byte-exact
+# ---------------------------------------------------------------------------
+REF_JPEGS = [Path("/root/reference/resource") / f for f in ("mooni.jpeg", "karina.jpeg",
+                                                              "winter.jpg")]
+
+
+def _pillow_bgr(data: bytes):
+    Image = pytest.importorskip("PIL.Image")
+    return np.asarray(Image.open(io.BytesIO(data)).convert("RGB"))[..., ::-1]
+
+
+def _smooth_image(h, w, seed, channels=3):
+    # gradients plus noise: smooth areas exercise the DC path, noise the AC
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    base = np.stack([(x * 255 // max(1, w - 1)), (y * 255 // max(1, h - 1)),
+                     ((x + y) * 127 // max(1, w + h - 2))], -1)[..., :channels]
+    return np.clip(base + rng.integers(-40, 41, base.shape), 0, 255).astype(np.uint8)
+
+
+@pytest.mark.skipif(not all(p.exists() for p in REF_JPEGS), reason="reference resources absent")
+@pytest.mark.parametrize("path", REF_JPEGS, ids=lambda p: p.name)
+def test_reference_jpegs_match_libjpeg(path):
+    data = path.read_bytes()
+    got = images.decode_jpeg(data)
+    np.testing.assert_array_equal(got, _pillow_bgr(data))
+    np.testing.assert_array_equal(images.load(path), got)
+
+
+@pytest.mark.parametrize("subsampling", [0, 1, 2])  # 4:4:4, 4:2:2, 4:2:0
+@pytest.mark.parametrize("size", [(1, 1), (7, 13), (16, 16), (37, 53), (120, 97)])
+@pytest.mark.parametrize("quality", [35, 95])
+def test_jpeg_matches_libjpeg(subsampling, size, quality):
+    Image = pytest.importorskip("PIL.Image")
+    h, w = size
+    arr = _smooth_image(h, w, h * 1000 + w + quality)
+    buf = io.BytesIO()
+    Image.fromarray(arr, "RGB").save(buf, "JPEG", quality=quality, subsampling=subsampling)
+    data = buf.getvalue()
+    np.testing.assert_array_equal(images.decode_jpeg(data), _pillow_bgr(data))
+
+
+def test_jpeg_grey_restart_and_optimized_tables():
+    Image = pytest.importorskip("PIL.Image")
+    arr = _smooth_image(45, 61, 9)
+    cases = [(Image.fromarray(arr[..., 0], "L"), {}),
+             (Image.fromarray(arr, "RGB"), {"optimize": True}),
+             (Image.fromarray(arr, "RGB"), {"restart_marker_blocks": 3, "subsampling": 2}),
+             (Image.fromarray(arr, "RGB"), {"restart_marker_blocks": 1, "subsampling": 0})]
+    for im, kw in cases:
+        buf = io.BytesIO()
+        im.save(buf, "JPEG", quality=80, **kw)
+        data = buf.getvalue()
+        if kw.get("restart_marker_blocks"):
+            assert b"\xff\xd0" in data
+        got = images.decode_jpeg(data)
+        np.testing.assert_array_equal(got, _pillow_bgr(data), err_msg=str(kw))
+        if im.mode == "L":
+            assert (got[..., 0] == got[..., 1]).all() and (got[..., 1] == got[..., 2]).all()
+
+
+def test_jpeg_refuses_progressive_and_corrupt(tmp_path):
+    Image = pytest.importorskip("PIL.Image")
+    arr = _smooth_image(32, 32, 3)
+    buf = io.BytesIO()
+    Image.fromarray(arr, "RGB").save(buf, "JPEG", progressive=True)
+    with pytest.raises(images.ImageError) as e:
+        images.decode_jpeg(buf.getvalue())
+    assert e.value.code == abi.BH_ERR_UNSUPPORTED
+    buf = io.BytesIO()
+    Image.fromarray(arr, "RGB").save(buf, "JPEG")
+    good = buf.getvalue()
+    for bad in (b"", good[:2], b"\xff\xd8\x00" + good[3:], good[:200]):
+        with pytest.raises(images.ImageError):
+            images.decode_jpeg(bad)
+    # files are told apart by signature
+    p = tmp_path / "t.bin"
+    p.write_bytes(good)
+    np.testing.assert_array_equal(images.load(p), images.decode_jpeg(good))
+
+
+def test_jpeg_solid_colour_known_answer():
+    # a flat 8x8-aligned image has only DC terms: every pixel decodes to the
+    # colour the encoder quantised it to, the same for all pixels
+    Image = pytest.importorskip("PIL.Image")
+    arr = np.zeros((32, 48, 3), np.uint8)
+    arr[...] = (200, 40, 90)
+    buf = io.BytesIO()
+    Image.fromarray(arr, "RGB").save(buf, "JPEG", quality=100, subsampling=0)
+    got = images.decode_jpeg(buf.getvalue())
+    assert (got == got[0, 0]).all()
+    np.testing.assert_allclose(got[0, 0].astype(int), [90, 40, 200], atol=3)
