@@ -165,6 +165,13 @@ SIGNATURES = {
     "bh_jpeg_decode": (C.c_int, [P, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_int),
                                  C.POINTER(C.c_int)]),
     "bh_jpeg_last_error": (C.c_char_p, []),
+    "bh_jpeg_encode": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.POINTER(C.c_uint8)),
+                                 C.POINTER(C.c_size_t)]),
+    "bh_free": (None, [P]),
+    "bh_video_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                C.POINTER(P)]),
+    "bh_video_write": (C.c_int, [P, P]),
+    "bh_video_close": (C.c_int, [P]),
     "bh_focal_length": (C.c_double, [C.c_int, C.c_double]),
     "bh_object_rotate": (None, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                 C.c_int, C.c_double]),

@@ -66,14 +66,26 @@ inline bool camera_key(bh::Camera& cam, char key, double move_d, double rotate_d
   }
 }
 
+// Frame file formats of --format: one image file per frame, or "avi": every
+// frame into DIR/video.avi, Motion-JPEG at 29 frames/s as the reference
+// records (blackhole_solution_test.cc:71-72).
+inline bool valid_format(const std::string& f) {
+  return f == "ppm" || f == "png" || f == "jpg" || f == "avi";
+}
+constexpr double kVideoFps = 29;
+
 // Drains ring slots on its own thread: waits for each frame's copy, writes
 // it (when a directory is given), prints its timing and hands the slot back.
 // Stands in for the reference's clean thread and cv::VideoWriter::write.
 class Writer {
  public:
   Writer(bh_ring* ring, int w, int h, std::string dir, std::string ext, bool quiet)
-      : ring_(ring), w_(w), h_(h), dir_(std::move(dir)), ext_(std::move(ext)), quiet_(quiet),
-        th_([this] { run(); }) {}
+      : ring_(ring), w_(w), h_(h), dir_(std::move(dir)), ext_(std::move(ext)), quiet_(quiet) {
+    if (!dir_.empty() && ext_ == "avi" &&
+        bh_video_open((dir_ + "/video.avi").c_str(), h_, w_, kVideoFps, 95, &video_) != BH_OK)
+      throw std::runtime_error(bh_io_last_error());
+    th_ = std::thread([this] { run(); });
+  }
   ~Writer() { finish(); }
 
   void push(int slot, int frame) {
@@ -91,6 +103,10 @@ class Writer {
     }
     cv_.notify_one();
     if (th_.joinable()) th_.join();
+    if (video_) {
+      if (bh_video_close(video_) != BH_OK && err_.empty()) err_ = bh_io_last_error();
+      video_ = nullptr;
+    }
     return err_;
   }
   double gpu_ms_total() const { return gpu_ms_; }
@@ -119,7 +135,9 @@ class Writer {
         (void)bh_ring_release(ring_, it.slot);
         continue;
       }
-      if (!dir_.empty()) {
+      if (video_) {  // frames arrive in the order they were queued
+        if (bh_video_write(video_, rgb) != BH_OK && err_.empty()) err_ = bh_io_last_error();
+      } else if (!dir_.empty()) {
         char name[64];
         std::snprintf(name, sizeof name, "/frame_%05d.%s", it.frame, ext_.c_str());
         if (bh_image_write((dir_ + name).c_str(), rgb, h_, w_) != BH_OK && err_.empty())
@@ -145,6 +163,7 @@ class Writer {
   std::string err_;
   double gpu_ms_ = 0;
   int frames_ = 0;
+  bh_video* video_ = nullptr;
   std::thread th_;
 };
 

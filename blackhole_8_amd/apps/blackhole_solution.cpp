@@ -22,12 +22,13 @@
 namespace {
 
 const char kUsage[] =
-    "usage: blackhole_solution [--frames N] [--out DIR] [--format ppm|png]\n"
+    "usage: blackhole_solution [--frames N] [--out DIR] [--format ppm|png|jpg|avi]\n"
     "                          [--width W --height H] [--fp32] [--escape R]\n"
     "                          [--disc-texture FILE] [--keys STRING] [--depth D]\n"
     "                          [--device I] [--quiet] [--dry-run]\n"
     "  --frames N          frames to render (default 3)\n"
-    "  --out DIR           write DIR/frame_NNNNN.<format> (default: no files)\n"
+    "  --out DIR           write DIR/frame_NNNNN.<format>, or for avi\n"
+    "                      DIR/video.avi (MJPG, 29 fps); default: no files\n"
     "  --width/--height    frame size (default 960x540, the reference's)\n"
     "  --fp32              FP32 march with the FP64 photon-sphere band\n"
     "                      (default: FP64, the reference's operation order)\n"
@@ -56,7 +57,7 @@ int main(int argc, char** argv) {
     const std::string out = a.get("--out", ""), fmt = a.get("--format", "ppm");
     const std::string keys = a.get("--keys", "");
     if (frames < 1 || W < 1 || H < 1) throw std::invalid_argument("frames/width/height must be positive");
-    if (fmt != "ppm" && fmt != "png") throw std::invalid_argument("--format is ppm or png");
+    if (!app::valid_format(fmt)) throw std::invalid_argument("--format is ppm, png, jpg or avi");
 
     const bool dry = a.flag("--dry-run");
     std::unique_ptr<bh::Context> ctx;

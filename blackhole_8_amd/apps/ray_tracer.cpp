@@ -23,11 +23,12 @@
 namespace {
 
 const char kUsage[] =
-    "usage: ray_tracer [--frames N] [--out DIR] [--format ppm|png]\n"
+    "usage: ray_tracer [--frames N] [--out DIR] [--format ppm|png|jpg|avi]\n"
     "                  [--width W --height H] [--textures A,B,C] [--keys STRING]\n"
     "                  [--depth D] [--device I] [--quiet] [--dry-run]\n"
     "  --frames N        frames to render (default 3)\n"
-    "  --out DIR         write DIR/frame_NNNNN.<format> (default: no files)\n"
+    "  --out DIR         write DIR/frame_NNNNN.<format>, or for avi\n"
+    "                    DIR/video.avi (MJPG, 29 fps); default: no files\n"
     "  --width/--height  frame size (default 800x450, the reference's)\n"
     "  --textures A,B,C  PNG, JPEG or binary PPM images for the three rectangles,\n"
     "                    e.g. the reference's resource/mooni.jpeg,karina.jpeg,\n"
@@ -55,7 +56,7 @@ int main(int argc, char** argv) {
     const std::string out = a.get("--out", ""), fmt = a.get("--format", "ppm");
     const std::string keys = a.get("--keys", "");
     if (frames < 1 || W < 1 || H < 1) throw std::invalid_argument("frames/width/height must be positive");
-    if (fmt != "ppm" && fmt != "png") throw std::invalid_argument("--format is ppm or png");
+    if (!app::valid_format(fmt)) throw std::invalid_argument("--format is ppm, png, jpg or avi");
 
     const bool dry = a.flag("--dry-run");
     std::unique_ptr<bh::Context> ctx;

@@ -1,7 +1,7 @@
 // bh_io.cpp — image files on the host: the PNG decoder that stands in for the
 // reference's texture input (cv::imread(path, IMREAD_COLOR), config.h:35-37),
-// and the PPM / PNG frame writer that stands in for its cv::VideoWriter
-// (blackhole_solution_test.cc:69-72,330).
+// and the PPM / PNG / JPEG frame writer and Motion-JPEG AVI writer that stand
+// in for its cv::VideoWriter (blackhole_solution_test.cc:69-72,330).
 //
 // The decoder follows the PNG specification (W3C PNG, 2nd edition): chunk
 // layout and CRC, IHDR, PLTE, IDAT as one zlib stream, the five scanline
@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -369,6 +370,194 @@ const char* bh_jpeg_last_error(void);
 
 const char* bh_io_last_error(void) { return t_err.c_str(); }
 
+}  // extern "C"
+
+namespace {
+
+bool ends_with(const std::string& p, const char* suf) {
+  const size_t n = std::strlen(suf);
+  return p.size() >= n && p.compare(p.size() - n, n, suf) == 0;
+}
+
+void put_le32(uint8_t* p, uint32_t x) {
+  for (int i = 0; i < 4; ++i) p[i] = (uint8_t)(x >> (8 * i));
+}
+
+// Header layout of the AVI file written below (byte offsets):
+//   0   RIFF <size> 'AVI '
+//   12  LIST <size> 'hdrl'
+//   24  'avih' 56: us/frame, max bytes/s, padding, flags (HASINDEX),
+//       total frames (@48), initial frames, streams, buffer size, W, H, 4 reserved
+//   88  LIST <size> 'strl'
+//   100 'strh' 56: 'vids' 'MJPG', flags, priority, language, initial frames,
+//       scale, rate, start, length (@140), buffer size, quality, sample size,
+//       rcFrame (4 x int16)
+//   164 'strf' 40: BITMAPINFOHEADER (24-bit, compression 'MJPG')
+//   212 LIST <size> 'movi'   (frames follow at 224)
+constexpr size_t kHdr = 224, kMoviList = 212;
+constexpr uint64_t kMaxAvi = (uint64_t)1 << 30;
+
+}  // namespace
+
+struct bh_video {
+  FILE* f = nullptr;
+  int rows = 0, cols = 0, quality = 95;
+  double fps = 29;
+  uint64_t pos = kHdr;           // file size so far
+  uint32_t max_frame = 0;
+  std::vector<uint8_t> idx;      // idx1 entries, 16 bytes each
+  int err = BH_OK;
+  std::string msg;
+  uint8_t hdr[kHdr] = {};
+};
+
+namespace {
+
+int video_fail(bh_video* v, int code, const std::string& m) {
+  if (v->err == BH_OK) {
+    v->err = code;
+    v->msg = m;
+  }
+  return io_fail(code, m);
+}
+
+// Header for `frames` frames written so far; the RIFF and movi sizes come
+// from the file position and the index built so far.
+void fill_header(bh_video* v, uint32_t frames) {
+  uint8_t* h = v->hdr;
+  const double fps = v->fps;
+  auto tag = [&](size_t at, const char* t) { std::memcpy(h + at, t, 4); };
+  // frame rate as rate / scale: whole fps exactly, else in 1/1000 frames/s
+  const bool whole = fps == (double)(uint32_t)fps;
+  const uint32_t scale = whole ? 1 : 1000, rate = whole ? (uint32_t)fps : (uint32_t)(fps * 1000 + 0.5);
+  const uint32_t frame_bytes = (uint32_t)v->rows * v->cols * 3;
+  tag(0, "RIFF");
+  put_le32(h + 4, (uint32_t)(v->pos + 8 + v->idx.size() - 8));  // file size - 8
+  tag(8, "AVI ");
+  tag(12, "LIST");
+  put_le32(h + 16, 212 - 20);
+  tag(20, "hdrl");
+  tag(24, "avih");
+  put_le32(h + 28, 56);
+  put_le32(h + 32, (uint32_t)(1e6 / fps + 0.5));
+  put_le32(h + 36, (uint32_t)(v->max_frame * fps + 0.5));
+  put_le32(h + 44, 0x10);  // AVIF_HASINDEX
+  put_le32(h + 48, frames);
+  put_le32(h + 56, 1);
+  put_le32(h + 60, v->max_frame);
+  put_le32(h + 64, (uint32_t)v->cols);
+  put_le32(h + 68, (uint32_t)v->rows);
+  tag(88, "LIST");
+  put_le32(h + 92, 212 - 96);
+  tag(96, "strl");
+  tag(100, "strh");
+  put_le32(h + 104, 56);
+  tag(108, "vids");
+  tag(112, "MJPG");
+  put_le32(h + 128, scale);
+  put_le32(h + 132, rate);
+  put_le32(h + 140, frames);
+  put_le32(h + 144, v->max_frame);
+  put_le32(h + 148, 0xFFFFFFFFu);  // default quality
+  h[160] = (uint8_t)(v->cols & 0xFF);
+  h[161] = (uint8_t)(v->cols >> 8);
+  h[162] = (uint8_t)(v->rows & 0xFF);
+  h[163] = (uint8_t)(v->rows >> 8);
+  tag(164, "strf");
+  put_le32(h + 168, 40);
+  put_le32(h + 172, 40);
+  put_le32(h + 176, (uint32_t)v->cols);
+  put_le32(h + 180, (uint32_t)v->rows);
+  h[184] = 1;   // planes
+  h[186] = 24;  // bits per pixel
+  tag(188, "MJPG");
+  put_le32(h + 192, frame_bytes);
+  tag(kMoviList, "LIST");
+  put_le32(h + kMoviList + 4, (uint32_t)(v->pos - kMoviList - 8));
+  tag(kMoviList + 8, "movi");
+}
+
+}  // namespace
+
+extern "C" {
+
+int bh_video_open(const char* path, int rows, int cols, double fps, int quality, bh_video** out) {
+  if (!out) return io_fail(BH_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (!path || rows <= 0 || cols <= 0 || rows > 65535 || cols > 65535)
+    return io_fail(BH_ERR_INVALID_ARG, "bad video size");
+  if (!(fps > 0 && fps <= 1000)) return io_fail(BH_ERR_INVALID_ARG, "fps must be in (0, 1000]");
+  if (quality < 1 || quality > 100) return io_fail(BH_ERR_INVALID_ARG, "quality must be 1..100");
+  bh_video* v = new (std::nothrow) bh_video;
+  if (!v) return io_fail(BH_ERR_OUT_OF_MEMORY, "out of host memory");
+  v->rows = rows;
+  v->cols = cols;
+  v->quality = quality;
+  v->f = std::fopen(path, "wb");
+  if (!v->f) {
+    delete v;
+    return io_fail(BH_ERR_INVALID_ARG, std::string("cannot write ") + path);
+  }
+  v->fps = fps;
+  fill_header(v, 0);  // rewritten by bh_video_close with the counts
+  if (std::fwrite(v->hdr, 1, kHdr, v->f) != kHdr) {
+    std::fclose(v->f);
+    delete v;
+    return io_fail(BH_ERR_INVALID_ARG, std::string("write failed: ") + path);
+  }
+  *out = v;
+  return BH_OK;
+}
+
+int bh_video_write(bh_video* v, const uint8_t* bgr) {
+  if (!v || !bgr) return io_fail(BH_ERR_INVALID_ARG, "video/frame is NULL");
+  if (v->err != BH_OK) return io_fail(v->err, v->msg);
+  uint8_t* jpg = nullptr;
+  size_t n = 0;
+  int rc = bh_jpeg_encode(bgr, v->rows, v->cols, v->quality, &jpg, &n);
+  if (rc != BH_OK) return video_fail(v, rc, bh_jpeg_last_error());
+  const uint64_t chunk = 8 + n + (n & 1);
+  if (v->pos + chunk + v->idx.size() + 16 + 8 > kMaxAvi) {
+    bh_free(jpg);
+    return video_fail(v, BH_ERR_UNSUPPORTED, "AVI file would pass 1 GiB");
+  }
+  uint8_t ch[8] = {'0', '0', 'd', 'c'};
+  put_le32(ch + 4, (uint32_t)n);
+  const uint8_t pad = 0;
+  bool ok = std::fwrite(ch, 1, 8, v->f) == 8 && std::fwrite(jpg, 1, n, v->f) == n &&
+            (!(n & 1) || std::fwrite(&pad, 1, 1, v->f) == 1);
+  bh_free(jpg);
+  if (!ok) return video_fail(v, BH_ERR_INVALID_ARG, "video write failed");
+  uint8_t e[16] = {'0', '0', 'd', 'c'};
+  put_le32(e + 4, 0x10);  // AVIIF_KEYFRAME
+  put_le32(e + 8, (uint32_t)(v->pos - (kMoviList + 8)));  // from the 'movi' tag
+  put_le32(e + 12, (uint32_t)n);
+  v->idx.insert(v->idx.end(), e, e + 16);
+  v->pos += chunk;
+  if (n > v->max_frame) v->max_frame = (uint32_t)n;
+  return BH_OK;
+}
+
+int bh_video_close(bh_video* v) {
+  if (!v) return io_fail(BH_ERR_INVALID_ARG, "video is NULL");
+  int rc = v->err;
+  std::string msg = v->msg;
+  uint8_t ih[8] = {'i', 'd', 'x', '1'};
+  put_le32(ih + 4, (uint32_t)v->idx.size());
+  bool ok = std::fwrite(ih, 1, 8, v->f) == 8 &&
+            std::fwrite(v->idx.data(), 1, v->idx.size(), v->f) == v->idx.size();
+  fill_header(v, (uint32_t)(v->idx.size() / 16));
+  ok = ok && std::fseek(v->f, 0, SEEK_SET) == 0 &&
+       std::fwrite(v->hdr, 1, kHdr, v->f) == kHdr;
+  ok = (std::fclose(v->f) == 0) && ok;
+  delete v;
+  if (rc == BH_OK && !ok) {
+    rc = BH_ERR_INVALID_ARG;
+    msg = "video write failed";
+  }
+  return rc == BH_OK ? BH_OK : io_fail(rc, msg);
+}
+
 int bh_png_decode(const uint8_t* data, size_t size, uint8_t* dst, size_t dst_size, int* rows,
                   int* cols) {
   if (!rows || !cols) return io_fail(BH_ERR_INVALID_ARG, "rows/cols are NULL");
@@ -401,12 +590,20 @@ int bh_image_load(const char* path, uint8_t* dst, size_t dst_size, int* rows, in
 int bh_image_write(const char* path, const uint8_t* bgr, int rows, int cols) {
   if (!path || !bgr || rows <= 0 || cols <= 0) return io_fail(BH_ERR_INVALID_ARG, "bad image");
   const std::string p(path);
-  const bool png = p.size() >= 4 && p.compare(p.size() - 4, 4, ".png") == 0;
-  const bool ppm = p.size() >= 4 && p.compare(p.size() - 4, 4, ".ppm") == 0;
-  if (!png && !ppm) return io_fail(BH_ERR_UNSUPPORTED, "path must end in .ppm or .png");
+  const bool png = ends_with(p, ".png"), ppm = ends_with(p, ".ppm");
+  const bool jpg = ends_with(p, ".jpg") || ends_with(p, ".jpeg");
+  if (!png && !ppm && !jpg)
+    return io_fail(BH_ERR_UNSUPPORTED, "path must end in .ppm, .png, .jpg or .jpeg");
   const size_t n = (size_t)rows * cols;
   std::vector<uint8_t> out;
-  if (ppm) {
+  if (jpg) {
+    uint8_t* d = nullptr;
+    size_t dn = 0;
+    const int rc = bh_jpeg_encode(bgr, rows, cols, 95, &d, &dn);
+    if (rc != BH_OK) return io_fail(rc, bh_jpeg_last_error());
+    out.assign(d, d + dn);
+    bh_free(d);
+  } else if (ppm) {
     char hdr[64];
     const int hl = std::snprintf(hdr, sizeof hdr, "P6\n%d %d\n255\n", cols, rows);
     out.assign(hdr, hdr + hl);

@@ -13,9 +13,21 @@
 // JFIF with 16-bit scaling; three 8-bit channels in B, G, R order.
 // Progressive, arithmetic-coded, 12-bit and 4-component (CMYK/YCCK) files
 // are refused with BH_ERR_UNSUPPORTED.
+//
+// The second half is a baseline encoder for frame output (the JPEG frames of
+// the MJPG video the reference writes with cv::VideoWriter,
+// blackhole_solution_test.cc:71-72,330): JFIF YCbCr, 2x2-averaged chroma
+// (4:2:0), the Annex K quantization tables scaled by quality, and Huffman
+// tables made for each image from its own symbol counts (Annex K.2).
+#include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <array>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "bh_api.h"
@@ -206,14 +218,9 @@ inline const uint8_t* crow(const Comp& c, int r) {
   return c.px.data() + (size_t)r * c.bw * 8;
 }
 
-// Horizontal 2:1 triangle filter of one row of w samples into 2w.
+// Horizontal 2:1 triangle filter of one row of w >= 2 samples into 2w.
 void h2_row(const int* cs, int w, uint8_t* out, int bias_even, int bias_odd, int scale) {
   // cs: column sums (already weighted vertically, or the samples x scale)
-  if (w == 1) {
-    out[0] = (uint8_t)((cs[0] * 4 + bias_even) >> scale);
-    out[1] = (uint8_t)((cs[0] * 4 + bias_odd) >> scale);
-    return;
-  }
   out[0] = (uint8_t)((cs[0] * 4 + bias_even) >> scale);
   out[1] = (uint8_t)((cs[0] * 3 + cs[1] + bias_odd) >> scale);
   for (int j = 1; j < w - 1; ++j) {
@@ -224,12 +231,16 @@ void h2_row(const int* cs, int w, uint8_t* out, int bias_even, int bias_odd, int
   out[2 * w - 1] = (uint8_t)((cs[w - 1] * 4 + bias_odd) >> scale);
 }
 
-// Full-resolution plane (W x H) of component c.
+// Full-resolution plane (W x H) of component c.  As in libjpeg, the
+// horizontal 2:1 cases (h2v1, h2v2) use the triangle filter only for
+// components more than 2 samples wide; narrower ones are replicated.
 std::vector<uint8_t> upsample(const Jpeg& j, const Comp& c) {
   const int W = j.W, H = j.H;
   std::vector<uint8_t> out((size_t)W * H);
-  const int fx = j.hmax / c.h, fy = j.vmax / c.v;
+  int fx = j.hmax / c.h;
+  const int fy = j.vmax / c.v;
   const bool exact_x = c.h * fx == j.hmax, exact_y = c.v * fy == j.vmax;
+  if (fx == 2 && c.dw <= 2) fx = 0;  // to the replication branch
   std::vector<int> cs((size_t)c.dw);
   std::vector<uint8_t> tmp((size_t)c.dw * 2 + 2);
   for (int y = 0; y < H; ++y) {
@@ -553,5 +564,427 @@ int bh_jpeg_decode(const uint8_t* data, size_t size, uint8_t* dst, size_t dst_si
     return BH_ERR_OUT_OF_MEMORY;
   }
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// Encoder: baseline JPEG (4:2:0, JFIF) for frame output and MJPG video.
+// ===========================================================================
+namespace {
+
+// Quantization tables of ITU-T T.81 Annex K.1 (natural order), scaled by
+// quality the usual way: s = q < 50 ? 5000 / q : 200 - 2 q, t = (b s + 50) / 100.
+const uint8_t kLumQ[64] = {16, 11, 10, 16, 24,  40,  51,  61,  12, 12, 14, 19, 26,  58,  60,  55,
+                           14, 13, 16, 24, 40,  57,  69,  56,  14, 17, 22, 29, 51,  87,  80,  62,
+                           18, 22, 37, 56, 68,  109, 103, 77,  24, 35, 55, 64, 81,  104, 113, 92,
+                           49, 64, 78, 87, 103, 121, 120, 101, 72, 92, 95, 98, 112, 100, 103, 99};
+const uint8_t kChrQ[64] = {17, 18, 24, 47, 99, 99, 99, 99, 18, 21, 26, 66, 99, 99, 99, 99,
+                           24, 26, 56, 99, 99, 99, 99, 99, 47, 66, 99, 99, 99, 99, 99, 99,
+                           99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99,
+                           99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99};
+
+// Entropy-coded data writer: bits MSB first, FF followed by a stuffed 00.
+// Whole 32-bit words go out at once unless one of their bytes is FF; `out`
+// grows ahead of the write position and is cut to length by flush().
+struct BitWriter {
+  std::vector<uint8_t>& out;
+  size_t len;        // bytes written
+  uint64_t acc = 0;  // the low n bits are pending, n < 32 between calls
+  int n = 0;
+  explicit BitWriter(std::vector<uint8_t>& o) : out(o), len(o.size()) {}
+  void room() {
+    if (len + 8 > out.size()) out.resize(out.size() * 2 + 4096);
+  }
+  void byte(uint8_t b) {
+    out[len++] = b;
+    if (b == 0xFF) out[len++] = 0x00;
+  }
+  void put(uint32_t bits, int nb) {  // nb <= 32, bits < 2^nb
+    acc = (acc << nb) | bits;
+    n += nb;
+    if (n >= 32) {
+      n -= 32;
+      const uint32_t w = (uint32_t)(acc >> n);
+      room();
+      if (((~w - 0x01010101u) & w & 0x80808080u) == 0) {  // no FF byte
+        const uint32_t be = __builtin_bswap32(w);
+        std::memcpy(out.data() + len, &be, 4);
+        len += 4;
+      } else {
+        for (int k = 24; k >= 0; k -= 8) byte((uint8_t)(w >> k));
+      }
+    }
+  }
+  void align() {  // whole bytes out, the last padded with 1 bits
+    const int pad = (8 - (n & 7)) & 7;
+    acc = (acc << pad) | ((1u << pad) - 1);
+    n += pad;
+    room();
+    while (n > 0) {
+      n -= 8;
+      byte((uint8_t)(acc >> n));
+    }
+  }
+  void marker(uint8_t m) {  // after align()
+    room();
+    out[len++] = 0xFF;
+    out[len++] = m;
+  }
+};
+
+// Code lengths (<= 16) of a Huffman code for the symbol frequencies, with one
+// reserved all-ones code point, by merging least-frequent pairs and then
+// shortening codes longer than 16 bits (T.81 Annex K.2); returns BITS[1..16]
+// and HUFFVAL in code order.
+void optimal_table(const uint32_t* freq_in, uint8_t* bits, std::vector<uint8_t>& vals) {
+  std::vector<int64_t> freq(257, 0);
+  for (int i = 0; i < 256; ++i) freq[i] = freq_in[i];
+  freq[256] = 1;  // reserved: no code may be all ones
+  std::vector<int> codesize(257, 0), others(257, -1);
+  for (;;) {
+    int c1 = -1, c2 = -1;
+    for (int i = 0; i <= 256; ++i) {  // least frequency, ties to the larger index
+      if (freq[i] == 0) continue;
+      if (c1 < 0 || freq[i] <= freq[c1]) c1 = i;
+    }
+    for (int i = 0; i <= 256; ++i) {
+      if (freq[i] == 0 || i == c1) continue;
+      if (c2 < 0 || freq[i] <= freq[c2]) c2 = i;
+    }
+    if (c2 < 0) break;
+    freq[c1] += freq[c2];
+    freq[c2] = 0;
+    ++codesize[c1];
+    while (others[c1] >= 0) {
+      c1 = others[c1];
+      ++codesize[c1];
+    }
+    others[c1] = c2;
+    ++codesize[c2];
+    while (others[c2] >= 0) {
+      c2 = others[c2];
+      ++codesize[c2];
+    }
+  }
+  int count[33] = {0};
+  for (int i = 0; i <= 256; ++i)
+    if (codesize[i]) ++count[codesize[i] > 32 ? 32 : codesize[i]];
+  for (int i = 32; i > 16; --i) {  // limit to 16 bits
+    while (count[i] > 0) {
+      int j = i - 2;
+      while (count[j] == 0) --j;
+      count[i] -= 2;
+      count[i - 1] += 1;
+      count[j + 1] += 2;
+      count[j] -= 1;
+    }
+  }
+  int i = 16;
+  while (count[i] == 0) --i;
+  count[i] -= 1;  // drop the reserved code
+  for (int l = 1; l <= 16; ++l) bits[l - 1] = (uint8_t)count[l];
+  vals.clear();
+  for (int l = 1; l <= 32; ++l)
+    for (int s = 0; s < 256; ++s)
+      if (codesize[s] == l) vals.push_back((uint8_t)s);
+}
+
+struct Code {
+  uint16_t code[256];
+  uint8_t len[256];
+};
+
+void make_codes(const uint8_t* bits, const std::vector<uint8_t>& vals, Code& c) {
+  std::memset(&c, 0, sizeof c);
+  int k = 0;
+  uint32_t code = 0;
+  for (int l = 1; l <= 16; ++l) {
+    for (int i = 0; i < bits[l - 1]; ++i, ++k) {
+      c.code[vals[(size_t)k]] = (uint16_t)code;
+      c.len[vals[(size_t)k]] = (uint8_t)l;
+      ++code;
+    }
+    code <<= 1;
+  }
+}
+
+inline int nbits(int v) {  // magnitude category: bits of |v|
+  const unsigned a = (unsigned)(v < 0 ? -v : v);
+  return a ? 32 - __builtin_clz(a) : 0;
+}
+
+// One 8-point forward DCT pass by the Arai-Agui-Nakajima factorisation
+// (Arai, Agui, Nakajima, Trans. IEICE E71(11), 1988; 5 multiplies): d[k *
+// stride] becomes 8 s(k) F(k) with the T.81 normalisation of F and
+// s(0) = 1, s(k) = sqrt 2 cos(k pi / 16) for k > 0.
+inline void aan_1d(float* d, int st) {
+  const float t0 = d[0] + d[7 * st], t7 = d[0] - d[7 * st];
+  const float t1 = d[st] + d[6 * st], t6 = d[st] - d[6 * st];
+  const float t2 = d[2 * st] + d[5 * st], t5 = d[2 * st] - d[5 * st];
+  const float t3 = d[3 * st] + d[4 * st], t4 = d[3 * st] - d[4 * st];
+  // even part
+  const float e10 = t0 + t3, e13 = t0 - t3, e11 = t1 + t2, e12 = t1 - t2;
+  d[0] = e10 + e11;
+  d[4 * st] = e10 - e11;
+  const float z1 = (e12 + e13) * 0.707106781f;  // cos(pi / 4)
+  d[2 * st] = e13 + z1;
+  d[6 * st] = e13 - z1;
+  // odd part
+  const float o10 = t4 + t5, o11 = t5 + t6, o12 = t6 + t7;
+  const float z5 = (o10 - o12) * 0.382683433f;  // cos(3 pi / 8)
+  const float z2 = 0.541196100f * o10 + z5;     // cos(pi / 8) - cos(3 pi / 8)
+  const float z4 = 1.306562965f * o12 + z5;     // cos(pi / 8) + cos(3 pi / 8)
+  const float z3 = o11 * 0.707106781f;
+  const float z11 = t7 + z3, z13 = t7 - z3;
+  d[5 * st] = z13 + z2;
+  d[3 * st] = z13 - z2;
+  d[st] = z11 + z4;
+  d[7 * st] = z11 - z4;
+}
+
+// 2-D DCT of a level-shifted block (rows, then columns), quantized by
+// `div` = 1 / (8 s(u) s(v) Q(v, u)), rounded to nearest, natural order
+// (row = vertical frequency), clamped to baseline's 10-bit AC categories.
+inline void fdct_quant(float* blk, const float* div, int16_t* out) {
+  for (int y = 0; y < 8; ++y) aan_1d(blk + 8 * y, 1);
+  for (int x = 0; x < 8; ++x) aan_1d(blk + x, 8);
+  for (int k = 0; k < 64; ++k) {
+    const float c = std::nearbyint(blk[k] * div[k]);
+    out[k] = (int16_t)(c < -1023.f ? -1023 : (c > 1023.f ? 1023 : (int)c));
+  }
+}
+
+// Transforms MCU rows [j0, j1) of the image into blocks Y0 Y1 Y2 Y3 Cb Cr per
+// 16x16 MCU (edge pixels replicated past the image); chroma is the mean of
+// each 2x2 pixel group.
+void mcu_rows(const uint8_t* bgr, int H, int W, int mx, int j0, int j1, const float (*div)[64],
+              int16_t* coefs) {
+  float y[4][64], cb[64], cr[64];
+  float rc[16][16], bc[16][16];
+  for (int j = j0; j < j1; ++j)
+    for (int i = 0; i < mx; ++i) {
+      for (int r = 0; r < 16; ++r) {
+        const int sy = j * 16 + r < H ? j * 16 + r : H - 1;
+        const uint8_t* row = bgr + (size_t)sy * W * 3;
+        for (int c = 0; c < 16; ++c) {
+          const int sx = i * 16 + c < W ? i * 16 + c : W - 1;
+          const float b = row[3 * sx], g = row[3 * sx + 1], rr = row[3 * sx + 2];
+          y[(r >> 3) * 2 + (c >> 3)][(r & 7) * 8 + (c & 7)] =
+              0.299f * rr + 0.587f * g + 0.114f * b - 128.f;
+          bc[r][c] = -0.168736f * rr - 0.331264f * g + 0.5f * b;
+          rc[r][c] = 0.5f * rr - 0.418688f * g - 0.081312f * b;
+        }
+      }
+      for (int r = 0; r < 8; ++r)
+        for (int c = 0; c < 8; ++c) {
+          cb[r * 8 + c] = 0.25f * (bc[2 * r][2 * c] + bc[2 * r][2 * c + 1] +
+                                   bc[2 * r + 1][2 * c] + bc[2 * r + 1][2 * c + 1]);
+          cr[r * 8 + c] = 0.25f * (rc[2 * r][2 * c] + rc[2 * r][2 * c + 1] +
+                                   rc[2 * r + 1][2 * c] + rc[2 * r + 1][2 * c + 1]);
+        }
+      int16_t* o = coefs + ((size_t)j * mx + i) * 6 * 64;
+      for (int b = 0; b < 4; ++b) fdct_quant(y[b], div[0], o + b * 64);
+      fdct_quant(cb, div[1], o + 4 * 64);
+      fdct_quant(cr, div[1], o + 5 * 64);
+    }
+}
+
+}  // namespace
+
+namespace {
+
+// Calls emit(table, symbol, value) for every Huffman symbol of MCU row j
+// (table 0/1: luminance DC/AC, 2/3: chrominance DC/AC), DC predictors
+// starting from zero as they do after a restart marker.
+template <class F>
+void walk_row(const int16_t* coefs, int mx, int j, F&& emit) {
+  int pred[3] = {0, 0, 0};
+  const int16_t* blk = coefs + (size_t)j * mx * 6 * 64;
+  for (int b = 0; b < mx * 6; ++b, blk += 64) {
+    const int comp = b % 6 < 4 ? 0 : b % 6 - 3;
+    const int t = comp ? 2 : 0;
+    const int diff = blk[0] - pred[comp];
+    pred[comp] = blk[0];
+    emit(t, nbits(diff), diff);
+    int run = 0;
+    for (int k = 1; k < 64; ++k) {
+      const int v = blk[kZigzag[k]];
+      if (v == 0) {
+        ++run;
+        continue;
+      }
+      for (; run > 15; run -= 16) emit(t + 1, 0xF0, 0);
+      emit(t + 1, (run << 4) | nbits(v), v);
+      run = 0;
+    }
+    if (run) emit(t + 1, 0x00, 0);
+  }
+}
+
+// Runs fn(part, j0, j1) over `parts` contiguous ranges of [0, n) on host
+// threads (range 0 on the calling thread; inline where no thread starts).
+template <class F>
+void parallel_ranges(int parts, int n, F&& fn) {
+  std::vector<std::thread> th;
+  for (int t = 1; t < parts; ++t) {
+    const int j0 = (int)((int64_t)n * t / parts), j1 = (int)((int64_t)n * (t + 1) / parts);
+    try {
+      th.emplace_back([&fn, t, j0, j1] { fn(t, j0, j1); });
+    } catch (const std::system_error&) {
+      fn(t, j0, j1);
+    }
+  }
+  fn(0, 0, (int)((int64_t)n / parts));
+  for (auto& t : th) t.join();
+}
+
+// Encode rows x cols x 3 B,G,R bytes; appends the file to `out`.  Each MCU
+// row is one restart interval, so host threads code their rows on their own
+// and the segments are joined with RST0..RST7 markers between them.
+void encode_jpeg(const uint8_t* bgr, int H, int W, int quality, std::vector<uint8_t>& out) {
+  quality = quality < 1 ? 1 : (quality > 100 ? 100 : quality);
+  const int scale = quality < 50 ? 5000 / quality : 200 - 2 * quality;
+  uint16_t q[2][64];
+  float div[2][64];
+  for (int k = 0; k < 64; ++k) {
+    int a = (kLumQ[k] * scale + 50) / 100, b = (kChrQ[k] * scale + 50) / 100;
+    q[0][k] = (uint16_t)(a < 1 ? 1 : (a > 255 ? 255 : a));
+    q[1][k] = (uint16_t)(b < 1 ? 1 : (b > 255 ? 255 : b));
+  }
+  auto aan_scale = [](int k) {
+    return k ? std::sqrt(2.0) * std::cos(k * 3.14159265358979323846 / 16) : 1.0;
+  };
+  for (int v = 0; v < 8; ++v)
+    for (int u = 0; u < 8; ++u) {
+      const double f = 8.0 * aan_scale(u) * aan_scale(v);
+      div[0][v * 8 + u] = (float)(1.0 / (f * q[0][v * 8 + u]));
+      div[1][v * 8 + u] = (float)(1.0 / (f * q[1][v * 8 + u]));
+    }
+  const int mx = (W + 15) / 16, my = (H + 15) / 16;
+  std::vector<int16_t> coefs((size_t)mx * my * 6 * 64);
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nt = std::max(1, std::min({(int)(hw ? hw : 1), 8, my / 2}));
+  // blocks and symbol counts per thread
+  std::vector<std::array<std::array<uint32_t, 256>, 4>> freq((size_t)nt);
+  parallel_ranges(nt, my, [&](int t, int j0, int j1) {
+    mcu_rows(bgr, H, W, mx, j0, j1, div, coefs.data());
+    auto& f = freq[(size_t)t];
+    for (auto& a : f) a.fill(0);
+    for (int j = j0; j < j1; ++j) walk_row(coefs.data(), mx, j, [&](int k, int sym, int) { ++f[k][sym]; });
+  });
+  uint32_t tot[4][256] = {};
+  for (const auto& f : freq)
+    for (int k = 0; k < 4; ++k)
+      for (int s = 0; s < 256; ++s) tot[k][s] += f[k][s];
+  uint8_t bits[4][16];
+  std::vector<uint8_t> vals[4];
+  Code codes[4];
+  for (int k = 0; k < 4; ++k) {
+    optimal_table(tot[k], bits[k], vals[k]);
+    make_codes(bits[k], vals[k], codes[k]);
+  }
+  // entropy-coded segments, one buffer per thread
+  std::vector<std::vector<uint8_t>> seg((size_t)nt);
+  parallel_ranges(nt, my, [&](int t, int j0, int j1) {
+    auto& o = seg[(size_t)t];
+    o.resize((size_t)(j1 - j0) * mx * 6 * 24 + 4096);
+    BitWriter w(o);
+    w.len = 0;
+    for (int j = j0; j < j1; ++j) {
+      walk_row(coefs.data(), mx, j, [&](int k, int sym, int v) {
+        const Code& c = codes[k];
+        const int nx = (k & 1) ? (sym & 15) : sym;  // code, then nx extra bits (<= 16 + 11)
+        const uint32_t extra = (uint32_t)(v < 0 ? v - 1 : v) & ((1u << nx) - 1);
+        w.put(((uint32_t)c.code[sym] << nx) | extra, c.len[sym] + nx);
+      });
+      w.align();
+      if (j + 1 < my) w.marker((uint8_t)(0xD0 + (j & 7)));
+    }
+    o.resize(w.len);
+  });
+  // headers
+  auto u8 = [&](int v) { out.push_back((uint8_t)v); };
+  auto u16 = [&](int v) {
+    u8(v >> 8);
+    u8(v & 0xFF);
+  };
+  u16(0xFFD8);
+  u16(0xFFE0);  // JFIF 1.01, no units, 1:1, no thumbnail
+  u16(16);
+  for (char ch : {'J', 'F', 'I', 'F', '\0'}) u8(ch);
+  u8(1);
+  u8(1);
+  u8(0);
+  u16(1);
+  u16(1);
+  u8(0);
+  u8(0);
+  for (int t = 0; t < 2; ++t) {
+    u16(0xFFDB);
+    u16(67);
+    u8(t);
+    for (int k = 0; k < 64; ++k) u8(q[t][kZigzag[k]]);
+  }
+  u16(0xFFC0);  // 8-bit, 3 components: Y 2x2 (table 0), Cb and Cr 1x1 (table 1)
+  u16(17);
+  u8(8);
+  u16(H);
+  u16(W);
+  u8(3);
+  u8(1), u8(0x22), u8(0);
+  u8(2), u8(0x11), u8(1);
+  u8(3), u8(0x11), u8(1);
+  for (int k = 0; k < 4; ++k) {
+    u16(0xFFC4);
+    u16(3 + 16 + (int)vals[k].size());
+    u8(((k & 1) << 4) | (k >> 1));  // class, id: DC0 AC0 DC1 AC1
+    for (int l = 0; l < 16; ++l) u8(bits[k][l]);
+    for (uint8_t v : vals[k]) u8(v);
+  }
+  u16(0xFFDD);  // restart interval: one MCU row
+  u16(4);
+  u16(mx);
+  u16(0xFFDA);
+  u16(12);
+  u8(3);
+  u8(1), u8(0x00);
+  u8(2), u8(0x11);
+  u8(3), u8(0x11);
+  u8(0);
+  u8(63);
+  u8(0);
+  for (const auto& o : seg) out.insert(out.end(), o.begin(), o.end());
+  u16(0xFFD9);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bh_jpeg_encode(const uint8_t* bgr, int rows, int cols, int quality, uint8_t** out_data,
+                   size_t* out_size) {
+  if (!bgr || rows <= 0 || cols <= 0 || rows > 65535 || cols > 65535 || !out_data || !out_size) {
+    t_jerr = "bad image";
+    return BH_ERR_INVALID_ARG;
+  }
+  try {
+    std::vector<uint8_t> v;
+    encode_jpeg(bgr, rows, cols, quality, v);
+    uint8_t* p = (uint8_t*)std::malloc(v.size());
+    if (!p) {
+      t_jerr = "out of host memory";
+      return BH_ERR_OUT_OF_MEMORY;
+    }
+    std::memcpy(p, v.data(), v.size());
+    *out_data = p;
+    *out_size = v.size();
+    return BH_OK;
+  } catch (const std::bad_alloc&) {
+    t_jerr = "out of host memory";
+    return BH_ERR_OUT_OF_MEMORY;
+  }
+}
+
+void bh_free(void* p) { std::free(p); }
 
 }  // extern "C"

@@ -4,7 +4,9 @@
 cv::imread(path, IMREAD_COLOR) (config.h:35-37): the result is rows x cols x 3
 bytes in B, G, R order with alpha dropped, the layout Material::SetTexture
 takes (material.h:29-35).  `write` saves a frame (B, G, R bytes, as the
-framebuffer holds them for such textures) as .ppm or .png.
+framebuffer holds them for such textures) as .ppm, .png or .jpg; `Video`
+writes a Motion-JPEG AVI in place of the reference's cv::VideoWriter
+(blackhole_solution_test.cc:69-72,330).
 """
 from __future__ import annotations
 
@@ -79,3 +81,64 @@ def write(path: os.PathLike | str, bgr: np.ndarray) -> None:
     rc = abi.load().bh_image_write(os.fsencode(path), abi.ptr(a), a.shape[0], a.shape[1])
     if rc:
         _err(rc)
+
+
+def encode_jpeg(bgr: np.ndarray, quality: int = 95) -> bytes:
+    """Baseline JFIF JPEG (4:2:0, optimal Huffman tables) of B, G, R bytes."""
+    a = np.ascontiguousarray(bgr, dtype=np.uint8)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("expected rows x cols x 3 bytes")
+    lib = abi.load()
+    out, n = C.POINTER(C.c_uint8)(), C.c_size_t()
+    rc = lib.bh_jpeg_encode(abi.ptr(a), a.shape[0], a.shape[1], quality, C.byref(out),
+                            C.byref(n))
+    if rc:
+        raise ImageError(rc, lib.bh_jpeg_last_error().decode())
+    try:
+        return C.string_at(out, n.value)
+    finally:
+        lib.bh_free(C.cast(out, C.c_void_p))
+
+
+class Video:
+    """Motion-JPEG AVI writer: `Video(path, rows, cols, fps=29)`, then
+    `write(frame)` per B, G, R frame and `close()` (or use it as a context
+    manager) to write the index."""
+
+    def __init__(self, path: os.PathLike | str, rows: int, cols: int, fps: float = 29.0,
+                 quality: int = 95):
+        self._lib = abi.load()
+        self.shape = (rows, cols, 3)
+        h = C.c_void_p()
+        rc = self._lib.bh_video_open(os.fsencode(path), rows, cols, fps, quality, C.byref(h))
+        if rc:
+            _err(rc)
+        self._h = h
+
+    def write(self, bgr: np.ndarray) -> None:
+        if self._h is None:
+            raise ValueError("video is closed")
+        a = np.ascontiguousarray(bgr, dtype=np.uint8)
+        if a.shape != self.shape:
+            raise ValueError(f"expected a frame of shape {self.shape}, got {a.shape}")
+        rc = self._lib.bh_video_write(self._h, abi.ptr(a))
+        if rc:
+            _err(rc)
+
+    def close(self) -> None:
+        if self._h is not None:
+            h, self._h = self._h, None
+            rc = self._lib.bh_video_close(h)
+            if rc:
+                _err(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.bh_video_close(self._h)
+            self._h = None

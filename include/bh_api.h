@@ -402,11 +402,35 @@ const char* bh_jpeg_last_error(void);
  * 255), told apart by their signatures. */
 int bh_image_load(const char* path, uint8_t* dst, size_t dst_size, int* rows, int* cols);
 /* Write rows x cols x 3 bytes in B, G, R order (the framebuffer's order for
- * textures loaded as above) to path: ".ppm" binary PPM or ".png" 8-bit RGB
- * PNG; both files store R, G, B. */
+ * textures loaded as above) to path: ".ppm" binary PPM, ".png" 8-bit RGB
+ * PNG (both store R, G, B) or ".jpg"/".jpeg" baseline JPEG at quality 95
+ * (imwrite's default). */
 int bh_image_write(const char* path, const uint8_t* bgr, int rows, int cols);
 /* Message of the last failing image-file call on this thread. */
 const char* bh_io_last_error(void);
+/* Encode rows x cols x 3 bytes in B, G, R order as a baseline JFIF JPEG
+ * (4:2:0, quality 1..100 scaling the Annex K tables, optimal Huffman tables)
+ * into a malloc'd buffer the caller frees with bh_free.  Frame output in the
+ * format of the reference's MJPG video frames (blackhole_solution_test.cc:71-72);
+ * message via bh_jpeg_last_error. */
+int bh_jpeg_encode(const uint8_t* bgr, int rows, int cols, int quality, uint8_t** out,
+                   size_t* out_size);
+void bh_free(void* p);
+
+/* Motion-JPEG AVI writer in place of cv::VideoWriter(path,
+ * fourcc('M','J','P','G'), 29, Size(W, H)) and `out_capture << frame`
+ * (blackhole_solution_test.cc:69-72,330; ray_tracer_test.cc:62-63): a RIFF
+ * 'AVI ' file with one 'vids' stream, handler and compression MJPG, one
+ * '00dc' JPEG chunk per frame and an idx1 index of key frames.  Files are
+ * limited to 1 GiB (RIFF AVI 1.0; no OpenDML extension).  Frames are
+ * rows x cols x 3 bytes in B, G, R order.  bh_video_close writes the index,
+ * fills in the frame counts and frees the writer, returning the first error
+ * of the writer's life.  Messages via bh_io_last_error. */
+typedef struct bh_video bh_video;
+int bh_video_open(const char* path, int rows, int cols, double fps, int quality,
+                  bh_video** out);
+int bh_video_write(bh_video* v, const uint8_t* bgr);
+int bh_video_close(bh_video* v);
 
 /* ======================================================================== */
 /* Host-side helpers shared by every caller (no GPU needed)                  */

@@ -22,6 +22,7 @@ import numpy as np
 import pytest
 
 from blackhole_8_amd import Ring, abi, images, scenes
+from tests import riff
 
 ROOT = Path(__file__).resolve().parent.parent
 BIN = ROOT / "blackhole_8_amd" / "bin"
@@ -109,7 +110,7 @@ def test_programs_are_built_and_print_usage(exe):
 
 @pytest.mark.parametrize("exe", [SOLUTION, TRACER])
 def test_bad_arguments_fail_with_status(exe):
-    for args in (("--frames", "0"), ("--frames", "x"), ("--format", "avi"),
+    for args in (("--frames", "0"), ("--frames", "x"), ("--format", "gif"),
                  ("--depth", "1", "--dry-run")):
         r = run(exe, *args, check=False)
         if "--dry-run" in args:
@@ -264,6 +265,23 @@ def test_tracer_frames_equal_library_renders(renderer, tmp_path):
         want = renderer.render_flat(s, ("rgb", "hit"))
         got = images.load(tmp_path / f"frame_{rec['frame']:05d}.ppm")
         assert np.array_equal(got, want["rgb"]), rec["frame"]
+
+
+@pytest.mark.gpu
+def test_avi_output_holds_the_library_renders(renderer, tmp_path):
+    # --format avi: one MJPG AVI whose frames are the JPEG encodings of the
+    # library's renders, in order
+    args = ["--frames", 3, "--width", 320, "--height", 180, "--keys", "Wd", "--fp32"]
+    print(f"[case] blackhole_solution avi {args}", flush=True)
+    run(SOLUTION, *args, "--out", tmp_path, "--format", "avi", "--quiet")
+    total, frames = riff.avi_frames((tmp_path / "video.avi").read_bytes())
+    recs = dry(SOLUTION, *args)
+    assert total == len(frames) == len(recs) == 3
+    for rec, jpg in zip(recs, frames):
+        s, o, tex = _solution_scene(rec, abi.PREC_FP32)
+        renderer.set_textures(tex)
+        want = renderer.render(s, o, ("rgb",))["rgb"]
+        assert jpg == images.encode_jpeg(want, 95), rec["frame"]
 
 
 @pytest.mark.gpu

@@ -1,6 +1,7 @@
 """Image files: the library's PNG and JPEG decoders (bh_png_decode,
-bh_jpeg_decode) with cv::imread IMREAD_COLOR semantics (config.h:35-37), and
-the PPM / PNG frame writer.
+bh_jpeg_decode) with cv::imread IMREAD_COLOR semantics (config.h:35-37), the
+PPM / PNG / JPEG frame writer and the Motion-JPEG AVI writer that stands in
+for cv::VideoWriter (blackhole_solution_test.cc:69-72,330).
 
 The JPEG decoder is pinned byte for byte against Pillow, whose JPEG decoder is
 libjpeg(-turbo), the library imread uses: on the reference's own
@@ -29,6 +30,7 @@ import numpy as np
 import pytest
 
 from blackhole_8_amd import abi, images
+from tests import riff
 
 ACC_DISC = Path("/root/reference/resource/acc_disc.png")
 ADAM7 = [(0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2),
@@ -244,7 +246,7 @@ def test_write_ppm_and_png_round_trip(tmp_path):
 
 def test_write_rejects_unknown_extension(tmp_path):
     with pytest.raises(images.ImageError) as e:
-        images.write(tmp_path / "x.jpg", np.zeros((2, 2, 3), np.uint8))
+        images.write(tmp_path / "x.bmp", np.zeros((2, 2, 3), np.uint8))
     assert e.value.code == abi.BH_ERR_UNSUPPORTED
 
 
@@ -277,6 +279,22 @@ def test_reference_jpegs_match_libjpeg(path):
     got = images.decode_jpeg(data)
     np.testing.assert_array_equal(got, _pillow_bgr(data))
     np.testing.assert_array_equal(images.load(path), got)
+
+
+def test_jpeg_narrow_images_match_libjpeg():
+    # libjpeg upsamples 2:1 chroma with the triangle filter only when the
+    # chroma plane is more than 2 samples wide, else by replication: every
+    # width 1..9 and height 1..9 over 4:4:4 / 4:2:2 / 4:2:0
+    Image = pytest.importorskip("PIL.Image")
+    for h in range(1, 10):
+        for w in range(1, 10):
+            arr = _smooth_image(h, w, 31 * h + w)
+            for sub in (0, 1, 2):
+                buf = io.BytesIO()
+                Image.fromarray(arr, "RGB").save(buf, "JPEG", quality=90, subsampling=sub)
+                data = buf.getvalue()
+                np.testing.assert_array_equal(images.decode_jpeg(data), _pillow_bgr(data),
+                                              err_msg=f"{h}x{w} subsampling {sub}")
 
 
 @pytest.mark.parametrize("subsampling", [0, 1, 2])  # 4:4:4, 4:2:2, 4:2:0
@@ -342,3 +360,120 @@ def test_jpeg_solid_colour_known_answer():
     got = images.decode_jpeg(buf.getvalue())
     assert (got == got[0, 0]).all()
     np.testing.assert_allclose(got[0, 0].astype(int), [90, 40, 200], atol=3)
+
+
+# ---------------------------------------------------------------------------
+# JPEG encoder (frames of the MJPG video) and the AVI container
+# ---------------------------------------------------------------------------
+def _psnr(a, b):
+    return 10 * np.log10(255.0 ** 2 / np.mean((a.astype(np.float64) - b) ** 2))
+
+
+@pytest.mark.parametrize("size", [(1, 1), (2, 3), (9, 17), (16, 16), (33, 2), (45, 61),
+                                  (270, 480)])
+@pytest.mark.parametrize("quality", [10, 75, 95, 100])
+def test_jpeg_encoder_decodes_under_libjpeg(size, quality):
+    # libjpeg reads every file the encoder writes to the same pixels as the
+    # library's own decoder, and the error is what libjpeg's own encoder
+    # gets at the same quality and 4:2:0 (within 0.5 dB)
+    Image = pytest.importorskip("PIL.Image")
+    h, w = size
+    arr = _smooth_image(h, w, h * 7 + w + quality)
+    bgr = arr[..., ::-1].copy()
+    data = images.encode_jpeg(bgr, quality)
+    assert data[:2] == b"\xff\xd8" and data[-2:] == b"\xff\xd9"
+    got = images.decode_jpeg(data)
+    np.testing.assert_array_equal(got, _pillow_bgr(data))
+    buf = io.BytesIO()
+    Image.fromarray(arr, "RGB").save(buf, "JPEG", quality=quality, subsampling=2)
+    theirs = _pillow_bgr(buf.getvalue())
+    if h * w >= 64:
+        assert _psnr(got, bgr) >= _psnr(theirs, bgr) - 0.5
+
+
+def test_jpeg_encoder_structure_and_known_answers():
+    # a flat image holds only DC terms: every pixel decodes to one colour,
+    # the input colour within the DC quantiser step
+    flat = np.zeros((48, 80, 3), np.uint8)
+    flat[...] = (90, 40, 200)
+    got = images.decode_jpeg(images.encode_jpeg(flat, 95))
+    assert (got == got[0, 0]).all()
+    np.testing.assert_allclose(got[0, 0].astype(int), [90, 40, 200], atol=3)
+    # one restart interval per MCU row: DRI = MCUs per row, RST0..7 cycling
+    arr = _smooth_image(100, 70, 5)[..., ::-1].copy()
+    data = images.encode_jpeg(arr, 80)
+    i = data.index(b"\xff\xdd")
+    assert struct.unpack(">HH", data[i + 2:i + 6]) == (4, (70 + 15) // 16)
+    sos = data.index(b"\xff\xda")
+    body = data[sos:]
+    rst = [body[k + 1] for k in range(len(body) - 1)
+           if body[k] == 0xFF and 0xD0 <= body[k + 1] <= 0xD7]
+    assert rst == [0xD0 + (k & 7) for k in range((100 + 15) // 16 - 1)]
+    # quality is clamped to 1..100; higher quality never loses to lower
+    p = [_psnr(images.decode_jpeg(images.encode_jpeg(arr, q)), arr) for q in (1, 30, 70, 100)]
+    assert p == sorted(p)
+    with pytest.raises(ValueError):
+        images.encode_jpeg(np.zeros((4, 4), np.uint8))
+
+
+def test_write_jpeg(tmp_path):
+    bgr = _smooth_image(37, 53, 4)
+    for ext in ("jpg", "jpeg"):
+        p = tmp_path / f"frame.{ext}"
+        images.write(p, bgr)
+        assert p.read_bytes() == images.encode_jpeg(bgr, 95)  # imwrite's default quality
+        np.testing.assert_array_equal(images.load(p), images.decode_jpeg(p.read_bytes()))
+
+
+@pytest.mark.parametrize("fps,scale,rate", [(29, 1, 29), (29.97, 1000, 29970)])
+def test_mjpg_avi_layout(tmp_path, fps, scale, rate):
+    h, w, n = 36, 50, 5
+    frames = [_smooth_image(h, w, 100 + k)[..., ::-1].copy() for k in range(n)]
+    path = tmp_path / "video.avi"
+    with images.Video(path, h, w, fps=fps) as v:
+        for f in frames:
+            v.write(f)
+    data = path.read_bytes()
+    (top,) = riff.parse(data)
+    assert top[0] == b"RIFF:AVI "
+    assert struct.unpack("<I", data[4:8])[0] == len(data) - 8
+    hdrl, movi, idx1 = top[1]
+    assert hdrl[0] == b"LIST:hdrl" and movi[0] == b"LIST:movi" and idx1[0] == b"idx1"
+    avih, strl = hdrl[1]
+    us, _, _, flags, total, _, streams, _, aw, ah = struct.unpack("<10I", avih[1][:40])
+    assert (total, streams, aw, ah) == (n, 1, w, h) and flags & 0x10
+    assert us == round(1e6 / fps)
+    strh, strf = strl[1]
+    assert strh[1][:8] == b"vidsMJPG"
+    assert struct.unpack("<II", strh[1][20:28]) == (scale, rate)
+    assert struct.unpack("<I", strh[1][32:36])[0] == n
+    assert struct.unpack("<4h", strh[1][48:56]) == (0, 0, w, h)
+    bi = struct.unpack("<IiiHH4s", strf[1][:20])
+    assert bi == (40, w, h, 1, 24, b"MJPG")
+    chunks = movi[1]
+    assert [c[0] for c in chunks] == [b"00dc"] * n
+    movi_tag = movi[2] + 8  # offset of the 'movi' fourcc
+    entries = [struct.unpack("<4sIII", idx1[1][16 * k:16 * k + 16]) for k in range(n)]
+    for (tag, payload, off), (etag, eflags, eoff, esize), f in zip(chunks, entries, frames):
+        assert (etag, eflags, eoff, esize) == (b"00dc", 0x10, off - movi_tag, len(payload))
+        assert payload == images.encode_jpeg(f, 95)
+        np.testing.assert_array_equal(_pillow_bgr(payload), images.decode_jpeg(payload))
+
+
+def test_video_errors(tmp_path):
+    with pytest.raises(images.ImageError):
+        images.Video(tmp_path / "a.avi", 10, 10, fps=0)
+    with pytest.raises(images.ImageError):
+        images.Video(tmp_path / "a.avi", 10, 10, quality=0)
+    with pytest.raises(images.ImageError):
+        images.Video(tmp_path / "no_such_dir" / "a.avi", 10, 10)
+    v = images.Video(tmp_path / "b.avi", 10, 12)
+    with pytest.raises(ValueError):
+        v.write(np.zeros((12, 10, 3), np.uint8))
+    v.close()
+    v.close()  # closing twice is harmless
+    with pytest.raises(ValueError):
+        v.write(np.zeros((10, 12, 3), np.uint8))
+    # an empty video is still a well-formed file
+    (top,) = riff.parse((tmp_path / "b.avi").read_bytes())
+    assert [c[0] for c in top[1]] == [b"LIST:hdrl", b"LIST:movi", b"idx1"]
