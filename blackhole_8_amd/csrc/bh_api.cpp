@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <condition_variable>
 #include <mutex>
 #include <string>
@@ -74,6 +76,30 @@ int fail(bh_ctx* ctx, int code, const std::string& msg) {
   else
     g_tls_err = msg;
   return code;
+}
+
+// Records an error without letting a second exception escape.
+int fail_nothrow(bh_ctx* ctx, int code, const char* msg) noexcept {
+  try {
+    fail(ctx, code, msg);
+  } catch (...) {
+  }
+  return code;
+}
+
+// Body of a C entry point: no exception (host allocation, thread or mutex
+// failure) crosses the C boundary.
+template <class F>
+int guarded(bh_ctx* ctx, F&& body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail_nothrow(ctx, BH_ERR_OUT_OF_MEMORY, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail_nothrow(ctx, BH_ERR_INVALID_ARG, e.what());
+  } catch (...) {
+    return fail_nothrow(ctx, BH_ERR_INVALID_ARG, "unknown host error");
+  }
 }
 
 int hip_fail(bh_ctx* ctx, hipError_t e, const char* what) {
@@ -608,41 +634,43 @@ hipError_t path_band(const bh_path* p, hipStream_t st) {
 extern "C" {
 
 int bh_ctx_create(int device, bh_ctx** out) {
-  if (!out) return fail(nullptr, BH_ERR_INVALID_ARG, "out is NULL");
-  *out = nullptr;
-  int n = 0;
-  hipError_t e = hipGetDeviceCount(&n);
-  if (e != hipSuccess || n <= 0) return fail(nullptr, BH_ERR_NO_DEVICE, "no HIP device available");
-  if (device < 0 || device >= n) return fail(nullptr, BH_ERR_NO_DEVICE, "device ordinal out of range");
-  e = hipSetDevice(device);
-  if (e != hipSuccess) return fail(nullptr, BH_ERR_HIP, "hipSetDevice failed");
-  bh_ctx* c = new bh_ctx();
-  c->device = device;
-  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    delete c;
-    return fail(nullptr, BH_ERR_HIP, "hipStreamCreate failed");
-  }
-  c->own_stream = true;
-  e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    (void)hipStreamDestroy(c->stream);
-    delete c;
-    return fail(nullptr, BH_ERR_HIP, "hipStreamCreate failed");
-  }
-  (void)hipEventCreate(&c->ev0);
-  (void)hipEventCreate(&c->ev1);
-  (void)hipEventCreate(&c->ev2);
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
-    std::snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
-    c->n_cu = prop.multiProcessorCount;
-    c->clock_khz = prop.clockRate;
-  }
-  if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
-    c->wall_khz = 0;
-  *out = c;
-  return BH_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!out) return fail(nullptr, BH_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(nullptr, BH_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(nullptr, BH_ERR_NO_DEVICE, "device ordinal out of range");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, BH_ERR_HIP, "hipSetDevice failed");
+    bh_ctx* c = new bh_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(nullptr, BH_ERR_HIP, "hipStreamCreate failed");
+    }
+    c->own_stream = true;
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      return fail(nullptr, BH_ERR_HIP, "hipStreamCreate failed");
+    }
+    (void)hipEventCreate(&c->ev0);
+    (void)hipEventCreate(&c->ev1);
+    (void)hipEventCreate(&c->ev2);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+      std::snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
+      c->n_cu = prop.multiProcessorCount;
+      c->clock_khz = prop.clockRate;
+    }
+    if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
+      c->wall_khz = 0;
+    *out = c;
+    return BH_OK;
+  });
 }
 
 void bh_ctx_destroy(bh_ctx* ctx) {
@@ -670,186 +698,204 @@ const char* bh_last_error(const bh_ctx* ctx) {
 }
 
 int bh_device_info(bh_ctx* ctx, char* name, int name_len, int* n_cu, int* clock_khz) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", ctx->name);
-  if (n_cu) *n_cu = ctx->n_cu;
-  if (clock_khz) *clock_khz = ctx->clock_khz;
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", ctx->name);
+    if (n_cu) *n_cu = ctx->n_cu;
+    if (clock_khz) *clock_khz = ctx->clock_khz;
+    return BH_OK;
+  });
 }
 
 int bh_ctx_set_stream(bh_ctx* ctx, void* stream) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  (void)hipSetDevice(ctx->device);
-  if (ctx->own_stream && ctx->stream) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-  }
-  if (stream) {
-    ctx->stream = (hipStream_t)stream;
-    ctx->own_stream = false;
-  } else {
-    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamCreate");
-    ctx->own_stream = true;
-  }
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream && ctx->stream) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipStreamDestroy(ctx->stream);
+    }
+    if (stream) {
+      ctx->stream = (hipStream_t)stream;
+      ctx->own_stream = false;
+    } else {
+      hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamCreate");
+      ctx->own_stream = true;
+    }
+    return BH_OK;
+  });
 }
 
 int bh_synchronize(bh_ctx* ctx) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BH_OK;
+  });
 }
 
 int bh_texture_set(bh_ctx* ctx, int slot, const bh_image* img) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (slot < 0 || slot >= BH_MAX_TEXTURES) return fail(ctx, BH_ERR_INVALID_ARG, "texture slot out of range");
-  if (!img || !img->data || img->rows <= 0 || img->cols <= 0)
-    return fail(ctx, BH_ERR_EMPTY_TEXTURE, "empty texture (Material::SetTexture)");
-  if (img->channels != 3 && img->channels != 4)
-    return fail(ctx, BH_ERR_INVALID_ARG, "texture channels must be 3 or 4");
-  if ((int64_t)img->rows * img->cols > ((int64_t)1 << 28))
-    return fail(ctx, BH_ERR_INVALID_ARG, "texture too large");
-  (void)hipSetDevice(ctx->device);
-  const size_t n = (size_t)img->rows * img->cols;
-  std::vector<uint32_t> packed(n);
-  const int C = img->channels;
-  for (size_t i = 0; i < n; ++i) {
-    const uint8_t* p = img->data + i * C;
-    packed[i] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
-                ((C == 4 ? (uint32_t)p[3] : 255u) << 24);
-  }
-  DeviceTexture& t = ctx->tex[slot];
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (t.d && (size_t)t.rows * t.cols != n) {
-    (void)hipFree(t.d);
-    t.d = nullptr;
-  }
-  if (!t.d) HIP_TRY(ctx, hipMalloc(&t.d, n * sizeof(uint32_t)));
-  HIP_TRY(ctx, hipMemcpy(t.d, packed.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-  t.rows = img->rows;
-  t.cols = img->cols;
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (slot < 0 || slot >= BH_MAX_TEXTURES) return fail(ctx, BH_ERR_INVALID_ARG, "texture slot out of range");
+    if (!img || !img->data || img->rows <= 0 || img->cols <= 0)
+      return fail(ctx, BH_ERR_EMPTY_TEXTURE, "empty texture (Material::SetTexture)");
+    if (img->channels != 3 && img->channels != 4)
+      return fail(ctx, BH_ERR_INVALID_ARG, "texture channels must be 3 or 4");
+    if ((int64_t)img->rows * img->cols > ((int64_t)1 << 28))
+      return fail(ctx, BH_ERR_INVALID_ARG, "texture too large");
+    (void)hipSetDevice(ctx->device);
+    const size_t n = (size_t)img->rows * img->cols;
+    std::vector<uint32_t> packed(n);
+    const int C = img->channels;
+    for (size_t i = 0; i < n; ++i) {
+      const uint8_t* p = img->data + i * C;
+      packed[i] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+                  ((C == 4 ? (uint32_t)p[3] : 255u) << 24);
+    }
+    DeviceTexture& t = ctx->tex[slot];
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (t.d && (size_t)t.rows * t.cols != n) {
+      (void)hipFree(t.d);
+      t.d = nullptr;
+    }
+    if (!t.d) HIP_TRY(ctx, hipMalloc(&t.d, n * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMemcpy(t.d, packed.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    t.rows = img->rows;
+    t.cols = img->cols;
+    return BH_OK;
+  });
 }
 
 int bh_texture_clear(bh_ctx* ctx, int slot) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (slot < 0 || slot >= BH_MAX_TEXTURES) return fail(ctx, BH_ERR_INVALID_ARG, "texture slot out of range");
-  DeviceTexture& t = ctx->tex[slot];
-  if (t.d) {
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipFree(t.d);
-  }
-  t = DeviceTexture{};
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (slot < 0 || slot >= BH_MAX_TEXTURES) return fail(ctx, BH_ERR_INVALID_ARG, "texture slot out of range");
+    DeviceTexture& t = ctx->tex[slot];
+    if (t.d) {
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      (void)hipFree(t.d);
+    }
+    t = DeviceTexture{};
+    return BH_OK;
+  });
 }
 
 int bh_scene_check(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  DevScene d;
-  return build_scene(ctx, scene, opt, &d);
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    DevScene d;
+    return build_scene(ctx, scene, opt, &d);
+  });
 }
 
 int bh_render_device(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                      const bh_outputs* out_device) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!out_device) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
-  DevScene d;
-  int rc = build_scene(ctx, scene, opt, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
-  uint32_t *qi, *qc;
-  const size_t npix = (size_t)d.W * d.H;
-  if ((rc = queue_buffers(ctx, &qi, &qc, npix))) return rc;
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
-  DevOut o{out_device->rgb, out_device->hit, out_device->phi_end, out_device->hit_point,
-           out_device->steps};
-  DevQueue q{qi, qc, (uint32_t)npix};
-  return launch_frame(ctx, d, (DevScene*)ctx->scene.p, o, q, ctx->stream);
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!out_device) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
+    DevScene d;
+    int rc = build_scene(ctx, scene, opt, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
+    uint32_t *qi, *qc;
+    const size_t npix = (size_t)d.W * d.H;
+    if ((rc = queue_buffers(ctx, &qi, &qc, npix))) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
+    DevOut o{out_device->rgb, out_device->hit, out_device->phi_end, out_device->hit_point,
+             out_device->steps};
+    DevQueue q{qi, qc, (uint32_t)npix};
+    return launch_frame(ctx, d, (DevScene*)ctx->scene.p, o, q, ctx->stream);
+  });
 }
 
 int bh_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, const bh_outputs* out) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
-  DevScene d;
-  int rc = build_scene(ctx, scene, opt, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  const size_t npix = (size_t)d.W * d.H;
-  DevOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
-  if (out->rgb) { if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc; o.rgb = (uint8_t*)ctx->rgb.p; }
-  if (out->hit) { if ((rc = ensure(ctx, ctx->hit, npix))) return rc; o.hit = (uint8_t*)ctx->hit.p; }
-  if (out->phi_end) { if ((rc = ensure(ctx, ctx->phi, npix * 8))) return rc; o.phi = (double*)ctx->phi.p; }
-  if (out->hit_point) { if ((rc = ensure(ctx, ctx->hitpt, npix * 24))) return rc; o.hitpt = (double*)ctx->hitpt.p; }
-  if (out->steps) { if ((rc = ensure(ctx, ctx->steps, npix * 4))) return rc; o.steps = (int32_t*)ctx->steps.p; }
-  bh_outputs dev{o.rgb, o.hit, o.phi, o.hitpt, o.steps};
-  if ((rc = bh_render_device(ctx, scene, opt, &dev))) return rc;
-  hipStream_t st = ctx->stream;
-  if (out->rgb) HIP_TRY(ctx, hipMemcpyAsync(out->rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, st));
-  if (out->hit) HIP_TRY(ctx, hipMemcpyAsync(out->hit, o.hit, npix, hipMemcpyDeviceToHost, st));
-  if (out->phi_end) HIP_TRY(ctx, hipMemcpyAsync(out->phi_end, o.phi, npix * 8, hipMemcpyDeviceToHost, st));
-  if (out->hit_point) HIP_TRY(ctx, hipMemcpyAsync(out->hit_point, o.hitpt, npix * 24, hipMemcpyDeviceToHost, st));
-  if (out->steps) HIP_TRY(ctx, hipMemcpyAsync(out->steps, o.steps, npix * 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
+    DevScene d;
+    int rc = build_scene(ctx, scene, opt, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)d.W * d.H;
+    DevOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (out->rgb) { if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc; o.rgb = (uint8_t*)ctx->rgb.p; }
+    if (out->hit) { if ((rc = ensure(ctx, ctx->hit, npix))) return rc; o.hit = (uint8_t*)ctx->hit.p; }
+    if (out->phi_end) { if ((rc = ensure(ctx, ctx->phi, npix * 8))) return rc; o.phi = (double*)ctx->phi.p; }
+    if (out->hit_point) { if ((rc = ensure(ctx, ctx->hitpt, npix * 24))) return rc; o.hitpt = (double*)ctx->hitpt.p; }
+    if (out->steps) { if ((rc = ensure(ctx, ctx->steps, npix * 4))) return rc; o.steps = (int32_t*)ctx->steps.p; }
+    bh_outputs dev{o.rgb, o.hit, o.phi, o.hitpt, o.steps};
+    if ((rc = bh_render_device(ctx, scene, opt, &dev))) return rc;
+    hipStream_t st = ctx->stream;
+    if (out->rgb) HIP_TRY(ctx, hipMemcpyAsync(out->rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, st));
+    if (out->hit) HIP_TRY(ctx, hipMemcpyAsync(out->hit, o.hit, npix, hipMemcpyDeviceToHost, st));
+    if (out->phi_end) HIP_TRY(ctx, hipMemcpyAsync(out->phi_end, o.phi, npix * 8, hipMemcpyDeviceToHost, st));
+    if (out->hit_point) HIP_TRY(ctx, hipMemcpyAsync(out->hit_point, o.hitpt, npix * 24, hipMemcpyDeviceToHost, st));
+    if (out->steps) HIP_TRY(ctx, hipMemcpyAsync(out->steps, o.steps, npix * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    return BH_OK;
+  });
 }
 
 int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, int iters,
                    double* avg_ms_main, double* avg_ms_total, int64_t* band_rays) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
-  DevScene d;
-  int rc = build_scene(ctx, scene, opt, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  const size_t npix = (size_t)d.W * d.H;
-  if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc;
-  if ((rc = ensure(ctx, ctx->hit, npix))) return rc;
-  if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
-  uint32_t *qi, *qc;
-  if ((rc = queue_buffers(ctx, &qi, &qc, npix))) return rc;
-  hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, st));
-  DevOut o{(uint8_t*)ctx->rgb.p, (uint8_t*)ctx->hit.p, nullptr, nullptr, nullptr};
-  DevScene* ds = (DevScene*)ctx->scene.p;
-  DevQueue q{qi, qc, (uint32_t)npix};
-  const bool fast = fp32_eligible(d);
-  // warm-up
-  if ((rc = launch_frame(ctx, d, ds, o, q, st))) return rc;
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  // main kernel: iters back-to-back launches between one event pair (the
-  // queue is cleared afterwards); whole frame: iters march + band launches
-  float main_ms = 0, total_ms = 0;
-  uint32_t nband = 0;
-  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-  for (int it = 0; it < iters; ++it) {
-    if (fast) {
-      HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0, d.n_spheres > 0));
-    } else {
-      HIP_TRY(ctx, bh::launch_march_fp64(ds, o, d.W, d.H, st));
-    }
-  }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-  if (fast) {
-    HIP_TRY(ctx, hipMemcpyAsync(&nband, qc, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, bh::launch_zero_u32(qc, 2, st));
-  }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
-  for (int it = 0; it < iters; ++it)
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
+    DevScene d;
+    int rc = build_scene(ctx, scene, opt, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)d.W * d.H;
+    if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc;
+    if ((rc = ensure(ctx, ctx->hit, npix))) return rc;
+    if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
+    uint32_t *qi, *qc;
+    if ((rc = queue_buffers(ctx, &qi, &qc, npix))) return rc;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, st));
+    DevOut o{(uint8_t*)ctx->rgb.p, (uint8_t*)ctx->hit.p, nullptr, nullptr, nullptr};
+    DevScene* ds = (DevScene*)ctx->scene.p;
+    DevQueue q{qi, qc, (uint32_t)npix};
+    const bool fast = fp32_eligible(d);
+    // warm-up
     if ((rc = launch_frame(ctx, d, ds, o, q, st))) return rc;
-  hipEvent_t end;
-  HIP_TRY(ctx, hipEventCreate(&end));
-  HIP_TRY(ctx, hipEventRecord(end, st));
-  HIP_TRY(ctx, hipEventSynchronize(end));
-  HIP_TRY(ctx, hipEventElapsedTime(&main_ms, ctx->ev0, ctx->ev1));
-  HIP_TRY(ctx, hipEventElapsedTime(&total_ms, ctx->ev2, end));
-  (void)hipEventDestroy(end);
-  if (fast) nband /= (uint32_t)iters;
-  if (avg_ms_main) *avg_ms_main = main_ms / iters;
-  if (avg_ms_total) *avg_ms_total = total_ms / iters;
-  if (band_rays) *band_rays = nband;
-  return BH_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    // main kernel: iters back-to-back launches between one event pair (the
+    // queue is cleared afterwards); whole frame: iters march + band launches
+    float main_ms = 0, total_ms = 0;
+    uint32_t nband = 0;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    for (int it = 0; it < iters; ++it) {
+      if (fast) {
+        HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0, d.n_spheres > 0));
+      } else {
+        HIP_TRY(ctx, bh::launch_march_fp64(ds, o, d.W, d.H, st));
+      }
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    if (fast) {
+      HIP_TRY(ctx, hipMemcpyAsync(&nband, qc, 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(ctx, bh::launch_zero_u32(qc, 2, st));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+    for (int it = 0; it < iters; ++it)
+      if ((rc = launch_frame(ctx, d, ds, o, q, st))) return rc;
+    hipEvent_t end;
+    HIP_TRY(ctx, hipEventCreate(&end));
+    HIP_TRY(ctx, hipEventRecord(end, st));
+    HIP_TRY(ctx, hipEventSynchronize(end));
+    HIP_TRY(ctx, hipEventElapsedTime(&main_ms, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&total_ms, ctx->ev2, end));
+    (void)hipEventDestroy(end);
+    if (fast) nband /= (uint32_t)iters;
+    if (avg_ms_main) *avg_ms_main = main_ms / iters;
+    if (avg_ms_total) *avg_ms_total = total_ms / iters;
+    if (band_rays) *band_rays = nband;
+    return BH_OK;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -857,147 +903,155 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
 // ---------------------------------------------------------------------------
 int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                    const bh_camera* cameras, int n_frames, bh_path** out) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!out || !scene || !cameras || n_frames <= 0)
-    return fail(ctx, BH_ERR_INVALID_ARG, "path needs scene, cameras and n_frames > 0");
-  *out = nullptr;
-  (void)hipSetDevice(ctx->device);
-  std::vector<DevScene> frames((size_t)n_frames);
-  for (int i = 0; i < n_frames; ++i) {
-    bh_scene s = *scene;
-    s.camera = cameras[i];
-    if (cameras[i].width != cameras[0].width || cameras[i].height != cameras[0].height)
-      return fail(ctx, BH_ERR_INVALID_ARG, "path cameras must share width/height");
-    int rc = build_scene(ctx, &s, opt, &frames[(size_t)i]);
-    if (rc) return rc;
-  }
-  bh_path* p = new bh_path();
-  p->ctx = ctx;
-  p->n_frames = n_frames;
-  p->W = frames[0].W;
-  p->H = frames[0].H;
-  const size_t npix = (size_t)p->W * p->H;
-  auto cleanup = [&](hipError_t e, const char* what) {
-    bh_path_destroy(p);
-    return hip_fail(ctx, e, what);
-  };
-  hipError_t e;
-  if ((e = hipMalloc(&p->d_scenes, sizeof(DevScene) * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc scenes");
-  if ((e = hipMalloc(&p->d_rgb, npix * 3 * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc rgb");
-  if ((e = hipMalloc(&p->d_hit, npix * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc hit");
-  // one queue slice and count pair per frame, so that the FP64 band kernel of
-  // frame i (side stream) overlaps the FP32 march of frame i+1
-  const size_t qcap = npix / 16 > 1024 ? npix / 16 : 1024;
-  p->qcap = (uint32_t)qcap;
-  p->fused = true;
-  for (const DevScene& f : frames) p->fused = p->fused && fp32_eligible(f);
-  p->esc = frames[0].escape != 0;
-  p->sph = frames[0].n_spheres > 0;
-  if ((e = hipMalloc(&p->d_qitems, qcap * n_frames * sizeof(uint32_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
-  if ((e = hipMalloc(&p->d_qcount, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMalloc queue count");
-  if ((e = hipMemset(p->d_qcount, 0, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMemset queue count");
-  if ((e = hipMemcpy(p->d_scenes, frames.data(), sizeof(DevScene) * n_frames, hipMemcpyHostToDevice)) != hipSuccess)
-    return cleanup(e, "hipMemcpy scenes");
-  p->events.resize((size_t)n_frames + 1, nullptr);
-  for (auto& ev : p->events)
-    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return cleanup(e, "hipEventCreate");
-  hipStream_t st = ctx->stream, side = ctx->side;
-  if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return cleanup(e, "capture");
-  int rc = BH_OK;
-  bool used_side = false;
-  if (p->fused) {
-    hipError_t le = path_march(p, st);
-    if (le == hipSuccess) le = path_band(p, st);
-    if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
-  }
-  for (int i = 0; i < n_frames && rc == BH_OK && !p->fused; ++i) {
-    DevOut o{p->d_rgb + npix * 3 * i, p->d_hit + npix * i, nullptr, nullptr, nullptr};
-    DevQueue q{p->d_qitems + qcap * i, p->d_qcount + 16 * i, (uint32_t)qcap};
-    const DevScene& f = frames[(size_t)i];
-    if (fp32_eligible(f)) {
-      const bool esc = f.escape != 0;
-      const bool sph = f.n_spheres > 0;
-      hipError_t le = bh::launch_march_fp32(p->d_scenes + i, o, p->W, p->H, q, st, esc, sph);
-      if (le == hipSuccess) le = hipEventRecord(p->events[(size_t)i], st);
-      if (le == hipSuccess) le = hipStreamWaitEvent(side, p->events[(size_t)i], 0);
-      if (le == hipSuccess) le = bh::launch_band(p->d_scenes + i, o, q, side, esc, sph);
-      if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
-      used_side = true;
-    } else {
-      hipError_t le = bh::launch_march_fp64(p->d_scenes + i, o, p->W, p->H, st);
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!out || !scene || !cameras || n_frames <= 0)
+      return fail(ctx, BH_ERR_INVALID_ARG, "path needs scene, cameras and n_frames > 0");
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    std::vector<DevScene> frames((size_t)n_frames);
+    for (int i = 0; i < n_frames; ++i) {
+      bh_scene s = *scene;
+      s.camera = cameras[i];
+      if (cameras[i].width != cameras[0].width || cameras[i].height != cameras[0].height)
+        return fail(ctx, BH_ERR_INVALID_ARG, "path cameras must share width/height");
+      int rc = build_scene(ctx, &s, opt, &frames[(size_t)i]);
+      if (rc) return rc;
+    }
+    bh_path* p = new bh_path();
+    p->ctx = ctx;
+    p->n_frames = n_frames;
+    p->W = frames[0].W;
+    p->H = frames[0].H;
+    const size_t npix = (size_t)p->W * p->H;
+    auto cleanup = [&](hipError_t e, const char* what) {
+      bh_path_destroy(p);
+      return hip_fail(ctx, e, what);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&p->d_scenes, sizeof(DevScene) * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc scenes");
+    if ((e = hipMalloc(&p->d_rgb, npix * 3 * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc rgb");
+    if ((e = hipMalloc(&p->d_hit, npix * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc hit");
+    // one queue slice and count pair per frame, so that the FP64 band kernel of
+    // frame i (side stream) overlaps the FP32 march of frame i+1
+    const size_t qcap = npix / 16 > 1024 ? npix / 16 : 1024;
+    p->qcap = (uint32_t)qcap;
+    p->fused = true;
+    for (const DevScene& f : frames) p->fused = p->fused && fp32_eligible(f);
+    p->esc = frames[0].escape != 0;
+    p->sph = frames[0].n_spheres > 0;
+    if ((e = hipMalloc(&p->d_qitems, qcap * n_frames * sizeof(uint32_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
+    if ((e = hipMalloc(&p->d_qcount, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMalloc queue count");
+    if ((e = hipMemset(p->d_qcount, 0, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMemset queue count");
+    if ((e = hipMemcpy(p->d_scenes, frames.data(), sizeof(DevScene) * n_frames, hipMemcpyHostToDevice)) != hipSuccess)
+      return cleanup(e, "hipMemcpy scenes");
+    p->events.resize((size_t)n_frames + 1, nullptr);
+    for (auto& ev : p->events)
+      if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return cleanup(e, "hipEventCreate");
+    hipStream_t st = ctx->stream, side = ctx->side;
+    if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return cleanup(e, "capture");
+    int rc = BH_OK;
+    bool used_side = false;
+    if (p->fused) {
+      hipError_t le = path_march(p, st);
+      if (le == hipSuccess) le = path_band(p, st);
       if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
     }
-  }
-  if (rc == BH_OK && used_side) {
-    // join the side stream back into the captured stream
-    hipError_t le = hipEventRecord(p->events[(size_t)n_frames], side);
-    if (le == hipSuccess) le = hipStreamWaitEvent(st, p->events[(size_t)n_frames], 0);
-    if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture join");
-  }
-  if (rc != BH_OK) {
-    hipGraph_t g = nullptr;
-    (void)hipStreamEndCapture(st, &g);
-    if (g) (void)hipGraphDestroy(g);
-    std::string m = ctx->err;
-    bh_path_destroy(p);
-    return fail(ctx, rc, m);
-  }
-  if ((e = hipStreamEndCapture(st, &p->graph)) != hipSuccess) return cleanup(e, "end capture");
-  if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess)
-    return cleanup(e, "hipGraphInstantiate");
-  *out = p;
-  return BH_OK;
+    for (int i = 0; i < n_frames && rc == BH_OK && !p->fused; ++i) {
+      DevOut o{p->d_rgb + npix * 3 * i, p->d_hit + npix * i, nullptr, nullptr, nullptr};
+      DevQueue q{p->d_qitems + qcap * i, p->d_qcount + 16 * i, (uint32_t)qcap};
+      const DevScene& f = frames[(size_t)i];
+      if (fp32_eligible(f)) {
+        const bool esc = f.escape != 0;
+        const bool sph = f.n_spheres > 0;
+        hipError_t le = bh::launch_march_fp32(p->d_scenes + i, o, p->W, p->H, q, st, esc, sph);
+        if (le == hipSuccess) le = hipEventRecord(p->events[(size_t)i], st);
+        if (le == hipSuccess) le = hipStreamWaitEvent(side, p->events[(size_t)i], 0);
+        if (le == hipSuccess) le = bh::launch_band(p->d_scenes + i, o, q, side, esc, sph);
+        if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
+        used_side = true;
+      } else {
+        hipError_t le = bh::launch_march_fp64(p->d_scenes + i, o, p->W, p->H, st);
+        if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
+      }
+    }
+    if (rc == BH_OK && used_side) {
+      // join the side stream back into the captured stream
+      hipError_t le = hipEventRecord(p->events[(size_t)n_frames], side);
+      if (le == hipSuccess) le = hipStreamWaitEvent(st, p->events[(size_t)n_frames], 0);
+      if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture join");
+    }
+    if (rc != BH_OK) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      std::string m = ctx->err;
+      bh_path_destroy(p);
+      return fail(ctx, rc, m);
+    }
+    if ((e = hipStreamEndCapture(st, &p->graph)) != hipSuccess) return cleanup(e, "end capture");
+    if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess)
+      return cleanup(e, "hipGraphInstantiate");
+    *out = p;
+    return BH_OK;
+  });
 }
 
 int bh_path_launch(bh_path* p, float* ms) {
-  if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
-  bh_ctx* ctx = p->ctx;
-  (void)hipSetDevice(ctx->device);
-  hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-  HIP_TRY(ctx, hipGraphLaunch(p->exec, st));
-  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
-  if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
-  return BH_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
+    bh_ctx* ctx = p->ctx;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    HIP_TRY(ctx, hipGraphLaunch(p->exec, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return BH_OK;
+  });
 }
 
 int bh_path_time(bh_path* p, int iters, double* march_ms, double* band_ms) {
-  if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
-  bh_ctx* ctx = p->ctx;
-  if (!p->fused) return fail(ctx, BH_ERR_UNSUPPORTED, "path frames are not all in the FP32 march");
-  if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
-  (void)hipSetDevice(ctx->device);
-  hipStream_t st = ctx->stream;
-  double tm = 0, tb = 0;
-  for (int it = 0; it < iters; ++it) {
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-    HIP_TRY(ctx, path_march(p, st));
-    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-    HIP_TRY(ctx, path_band(p, st));
-    HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
-    HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
-    float a = 0, b = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->ev0, ctx->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(&b, ctx->ev1, ctx->ev2));
-    tm += a;
-    tb += b;
-  }
-  if (march_ms) *march_ms = tm / iters;
-  if (band_ms) *band_ms = tb / iters;
-  return BH_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
+    bh_ctx* ctx = p->ctx;
+    if (!p->fused) return fail(ctx, BH_ERR_UNSUPPORTED, "path frames are not all in the FP32 march");
+    if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    double tm = 0, tb = 0;
+    for (int it = 0; it < iters; ++it) {
+      HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+      HIP_TRY(ctx, path_march(p, st));
+      HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+      HIP_TRY(ctx, path_band(p, st));
+      HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+      HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+      float a = 0, b = 0;
+      HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->ev0, ctx->ev1));
+      HIP_TRY(ctx, hipEventElapsedTime(&b, ctx->ev1, ctx->ev2));
+      tm += a;
+      tb += b;
+    }
+    if (march_ms) *march_ms = tm / iters;
+    if (band_ms) *band_ms = tb / iters;
+    return BH_OK;
+  });
 }
 
 int bh_path_read(bh_path* p, int frame, uint8_t* rgb, uint8_t* hit) {
-  if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
-  bh_ctx* ctx = p->ctx;
-  if (frame < 0 || frame >= p->n_frames) return fail(ctx, BH_ERR_INVALID_ARG, "frame out of range");
-  (void)hipSetDevice(ctx->device);
-  const size_t npix = (size_t)p->W * p->H;
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (rgb) HIP_TRY(ctx, hipMemcpy(rgb, p->d_rgb + npix * 3 * frame, npix * 3, hipMemcpyDeviceToHost));
-  if (hit) HIP_TRY(ctx, hipMemcpy(hit, p->d_hit + npix * frame, npix, hipMemcpyDeviceToHost));
-  return BH_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
+    bh_ctx* ctx = p->ctx;
+    if (frame < 0 || frame >= p->n_frames) return fail(ctx, BH_ERR_INVALID_ARG, "frame out of range");
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)p->W * p->H;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (rgb) HIP_TRY(ctx, hipMemcpy(rgb, p->d_rgb + npix * 3 * frame, npix * 3, hipMemcpyDeviceToHost));
+    if (hit) HIP_TRY(ctx, hipMemcpy(hit, p->d_hit + npix * frame, npix, hipMemcpyDeviceToHost));
+    return BH_OK;
+  });
 }
 
 void bh_path_destroy(bh_path* p) {
@@ -1021,159 +1075,169 @@ void bh_path_destroy(bh_path* p) {
 /* ---- differentiable render -------------------------------------------------- */
 int bh_diff_render_device(bh_ctx* ctx, const bh_diff_params* p, int precision, float* rgb_out,
                           uint8_t* hit_out) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  bh::DiffScene d;
-  int rc = build_diff(ctx, p, precision, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  HIP_TRY(ctx, bh::launch_diff_forward(d, precision == BH_PREC_FP64, rgb_out, hit_out, ctx->stream));
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    bh::DiffScene d;
+    int rc = build_diff(ctx, p, precision, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, bh::launch_diff_forward(d, precision == BH_PREC_FP64, rgb_out, hit_out, ctx->stream));
+    return BH_OK;
+  });
 }
 
 int bh_diff_adjoint_device(bh_ctx* ctx, const bh_diff_params* p, int precision,
                            const float* grad_rgb, float* rgb_out, uint8_t* hit_out,
                            double* grad_out, uint32_t* overflow_out) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!grad_rgb || !grad_out) return fail(ctx, BH_ERR_INVALID_ARG, "grad_rgb and grad_out are required");
-  bh::DiffScene d;
-  int rc = build_diff(ctx, p, precision, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  const size_t nb = (size_t)bh::diff_partial_blocks(d.W, d.H);
-  if ((rc = ensure(ctx, ctx->dpart, nb * 5 * sizeof(double)))) return rc;
-  HIP_TRY(ctx, bh::launch_diff_adjoint(d, precision == BH_PREC_FP64, grad_rgb, rgb_out, hit_out,
-                                       (double*)ctx->dpart.p, grad_out, overflow_out, ctx->stream));
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!grad_rgb || !grad_out) return fail(ctx, BH_ERR_INVALID_ARG, "grad_rgb and grad_out are required");
+    bh::DiffScene d;
+    int rc = build_diff(ctx, p, precision, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t nb = (size_t)bh::diff_partial_blocks(d.W, d.H);
+    if ((rc = ensure(ctx, ctx->dpart, nb * 5 * sizeof(double)))) return rc;
+    HIP_TRY(ctx, bh::launch_diff_adjoint(d, precision == BH_PREC_FP64, grad_rgb, rgb_out, hit_out,
+                                         (double*)ctx->dpart.p, grad_out, overflow_out, ctx->stream));
+    return BH_OK;
+  });
 }
 
 int bh_diff_time(bh_ctx* ctx, const bh_diff_params* p, int precision, int iters, double* fwd_ms,
                  double* fwd_adj_ms) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
-  bh::DiffScene d;
-  int rc = build_diff(ctx, p, precision, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  const size_t npix = (size_t)d.W * d.H;
-  const size_t nb = (size_t)bh::diff_partial_blocks(d.W, d.H);
-  if ((rc = ensure(ctx, ctx->dpart, nb * 5 * sizeof(double)))) return rc;
-  if ((rc = ensure(ctx, ctx->dgrad, 8 * sizeof(double)))) return rc;
-  if ((rc = ensure(ctx, ctx->drgb, npix * 3 * sizeof(float)))) return rc;
-  if ((rc = ensure(ctx, ctx->dhit, npix))) return rc;
-  if ((rc = ensure(ctx, ctx->dgin, npix * 3 * sizeof(float)))) return rc;
-  hipStream_t st = ctx->stream;
-  // dL/drgb: every byte 0x3c, i.e. 0.0115 per channel
-  HIP_TRY(ctx, hipMemsetAsync(ctx->dgin.p, 0x3c, npix * 3 * sizeof(float), st));
-  float* rgb = (float*)ctx->drgb.p;
-  uint8_t* hit = (uint8_t*)ctx->dhit.p;
-  const float* gin = (const float*)ctx->dgin.p;
-  double* part = (double*)ctx->dpart.p;
-  double* grad = (double*)ctx->dgrad.p;
-  const bool f64 = precision == BH_PREC_FP64;
-  HIP_TRY(ctx, bh::launch_diff_forward(d, f64, rgb, hit, st));
-  HIP_TRY(ctx, bh::launch_diff_adjoint(d, f64, gin, rgb, hit, part, grad, nullptr, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  float ms_f = 0, ms_a = 0;
-  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-  for (int i = 0; i < iters; ++i) HIP_TRY(ctx, bh::launch_diff_forward(d, f64, rgb, hit, st));
-  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-  for (int i = 0; i < iters; ++i)
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
+    bh::DiffScene d;
+    int rc = build_diff(ctx, p, precision, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)d.W * d.H;
+    const size_t nb = (size_t)bh::diff_partial_blocks(d.W, d.H);
+    if ((rc = ensure(ctx, ctx->dpart, nb * 5 * sizeof(double)))) return rc;
+    if ((rc = ensure(ctx, ctx->dgrad, 8 * sizeof(double)))) return rc;
+    if ((rc = ensure(ctx, ctx->drgb, npix * 3 * sizeof(float)))) return rc;
+    if ((rc = ensure(ctx, ctx->dhit, npix))) return rc;
+    if ((rc = ensure(ctx, ctx->dgin, npix * 3 * sizeof(float)))) return rc;
+    hipStream_t st = ctx->stream;
+    // dL/drgb: every byte 0x3c, i.e. 0.0115 per channel
+    HIP_TRY(ctx, hipMemsetAsync(ctx->dgin.p, 0x3c, npix * 3 * sizeof(float), st));
+    float* rgb = (float*)ctx->drgb.p;
+    uint8_t* hit = (uint8_t*)ctx->dhit.p;
+    const float* gin = (const float*)ctx->dgin.p;
+    double* part = (double*)ctx->dpart.p;
+    double* grad = (double*)ctx->dgrad.p;
+    const bool f64 = precision == BH_PREC_FP64;
+    HIP_TRY(ctx, bh::launch_diff_forward(d, f64, rgb, hit, st));
     HIP_TRY(ctx, bh::launch_diff_adjoint(d, f64, gin, rgb, hit, part, grad, nullptr, st));
-  HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
-  HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
-  HIP_TRY(ctx, hipEventElapsedTime(&ms_f, ctx->ev0, ctx->ev1));
-  HIP_TRY(ctx, hipEventElapsedTime(&ms_a, ctx->ev1, ctx->ev2));
-  if (fwd_ms) *fwd_ms = ms_f / iters;
-  if (fwd_adj_ms) *fwd_adj_ms = ms_a / iters;
-  return BH_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    float ms_f = 0, ms_a = 0;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    for (int i = 0; i < iters; ++i) HIP_TRY(ctx, bh::launch_diff_forward(d, f64, rgb, hit, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    for (int i = 0; i < iters; ++i)
+      HIP_TRY(ctx, bh::launch_diff_adjoint(d, f64, gin, rgb, hit, part, grad, nullptr, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+    HIP_TRY(ctx, hipEventElapsedTime(&ms_f, ctx->ev0, ctx->ev1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ms_a, ctx->ev1, ctx->ev2));
+    if (fwd_ms) *fwd_ms = ms_f / iters;
+    if (fwd_adj_ms) *fwd_adj_ms = ms_a / iters;
+    return BH_OK;
+  });
 }
 
 /* ---- batches ------------------------------------------------------------------- */
 int bh_render_batch_device(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* options,
                            const bh_outputs* outs, const bh_batch_tuning* tune, double* scene_ms,
                            bh_batch_stats* stats) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  const bh_batch_tuning t = normalized_tuning(tune);
-  int rc = check_tuning(ctx, t);
-  if (rc) return rc;
-  BatchPlan p;
-  if ((rc = plan_batch(ctx, n, scenes, options, outs, t.mode == BH_BATCH_LANES, &p))) return rc;
-  (void)hipSetDevice(ctx->device);
-  bh::BatchDev b;
-  if ((rc = run_batch(ctx, p, t, &b))) return rc;
-  if (!scene_ms && !stats) return BH_OK;
-  HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
-  if (stats) {
-    float m = 0, tot = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&m, ctx->ev0, ctx->ev1));
-    HIP_TRY(ctx, hipEventElapsedTime(&tot, ctx->ev0, ctx->ev2));
-    uint32_t nb = 0;
-    HIP_TRY(ctx, hipMemcpy(&nb, b.band_count, 4, hipMemcpyDeviceToHost));
-    stats->march_ms = m;
-    stats->band_ms = tot - m;
-    stats->total_ms = tot;
-    int64_t px = 0;
-    for (const auto& d : p.scenes) px += (int64_t)d.W * d.H;
-    stats->rays = px;
-    stats->band_rays = std::min(nb, b.band_capacity);
-    stats->solo_scenes = (int32_t)p.solo.size();
-    stats->pad_ = 0;
-  }
-  if (scene_ms) {
-    std::vector<unsigned long long> end(p.fast.size());
-    unsigned long long t0 = 0;
-    if (!end.empty())
-      HIP_TRY(ctx, hipMemcpy(end.data(), b.scene_end, end.size() * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(ctx, hipMemcpy(&t0, b.t_start, 8, hipMemcpyDeviceToHost));
-    const double khz = ctx->wall_khz > 0 ? ctx->wall_khz : 100000.0;
-    float tot = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&tot, ctx->ev0, ctx->ev2));
-    for (int i = 0; i < n; ++i) scene_ms[i] = tot;  // solo scenes: end of the call
-    for (size_t k = 0; k < p.fast.size(); ++k)
-      scene_ms[p.fast[k]] = end[k] >= t0 ? (double)(end[k] - t0) / khz : 0.0;
-  }
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    const bh_batch_tuning t = normalized_tuning(tune);
+    int rc = check_tuning(ctx, t);
+    if (rc) return rc;
+    BatchPlan p;
+    if ((rc = plan_batch(ctx, n, scenes, options, outs, t.mode == BH_BATCH_LANES, &p))) return rc;
+    (void)hipSetDevice(ctx->device);
+    bh::BatchDev b;
+    if ((rc = run_batch(ctx, p, t, &b))) return rc;
+    if (!scene_ms && !stats) return BH_OK;
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+    if (stats) {
+      float m = 0, tot = 0;
+      HIP_TRY(ctx, hipEventElapsedTime(&m, ctx->ev0, ctx->ev1));
+      HIP_TRY(ctx, hipEventElapsedTime(&tot, ctx->ev0, ctx->ev2));
+      uint32_t nb = 0;
+      HIP_TRY(ctx, hipMemcpy(&nb, b.band_count, 4, hipMemcpyDeviceToHost));
+      stats->march_ms = m;
+      stats->band_ms = tot - m;
+      stats->total_ms = tot;
+      int64_t px = 0;
+      for (const auto& d : p.scenes) px += (int64_t)d.W * d.H;
+      stats->rays = px;
+      stats->band_rays = std::min(nb, b.band_capacity);
+      stats->solo_scenes = (int32_t)p.solo.size();
+      stats->pad_ = 0;
+    }
+    if (scene_ms) {
+      std::vector<unsigned long long> end(p.fast.size());
+      unsigned long long t0 = 0;
+      if (!end.empty())
+        HIP_TRY(ctx, hipMemcpy(end.data(), b.scene_end, end.size() * 8, hipMemcpyDeviceToHost));
+      HIP_TRY(ctx, hipMemcpy(&t0, b.t_start, 8, hipMemcpyDeviceToHost));
+      const double khz = ctx->wall_khz > 0 ? ctx->wall_khz : 100000.0;
+      float tot = 0;
+      HIP_TRY(ctx, hipEventElapsedTime(&tot, ctx->ev0, ctx->ev2));
+      for (int i = 0; i < n; ++i) scene_ms[i] = tot;  // solo scenes: end of the call
+      for (size_t k = 0; k < p.fast.size(); ++k)
+        scene_ms[p.fast[k]] = end[k] >= t0 ? (double)(end[k] - t0) / khz : 0.0;
+    }
+    return BH_OK;
+  });
 }
 
 int bh_batch_autotune(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* options,
                       const bh_outputs* outs, const bh_batch_tuning* cand, int n_cand, int reps,
                       bh_batch_tuning* best, double* best_ms, double* cand_ms) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!cand || n_cand <= 0 || !best) return fail(ctx, BH_ERR_INVALID_ARG, "need candidates and best");
-  if (reps <= 0) reps = 1;
-  // one plan per mode (the lane-refill mode renders sphere scenes solo)
-  BatchPlan plans[2];
-  bool planned[2] = {false, false};
-  (void)hipSetDevice(ctx->device);
-  double bms = 1e300;
-  int bi = -1;
-  int rc;
-  for (int c = 0; c < n_cand; ++c) {
-    const bh_batch_tuning t = normalized_tuning(&cand[c]);
-    if ((rc = check_tuning(ctx, t))) return rc;
-    const int m = t.mode == BH_BATCH_LANES ? 1 : 0;
-    if (!planned[m]) {
-      if ((rc = plan_batch(ctx, n, scenes, options, outs, m == 1, &plans[m]))) return rc;
-      planned[m] = true;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!cand || n_cand <= 0 || !best) return fail(ctx, BH_ERR_INVALID_ARG, "need candidates and best");
+    if (reps <= 0) reps = 1;
+    // one plan per mode (the lane-refill mode renders sphere scenes solo)
+    BatchPlan plans[2];
+    bool planned[2] = {false, false};
+    (void)hipSetDevice(ctx->device);
+    double bms = 1e300;
+    int bi = -1;
+    int rc;
+    for (int c = 0; c < n_cand; ++c) {
+      const bh_batch_tuning t = normalized_tuning(&cand[c]);
+      if ((rc = check_tuning(ctx, t))) return rc;
+      const int m = t.mode == BH_BATCH_LANES ? 1 : 0;
+      if (!planned[m]) {
+        if ((rc = plan_batch(ctx, n, scenes, options, outs, m == 1, &plans[m]))) return rc;
+        planned[m] = true;
+      }
+      const BatchPlan& p = plans[m];
+      double cms = 1e300;
+      for (int r = 0; r < reps + 1; ++r) {  // first run warms up
+        if ((rc = run_batch(ctx, p, t, nullptr))) return rc;
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
+        float ms = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev2));
+        if (r > 0) cms = std::min(cms, (double)ms);
+      }
+      if (cand_ms) cand_ms[c] = cms;
+      if (cms < bms) {
+        bms = cms;
+        bi = c;
+      }
     }
-    const BatchPlan& p = plans[m];
-    double cms = 1e300;
-    for (int r = 0; r < reps + 1; ++r) {  // first run warms up
-      if ((rc = run_batch(ctx, p, t, nullptr))) return rc;
-      HIP_TRY(ctx, hipEventSynchronize(ctx->ev2));
-      float ms = 0;
-      HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev2));
-      if (r > 0) cms = std::min(cms, (double)ms);
-    }
-    if (cand_ms) cand_ms[c] = cms;
-    if (cms < bms) {
-      bms = cms;
-      bi = c;
-    }
-  }
-  *best = normalized_tuning(&cand[bi]);
-  if (best_ms) *best_ms = bms;
-  return BH_OK;
+    *best = normalized_tuning(&cand[bi]);
+    if (best_ms) *best_ms = bms;
+    return BH_OK;
+  });
 }
 
 static int build_flat(bh_ctx* ctx, const bh_flat_scene* sc, bh::DevFlat* dp) {
@@ -1225,63 +1289,69 @@ static int build_flat(bh_ctx* ctx, const bh_flat_scene* sc, bh::DevFlat* dp) {
 }
 
 int bh_render_flat_device(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
-  bh::DevFlat d;
-  int rc = build_flat(ctx, sc, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  DevOut o{out->rgb, out->hit, out->phi_end, out->hit_point, out->steps};
-  HIP_TRY(ctx, bh::launch_flat(d, o, ctx->stream));
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
+    bh::DevFlat d;
+    int rc = build_flat(ctx, sc, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    DevOut o{out->rgb, out->hit, out->phi_end, out->hit_point, out->steps};
+    HIP_TRY(ctx, bh::launch_flat(d, o, ctx->stream));
+    return BH_OK;
+  });
 }
 
 int bh_render_flat(bh_ctx* ctx, const bh_flat_scene* sc, const bh_outputs* out) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!sc || !out) return fail(ctx, BH_ERR_INVALID_ARG, "scene and outputs are required");
-  bh::DevFlat d;
-  int rc = build_flat(ctx, sc, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  const size_t npix = (size_t)d.W * d.H;
-  DevOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
-  if (out->rgb) { if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc; o.rgb = (uint8_t*)ctx->rgb.p; }
-  if (out->hit) { if ((rc = ensure(ctx, ctx->hit, npix))) return rc; o.hit = (uint8_t*)ctx->hit.p; }
-  if (out->phi_end) { if ((rc = ensure(ctx, ctx->phi, npix * 8))) return rc; o.phi = (double*)ctx->phi.p; }
-  if (out->hit_point) { if ((rc = ensure(ctx, ctx->hitpt, npix * 24))) return rc; o.hitpt = (double*)ctx->hitpt.p; }
-  if (out->steps) { if ((rc = ensure(ctx, ctx->steps, npix * 4))) return rc; o.steps = (int32_t*)ctx->steps.p; }
-  hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, bh::launch_flat(d, o, st));
-  if (out->rgb) HIP_TRY(ctx, hipMemcpyAsync(out->rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, st));
-  if (out->hit) HIP_TRY(ctx, hipMemcpyAsync(out->hit, o.hit, npix, hipMemcpyDeviceToHost, st));
-  if (out->phi_end) HIP_TRY(ctx, hipMemcpyAsync(out->phi_end, o.phi, npix * 8, hipMemcpyDeviceToHost, st));
-  if (out->hit_point) HIP_TRY(ctx, hipMemcpyAsync(out->hit_point, o.hitpt, npix * 24, hipMemcpyDeviceToHost, st));
-  if (out->steps) HIP_TRY(ctx, hipMemcpyAsync(out->steps, o.steps, npix * 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!sc || !out) return fail(ctx, BH_ERR_INVALID_ARG, "scene and outputs are required");
+    bh::DevFlat d;
+    int rc = build_flat(ctx, sc, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)d.W * d.H;
+    DevOut o{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (out->rgb) { if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc; o.rgb = (uint8_t*)ctx->rgb.p; }
+    if (out->hit) { if ((rc = ensure(ctx, ctx->hit, npix))) return rc; o.hit = (uint8_t*)ctx->hit.p; }
+    if (out->phi_end) { if ((rc = ensure(ctx, ctx->phi, npix * 8))) return rc; o.phi = (double*)ctx->phi.p; }
+    if (out->hit_point) { if ((rc = ensure(ctx, ctx->hitpt, npix * 24))) return rc; o.hitpt = (double*)ctx->hitpt.p; }
+    if (out->steps) { if ((rc = ensure(ctx, ctx->steps, npix * 4))) return rc; o.steps = (int32_t*)ctx->steps.p; }
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, bh::launch_flat(d, o, st));
+    if (out->rgb) HIP_TRY(ctx, hipMemcpyAsync(out->rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, st));
+    if (out->hit) HIP_TRY(ctx, hipMemcpyAsync(out->hit, o.hit, npix, hipMemcpyDeviceToHost, st));
+    if (out->phi_end) HIP_TRY(ctx, hipMemcpyAsync(out->phi_end, o.phi, npix * 8, hipMemcpyDeviceToHost, st));
+    if (out->hit_point) HIP_TRY(ctx, hipMemcpyAsync(out->hit_point, o.hitpt, npix * 24, hipMemcpyDeviceToHost, st));
+    if (out->steps) HIP_TRY(ctx, hipMemcpyAsync(out->steps, o.steps, npix * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    return BH_OK;
+  });
 }
 
 int bh_time_flat(bh_ctx* ctx, const bh_flat_scene* sc, int iters, double* avg_ms) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
-  bh::DevFlat d;
-  int rc = build_flat(ctx, sc, &d);
-  if (rc) return rc;
-  (void)hipSetDevice(ctx->device);
-  const size_t npix = (size_t)d.W * d.H;
-  if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc;
-  if ((rc = ensure(ctx, ctx->hit, npix))) return rc;
-  DevOut o{(uint8_t*)ctx->rgb.p, (uint8_t*)ctx->hit.p, nullptr, nullptr, nullptr};
-  hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, bh::launch_flat(d, o, st));  // warm-up
-  HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-  for (int it = 0; it < iters; ++it) HIP_TRY(ctx, bh::launch_flat(d, o, st));
-  HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
-  float ms = 0;
-  HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  if (avg_ms) *avg_ms = ms / iters;
-  return BH_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (iters <= 0) return fail(ctx, BH_ERR_INVALID_ARG, "iters must be positive");
+    bh::DevFlat d;
+    int rc = build_flat(ctx, sc, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)d.W * d.H;
+    if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc;
+    if ((rc = ensure(ctx, ctx->hit, npix))) return rc;
+    DevOut o{(uint8_t*)ctx->rgb.p, (uint8_t*)ctx->hit.p, nullptr, nullptr, nullptr};
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, bh::launch_flat(d, o, st));  // warm-up
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    for (int it = 0; it < iters; ++it) HIP_TRY(ctx, bh::launch_flat(d, o, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (avg_ms) *avg_ms = ms / iters;
+    return BH_OK;
+  });
 }
 
 // ---- frame ring ------------------------------------------------------------
@@ -1314,33 +1384,35 @@ static void ring_free(bh_ring* r) {
 }
 
 int bh_ring_create(bh_ctx* ctx, int width, int height, int depth, bh_ring** out) {
-  if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
-  if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "out is NULL");
-  *out = nullptr;
-  if (width <= 0 || height <= 0 || width > 32768 || height > 32768)
-    return fail(ctx, BH_ERR_INVALID_ARG, "width/height must be in 1..32768");
-  if (depth < 2 || depth > 8) return fail(ctx, BH_ERR_INVALID_ARG, "depth must be in 2..8");
-  (void)hipSetDevice(ctx->device);
-  bh_ring* r = new bh_ring();
-  r->ctx = ctx;
-  r->W = width;
-  r->H = height;
-  r->slots.resize(depth);
-  const size_t npix = (size_t)width * height;
-  for (auto& s : r->slots) {
-    hipError_t e = hipMalloc(&s.d_rgb, npix * 3);
-    if (e == hipSuccess) e = hipMalloc(&s.d_hit, npix);
-    if (e == hipSuccess) e = hipHostMalloc(&s.h_rgb, npix * 3, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc(&s.h_hit, npix, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreate(&s.start);
-    if (e == hipSuccess) e = hipEventCreate(&s.done);
-    if (e != hipSuccess) {
-      ring_free(r);
-      return hip_fail(ctx, e, "frame ring allocation");
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!out) return fail(ctx, BH_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || width > 32768 || height > 32768)
+      return fail(ctx, BH_ERR_INVALID_ARG, "width/height must be in 1..32768");
+    if (depth < 2 || depth > 8) return fail(ctx, BH_ERR_INVALID_ARG, "depth must be in 2..8");
+    (void)hipSetDevice(ctx->device);
+    bh_ring* r = new bh_ring();
+    r->ctx = ctx;
+    r->W = width;
+    r->H = height;
+    r->slots.resize(depth);
+    const size_t npix = (size_t)width * height;
+    for (auto& s : r->slots) {
+      hipError_t e = hipMalloc(&s.d_rgb, npix * 3);
+      if (e == hipSuccess) e = hipMalloc(&s.d_hit, npix);
+      if (e == hipSuccess) e = hipHostMalloc(&s.h_rgb, npix * 3, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipHostMalloc(&s.h_hit, npix, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreate(&s.start);
+      if (e == hipSuccess) e = hipEventCreate(&s.done);
+      if (e != hipSuccess) {
+        ring_free(r);
+        return hip_fail(ctx, e, "frame ring allocation");
+      }
     }
-  }
-  *out = r;
-  return BH_OK;
+    *out = r;
+    return BH_OK;
+  });
 }
 
 // Take the next slot, waiting while the host still holds it.
@@ -1370,63 +1442,71 @@ static int ring_finish(bh_ring* r, bh_ring::Slot* s, int idx, int rc, int* slot)
 }
 
 int bh_ring_render(bh_ring* r, const bh_scene* scene, const bh_options* opt, int* slot) {
-  if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
-  bh_ctx* ctx = r->ctx;
-  if (!scene || !slot) return fail(ctx, BH_ERR_INVALID_ARG, "scene/slot are NULL");
-  if (scene->camera.width != r->W || scene->camera.height != r->H)
-    return fail(ctx, BH_ERR_INVALID_ARG, "camera size differs from the ring's");
-  int idx;
-  bh_ring::Slot* s = ring_acquire(r, &idx);
-  (void)hipSetDevice(ctx->device);
-  HIP_TRY(ctx, hipEventRecord(s->start, ctx->stream));
-  bh_outputs o{s->d_rgb, s->d_hit, nullptr, nullptr, nullptr};
-  return ring_finish(r, s, idx, bh_render_device(ctx, scene, opt, &o), slot);
+  return guarded(nullptr, [&]() -> int {
+    if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
+    bh_ctx* ctx = r->ctx;
+    if (!scene || !slot) return fail(ctx, BH_ERR_INVALID_ARG, "scene/slot are NULL");
+    if (scene->camera.width != r->W || scene->camera.height != r->H)
+      return fail(ctx, BH_ERR_INVALID_ARG, "camera size differs from the ring's");
+    int idx;
+    bh_ring::Slot* s = ring_acquire(r, &idx);
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipEventRecord(s->start, ctx->stream));
+    bh_outputs o{s->d_rgb, s->d_hit, nullptr, nullptr, nullptr};
+    return ring_finish(r, s, idx, bh_render_device(ctx, scene, opt, &o), slot);
+  });
 }
 
 int bh_ring_render_flat(bh_ring* r, const bh_flat_scene* scene, int* slot) {
-  if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
-  bh_ctx* ctx = r->ctx;
-  if (!scene || !slot) return fail(ctx, BH_ERR_INVALID_ARG, "scene/slot are NULL");
-  if (scene->camera.width != r->W || scene->camera.height != r->H)
-    return fail(ctx, BH_ERR_INVALID_ARG, "camera size differs from the ring's");
-  int idx;
-  bh_ring::Slot* s = ring_acquire(r, &idx);
-  (void)hipSetDevice(ctx->device);
-  HIP_TRY(ctx, hipEventRecord(s->start, ctx->stream));
-  bh_outputs o{s->d_rgb, s->d_hit, nullptr, nullptr, nullptr};
-  return ring_finish(r, s, idx, bh_render_flat_device(ctx, scene, &o), slot);
+  return guarded(nullptr, [&]() -> int {
+    if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
+    bh_ctx* ctx = r->ctx;
+    if (!scene || !slot) return fail(ctx, BH_ERR_INVALID_ARG, "scene/slot are NULL");
+    if (scene->camera.width != r->W || scene->camera.height != r->H)
+      return fail(ctx, BH_ERR_INVALID_ARG, "camera size differs from the ring's");
+    int idx;
+    bh_ring::Slot* s = ring_acquire(r, &idx);
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipEventRecord(s->start, ctx->stream));
+    bh_outputs o{s->d_rgb, s->d_hit, nullptr, nullptr, nullptr};
+    return ring_finish(r, s, idx, bh_render_flat_device(ctx, scene, &o), slot);
+  });
 }
 
 int bh_ring_wait(bh_ring* r, int slot, const uint8_t** rgb, const uint8_t** hit, float* gpu_ms) {
-  if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
-  if (slot < 0 || slot >= (int)r->slots.size())
-    return fail(nullptr, BH_ERR_INVALID_ARG, "slot out of range");
-  bh_ring::Slot& s = r->slots[slot];
-  {
-    std::lock_guard<std::mutex> lk(r->m);
-    if (!s.busy) return fail(nullptr, BH_ERR_INVALID_ARG, "slot holds no frame");
-  }
-  hipError_t e = hipEventSynchronize(s.done);
-  if (e != hipSuccess) return fail(nullptr, BH_ERR_HIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
-  if (gpu_ms) {
-    e = hipEventElapsedTime(gpu_ms, s.start, s.done);
-    if (e != hipSuccess) *gpu_ms = -1.0f;
-  }
-  if (rgb) *rgb = s.h_rgb;
-  if (hit) *hit = s.h_hit;
-  return BH_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
+    if (slot < 0 || slot >= (int)r->slots.size())
+      return fail(nullptr, BH_ERR_INVALID_ARG, "slot out of range");
+    bh_ring::Slot& s = r->slots[slot];
+    {
+      std::lock_guard<std::mutex> lk(r->m);
+      if (!s.busy) return fail(nullptr, BH_ERR_INVALID_ARG, "slot holds no frame");
+    }
+    hipError_t e = hipEventSynchronize(s.done);
+    if (e != hipSuccess) return fail(nullptr, BH_ERR_HIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+    if (gpu_ms) {
+      e = hipEventElapsedTime(gpu_ms, s.start, s.done);
+      if (e != hipSuccess) *gpu_ms = -1.0f;
+    }
+    if (rgb) *rgb = s.h_rgb;
+    if (hit) *hit = s.h_hit;
+    return BH_OK;
+  });
 }
 
 int bh_ring_release(bh_ring* r, int slot) {
-  if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
-  if (slot < 0 || slot >= (int)r->slots.size())
-    return fail(nullptr, BH_ERR_INVALID_ARG, "slot out of range");
-  {
-    std::lock_guard<std::mutex> lk(r->m);
-    r->slots[slot].busy = false;
-  }
-  r->cv.notify_all();
-  return BH_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!r) return fail(nullptr, BH_ERR_BAD_HANDLE, "ring is NULL");
+    if (slot < 0 || slot >= (int)r->slots.size())
+      return fail(nullptr, BH_ERR_INVALID_ARG, "slot out of range");
+    {
+      std::lock_guard<std::mutex> lk(r->m);
+      r->slots[slot].busy = false;
+    }
+    r->cv.notify_all();
+    return BH_OK;
+  });
 }
 
 void bh_ring_destroy(bh_ring* r) {

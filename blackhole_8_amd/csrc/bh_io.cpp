@@ -16,8 +16,10 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "bh_api.h"
@@ -305,6 +307,18 @@ int io_fail(int code, const std::string& msg) {
   return code;
 }
 
+// Body of a C entry point: no exception crosses the C boundary.
+template <class F>
+int guarded(F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return io_fail(BH_ERR_OUT_OF_MEMORY, "out of host memory");
+  } catch (const std::exception& e) {
+    return io_fail(BH_ERR_INVALID_ARG, e.what());
+  }
+}
+
 bool read_file(const char* path, std::vector<uint8_t>& out) {
   FILE* f = std::fopen(path, "rb");
   if (!f) return false;
@@ -482,171 +496,189 @@ void fill_header(bh_video* v, uint32_t frames) {
 extern "C" {
 
 int bh_video_open(const char* path, int rows, int cols, double fps, int quality, bh_video** out) {
-  if (!out) return io_fail(BH_ERR_INVALID_ARG, "out is NULL");
-  *out = nullptr;
-  if (!path || rows <= 0 || cols <= 0 || rows > 65535 || cols > 65535)
-    return io_fail(BH_ERR_INVALID_ARG, "bad video size");
-  if (!(fps > 0 && fps <= 1000)) return io_fail(BH_ERR_INVALID_ARG, "fps must be in (0, 1000]");
-  if (quality < 1 || quality > 100) return io_fail(BH_ERR_INVALID_ARG, "quality must be 1..100");
-  bh_video* v = new (std::nothrow) bh_video;
-  if (!v) return io_fail(BH_ERR_OUT_OF_MEMORY, "out of host memory");
-  v->rows = rows;
-  v->cols = cols;
-  v->quality = quality;
-  v->f = std::fopen(path, "wb");
-  if (!v->f) {
-    delete v;
-    return io_fail(BH_ERR_INVALID_ARG, std::string("cannot write ") + path);
-  }
-  v->fps = fps;
-  fill_header(v, 0);  // rewritten by bh_video_close with the counts
-  if (std::fwrite(v->hdr, 1, kHdr, v->f) != kHdr) {
-    std::fclose(v->f);
-    delete v;
-    return io_fail(BH_ERR_INVALID_ARG, std::string("write failed: ") + path);
-  }
-  *out = v;
-  return BH_OK;
+  return guarded([&]() -> int {
+    if (!out) return io_fail(BH_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!path || rows <= 0 || cols <= 0 || rows > 65535 || cols > 65535)
+      return io_fail(BH_ERR_INVALID_ARG, "bad video size");
+    if (!(fps > 0 && fps <= 1000)) return io_fail(BH_ERR_INVALID_ARG, "fps must be in (0, 1000]");
+    if (quality < 1 || quality > 100) return io_fail(BH_ERR_INVALID_ARG, "quality must be 1..100");
+    bh_video* v = new (std::nothrow) bh_video;
+    if (!v) return io_fail(BH_ERR_OUT_OF_MEMORY, "out of host memory");
+    v->rows = rows;
+    v->cols = cols;
+    v->quality = quality;
+    v->f = std::fopen(path, "wb");
+    if (!v->f) {
+      delete v;
+      return io_fail(BH_ERR_INVALID_ARG, std::string("cannot write ") + path);
+    }
+    v->fps = fps;
+    fill_header(v, 0);  // rewritten by bh_video_close with the counts
+    if (std::fwrite(v->hdr, 1, kHdr, v->f) != kHdr) {
+      std::fclose(v->f);
+      delete v;
+      return io_fail(BH_ERR_INVALID_ARG, std::string("write failed: ") + path);
+    }
+    *out = v;
+    return BH_OK;
+  });
 }
 
 int bh_video_write(bh_video* v, const uint8_t* bgr) {
-  if (!v || !bgr) return io_fail(BH_ERR_INVALID_ARG, "video/frame is NULL");
-  if (v->err != BH_OK) return io_fail(v->err, v->msg);
-  uint8_t* jpg = nullptr;
-  size_t n = 0;
-  int rc = bh_jpeg_encode(bgr, v->rows, v->cols, v->quality, &jpg, &n);
-  if (rc != BH_OK) return video_fail(v, rc, bh_jpeg_last_error());
-  const uint64_t chunk = 8 + n + (n & 1);
-  if (v->pos + chunk + v->idx.size() + 16 + 8 > kMaxAvi) {
+  return guarded([&]() -> int {
+    if (!v || !bgr) return io_fail(BH_ERR_INVALID_ARG, "video/frame is NULL");
+    if (v->err != BH_OK) return io_fail(v->err, v->msg);
+    uint8_t* jpg = nullptr;
+    size_t n = 0;
+    int rc = bh_jpeg_encode(bgr, v->rows, v->cols, v->quality, &jpg, &n);
+    if (rc != BH_OK) return video_fail(v, rc, bh_jpeg_last_error());
+    const uint64_t chunk = 8 + n + (n & 1);
+    if (v->pos + chunk + v->idx.size() + 16 + 8 > kMaxAvi) {
+      bh_free(jpg);
+      return video_fail(v, BH_ERR_UNSUPPORTED, "AVI file would pass 1 GiB");
+    }
+    try {
+      v->idx.reserve(v->idx.size() + 16);  // the index entry cannot fail once the chunk is out
+    } catch (...) {
+      bh_free(jpg);
+      throw;
+    }
+    uint8_t ch[8] = {'0', '0', 'd', 'c'};
+    put_le32(ch + 4, (uint32_t)n);
+    const uint8_t pad = 0;
+    bool ok = std::fwrite(ch, 1, 8, v->f) == 8 && std::fwrite(jpg, 1, n, v->f) == n &&
+              (!(n & 1) || std::fwrite(&pad, 1, 1, v->f) == 1);
     bh_free(jpg);
-    return video_fail(v, BH_ERR_UNSUPPORTED, "AVI file would pass 1 GiB");
-  }
-  uint8_t ch[8] = {'0', '0', 'd', 'c'};
-  put_le32(ch + 4, (uint32_t)n);
-  const uint8_t pad = 0;
-  bool ok = std::fwrite(ch, 1, 8, v->f) == 8 && std::fwrite(jpg, 1, n, v->f) == n &&
-            (!(n & 1) || std::fwrite(&pad, 1, 1, v->f) == 1);
-  bh_free(jpg);
-  if (!ok) return video_fail(v, BH_ERR_INVALID_ARG, "video write failed");
-  uint8_t e[16] = {'0', '0', 'd', 'c'};
-  put_le32(e + 4, 0x10);  // AVIIF_KEYFRAME
-  put_le32(e + 8, (uint32_t)(v->pos - (kMoviList + 8)));  // from the 'movi' tag
-  put_le32(e + 12, (uint32_t)n);
-  v->idx.insert(v->idx.end(), e, e + 16);
-  v->pos += chunk;
-  if (n > v->max_frame) v->max_frame = (uint32_t)n;
-  return BH_OK;
+    if (!ok) return video_fail(v, BH_ERR_INVALID_ARG, "video write failed");
+    uint8_t e[16] = {'0', '0', 'd', 'c'};
+    put_le32(e + 4, 0x10);  // AVIIF_KEYFRAME
+    put_le32(e + 8, (uint32_t)(v->pos - (kMoviList + 8)));  // from the 'movi' tag
+    put_le32(e + 12, (uint32_t)n);
+    v->idx.insert(v->idx.end(), e, e + 16);
+    v->pos += chunk;
+    if (n > v->max_frame) v->max_frame = (uint32_t)n;
+    return BH_OK;
+  });
 }
 
 int bh_video_close(bh_video* v) {
-  if (!v) return io_fail(BH_ERR_INVALID_ARG, "video is NULL");
-  int rc = v->err;
-  std::string msg = v->msg;
-  uint8_t ih[8] = {'i', 'd', 'x', '1'};
-  put_le32(ih + 4, (uint32_t)v->idx.size());
-  bool ok = std::fwrite(ih, 1, 8, v->f) == 8 &&
-            std::fwrite(v->idx.data(), 1, v->idx.size(), v->f) == v->idx.size();
-  fill_header(v, (uint32_t)(v->idx.size() / 16));
-  ok = ok && std::fseek(v->f, 0, SEEK_SET) == 0 &&
-       std::fwrite(v->hdr, 1, kHdr, v->f) == kHdr;
-  ok = (std::fclose(v->f) == 0) && ok;
-  delete v;
-  if (rc == BH_OK && !ok) {
-    rc = BH_ERR_INVALID_ARG;
-    msg = "video write failed";
-  }
-  return rc == BH_OK ? BH_OK : io_fail(rc, msg);
+  return guarded([&]() -> int {
+    if (!v) return io_fail(BH_ERR_INVALID_ARG, "video is NULL");
+    int rc = v->err;
+    std::string msg = std::move(v->msg);
+    uint8_t ih[8] = {'i', 'd', 'x', '1'};
+    put_le32(ih + 4, (uint32_t)v->idx.size());
+    bool ok = std::fwrite(ih, 1, 8, v->f) == 8 &&
+              std::fwrite(v->idx.data(), 1, v->idx.size(), v->f) == v->idx.size();
+    fill_header(v, (uint32_t)(v->idx.size() / 16));
+    ok = ok && std::fseek(v->f, 0, SEEK_SET) == 0 &&
+         std::fwrite(v->hdr, 1, kHdr, v->f) == kHdr;
+    ok = (std::fclose(v->f) == 0) && ok;
+    delete v;
+    if (rc == BH_OK && !ok) {
+      rc = BH_ERR_INVALID_ARG;
+      msg = "video write failed";
+    }
+    return rc == BH_OK ? BH_OK : io_fail(rc, msg);
+  });
 }
 
 int bh_png_decode(const uint8_t* data, size_t size, uint8_t* dst, size_t dst_size, int* rows,
                   int* cols) {
-  if (!rows || !cols) return io_fail(BH_ERR_INVALID_ARG, "rows/cols are NULL");
-  Png g;
-  std::string err;
-  int rc = parse(data, size, g, err);
-  if (rc) return io_fail(rc, err);
-  *rows = (int)g.h;
-  *cols = (int)g.w;
-  if (!dst) return BH_OK;
-  if (dst_size < (size_t)g.w * g.h * 3) return io_fail(BH_ERR_INVALID_ARG, "dst is too small");
-  rc = decode(g, dst, err);
-  return rc ? io_fail(rc, err) : BH_OK;
+  return guarded([&]() -> int {
+    if (!rows || !cols) return io_fail(BH_ERR_INVALID_ARG, "rows/cols are NULL");
+    Png g;
+    std::string err;
+    int rc = parse(data, size, g, err);
+    if (rc) return io_fail(rc, err);
+    *rows = (int)g.h;
+    *cols = (int)g.w;
+    if (!dst) return BH_OK;
+    if (dst_size < (size_t)g.w * g.h * 3) return io_fail(BH_ERR_INVALID_ARG, "dst is too small");
+    rc = decode(g, dst, err);
+    return rc ? io_fail(rc, err) : BH_OK;
+  });
 }
 
 int bh_image_load(const char* path, uint8_t* dst, size_t dst_size, int* rows, int* cols) {
-  if (!path || !rows || !cols) return io_fail(BH_ERR_INVALID_ARG, "path/rows/cols are NULL");
-  std::vector<uint8_t> f;
-  if (!read_file(path, f)) return io_fail(BH_ERR_INVALID_ARG, std::string("cannot read ") + path);
-  if (f.size() >= 8 && std::memcmp(f.data(), kSig, 8) == 0)
-    return bh_png_decode(f.data(), f.size(), dst, dst_size, rows, cols);
-  if (f.size() >= 2 && f[0] == 'P' && f[1] == '6') return decode_ppm(f, dst, dst_size, rows, cols);
-  if (f.size() >= 3 && f[0] == 0xFF && f[1] == 0xD8 && f[2] == 0xFF) {
-    const int rc = bh_jpeg_decode(f.data(), f.size(), dst, dst_size, rows, cols);
-    return rc ? io_fail(rc, bh_jpeg_last_error()) : BH_OK;
-  }
-  return io_fail(BH_ERR_UNSUPPORTED, "only PNG, JPEG and binary PPM files are read");
+  return guarded([&]() -> int {
+    if (!path || !rows || !cols) return io_fail(BH_ERR_INVALID_ARG, "path/rows/cols are NULL");
+    std::vector<uint8_t> f;
+    if (!read_file(path, f)) return io_fail(BH_ERR_INVALID_ARG, std::string("cannot read ") + path);
+    if (f.size() >= 8 && std::memcmp(f.data(), kSig, 8) == 0)
+      return bh_png_decode(f.data(), f.size(), dst, dst_size, rows, cols);
+    if (f.size() >= 2 && f[0] == 'P' && f[1] == '6') return decode_ppm(f, dst, dst_size, rows, cols);
+    if (f.size() >= 3 && f[0] == 0xFF && f[1] == 0xD8 && f[2] == 0xFF) {
+      const int rc = bh_jpeg_decode(f.data(), f.size(), dst, dst_size, rows, cols);
+      return rc ? io_fail(rc, bh_jpeg_last_error()) : BH_OK;
+    }
+    return io_fail(BH_ERR_UNSUPPORTED, "only PNG, JPEG and binary PPM files are read");
+  });
 }
 
 int bh_image_write(const char* path, const uint8_t* bgr, int rows, int cols) {
-  if (!path || !bgr || rows <= 0 || cols <= 0) return io_fail(BH_ERR_INVALID_ARG, "bad image");
-  const std::string p(path);
-  const bool png = ends_with(p, ".png"), ppm = ends_with(p, ".ppm");
-  const bool jpg = ends_with(p, ".jpg") || ends_with(p, ".jpeg");
-  if (!png && !ppm && !jpg)
-    return io_fail(BH_ERR_UNSUPPORTED, "path must end in .ppm, .png, .jpg or .jpeg");
-  const size_t n = (size_t)rows * cols;
-  std::vector<uint8_t> out;
-  if (jpg) {
-    uint8_t* d = nullptr;
-    size_t dn = 0;
-    const int rc = bh_jpeg_encode(bgr, rows, cols, 95, &d, &dn);
-    if (rc != BH_OK) return io_fail(rc, bh_jpeg_last_error());
-    out.assign(d, d + dn);
-    bh_free(d);
-  } else if (ppm) {
-    char hdr[64];
-    const int hl = std::snprintf(hdr, sizeof hdr, "P6\n%d %d\n255\n", cols, rows);
-    out.assign(hdr, hdr + hl);
-    out.resize(hl + n * 3);
-    uint8_t* o = out.data() + hl;
-    for (size_t k = 0; k < n; ++k) {
-      o[3 * k] = bgr[3 * k + 2];
-      o[3 * k + 1] = bgr[3 * k + 1];
-      o[3 * k + 2] = bgr[3 * k];
-    }
-  } else {
-    // RGB 8-bit, filter 0 on every line, one zlib stream at speed level 1.
-    const size_t lb = (size_t)cols * 3 + 1;
-    std::vector<uint8_t> raw(lb * rows);
-    for (int y = 0; y < rows; ++y) {
-      uint8_t* l = raw.data() + y * lb;
-      l[0] = 0;
-      const uint8_t* s = bgr + (size_t)y * cols * 3;
-      for (int x = 0; x < cols; ++x) {
-        l[1 + 3 * x] = s[3 * x + 2];
-        l[2 + 3 * x] = s[3 * x + 1];
-        l[3 + 3 * x] = s[3 * x];
+  return guarded([&]() -> int {
+    if (!path || !bgr || rows <= 0 || cols <= 0) return io_fail(BH_ERR_INVALID_ARG, "bad image");
+    const std::string p(path);
+    const bool png = ends_with(p, ".png"), ppm = ends_with(p, ".ppm");
+    const bool jpg = ends_with(p, ".jpg") || ends_with(p, ".jpeg");
+    if (!png && !ppm && !jpg)
+      return io_fail(BH_ERR_UNSUPPORTED, "path must end in .ppm, .png, .jpg or .jpeg");
+    const size_t n = (size_t)rows * cols;
+    std::vector<uint8_t> out;
+    if (jpg) {
+      uint8_t* d = nullptr;
+      size_t dn = 0;
+      const int rc = bh_jpeg_encode(bgr, rows, cols, 95, &d, &dn);
+      if (rc != BH_OK) return io_fail(rc, bh_jpeg_last_error());
+      out.assign(d, d + dn);
+      bh_free(d);
+    } else if (ppm) {
+      char hdr[64];
+      const int hl = std::snprintf(hdr, sizeof hdr, "P6\n%d %d\n255\n", cols, rows);
+      out.assign(hdr, hdr + hl);
+      out.resize(hl + n * 3);
+      uint8_t* o = out.data() + hl;
+      for (size_t k = 0; k < n; ++k) {
+        o[3 * k] = bgr[3 * k + 2];
+        o[3 * k + 1] = bgr[3 * k + 1];
+        o[3 * k + 2] = bgr[3 * k];
       }
+    } else {
+      // RGB 8-bit, filter 0 on every line, one zlib stream at speed level 1.
+      const size_t lb = (size_t)cols * 3 + 1;
+      std::vector<uint8_t> raw(lb * rows);
+      for (int y = 0; y < rows; ++y) {
+        uint8_t* l = raw.data() + y * lb;
+        l[0] = 0;
+        const uint8_t* s = bgr + (size_t)y * cols * 3;
+        for (int x = 0; x < cols; ++x) {
+          l[1 + 3 * x] = s[3 * x + 2];
+          l[2 + 3 * x] = s[3 * x + 1];
+          l[3 + 3 * x] = s[3 * x];
+        }
+      }
+      uLongf zlen = compressBound((uLong)raw.size());
+      std::vector<uint8_t> z(zlen);
+      if (compress2(z.data(), &zlen, raw.data(), (uLong)raw.size(), 1) != Z_OK)
+        return io_fail(BH_ERR_INVALID_ARG, "deflate failed");
+      out.assign(kSig, kSig + 8);
+      std::vector<uint8_t> ihdr;
+      put_be32(ihdr, (uint32_t)cols);
+      put_be32(ihdr, (uint32_t)rows);
+      const uint8_t tail[5] = {8, 2, 0, 0, 0};
+      ihdr.insert(ihdr.end(), tail, tail + 5);
+      png_chunk(out, "IHDR", ihdr.data(), ihdr.size());
+      png_chunk(out, "IDAT", z.data(), zlen);
+      png_chunk(out, "IEND", nullptr, 0);
     }
-    uLongf zlen = compressBound((uLong)raw.size());
-    std::vector<uint8_t> z(zlen);
-    if (compress2(z.data(), &zlen, raw.data(), (uLong)raw.size(), 1) != Z_OK)
-      return io_fail(BH_ERR_INVALID_ARG, "deflate failed");
-    out.assign(kSig, kSig + 8);
-    std::vector<uint8_t> ihdr;
-    put_be32(ihdr, (uint32_t)cols);
-    put_be32(ihdr, (uint32_t)rows);
-    const uint8_t tail[5] = {8, 2, 0, 0, 0};
-    ihdr.insert(ihdr.end(), tail, tail + 5);
-    png_chunk(out, "IHDR", ihdr.data(), ihdr.size());
-    png_chunk(out, "IDAT", z.data(), zlen);
-    png_chunk(out, "IEND", nullptr, 0);
-  }
-  FILE* f = std::fopen(path, "wb");
-  if (!f) return io_fail(BH_ERR_INVALID_ARG, std::string("cannot write ") + path);
-  const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
-  if (std::fclose(f) != 0 || !ok) return io_fail(BH_ERR_INVALID_ARG, std::string("write failed: ") + path);
-  return BH_OK;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return io_fail(BH_ERR_INVALID_ARG, std::string("cannot write ") + path);
+    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    if (std::fclose(f) != 0 || !ok) return io_fail(BH_ERR_INVALID_ARG, std::string("write failed: ") + path);
+    return BH_OK;
+  });
 }
 
 }  // extern "C"

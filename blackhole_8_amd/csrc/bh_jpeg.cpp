@@ -25,6 +25,7 @@
 #include <cstring>
 #include <algorithm>
 #include <array>
+#include <exception>
 #include <string>
 #include <system_error>
 #include <thread>
@@ -823,19 +824,30 @@ void walk_row(const int16_t* coefs, int mx, int j, F&& emit) {
 
 // Runs fn(part, j0, j1) over `parts` contiguous ranges of [0, n) on host
 // threads (range 0 on the calling thread; inline where no thread starts).
+// An exception from any part is rethrown here once every thread has joined.
 template <class F>
 void parallel_ranges(int parts, int n, F&& fn) {
+  std::vector<std::exception_ptr> err((size_t)parts);
+  auto run = [&](int t, int j0, int j1) {
+    try {
+      fn(t, j0, j1);
+    } catch (...) {
+      err[(size_t)t] = std::current_exception();
+    }
+  };
   std::vector<std::thread> th;
   for (int t = 1; t < parts; ++t) {
     const int j0 = (int)((int64_t)n * t / parts), j1 = (int)((int64_t)n * (t + 1) / parts);
     try {
-      th.emplace_back([&fn, t, j0, j1] { fn(t, j0, j1); });
+      th.emplace_back(run, t, j0, j1);
     } catch (const std::system_error&) {
-      fn(t, j0, j1);
+      run(t, j0, j1);
     }
   }
-  fn(0, 0, (int)((int64_t)n / parts));
+  run(0, 0, (int)((int64_t)n / parts));
   for (auto& t : th) t.join();
+  for (const auto& e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 // Encode rows x cols x 3 B,G,R bytes; appends the file to `out`.  Each MCU
@@ -982,6 +994,9 @@ int bh_jpeg_encode(const uint8_t* bgr, int rows, int cols, int quality, uint8_t*
   } catch (const std::bad_alloc&) {
     t_jerr = "out of host memory";
     return BH_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {  // e.g. a failed thread join
+    t_jerr = e.what();
+    return BH_ERR_INVALID_ARG;
   }
 }
 

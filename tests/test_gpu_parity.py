@@ -8,6 +8,7 @@ Bars (from the workload definitions):
               between the two renders (nearest-texel lookup of a noise texture);
               phi_end within 1e-5 rad on pixels with the same class and steps.
 """
+import ctypes as C
 import os
 
 import numpy as np
@@ -173,6 +174,44 @@ def test_path_graph_matches_eager(renderer, precision):
             assert err.value.code == abi.BH_ERR_UNSUPPORTED
     finally:
         path.close()
+
+
+def test_misaligned_device_planes(renderer):
+    # bh_render_device into caller planes at odd byte offsets gives the frame
+    # bh_render gives, band-pass pixels included, and writes nothing outside
+    # the planes
+    import torch
+
+    p = scenes.config1(640, 360, frames=1)
+    o = abi.Options.from_buffer_copy(p.options)
+    o.band = 1e-2  # a wide band: many pixels go through the band pass
+    renderer.set_textures(p.textures)
+    print("[case] misaligned device planes 640x360", flush=True)
+    want = renderer.render(p.scene, o, ("rgb", "hit"))
+    npix = 640 * 360
+    buf_rgb = torch.zeros(3 * npix + 8, dtype=torch.uint8, device="cuda")
+    buf_hit = torch.zeros(npix + 8, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    outs = abi.Outputs()
+    outs.rgb = buf_rgb.data_ptr() + 1
+    outs.hit = buf_hit.data_ptr() + 3
+    assert renderer.lib.bh_render_device(renderer.ctx, C.byref(p.scene), C.byref(o),
+                                         C.byref(outs)) == abi.BH_OK
+    assert renderer.lib.bh_synchronize(renderer.ctx) == abi.BH_OK
+    rgb = buf_rgb.cpu().numpy()[1:1 + 3 * npix].reshape(360, 640, 3)
+    hit = buf_hit.cpu().numpy()[3:3 + npix].reshape(360, 640)
+    assert np.array_equal(rgb, want["rgb"]) and np.array_equal(hit, want["hit"])
+    # bytes outside the planes are untouched
+    assert buf_rgb[0].item() == 0 and buf_rgb[1 + 3 * npix:].sum().item() == 0
+    assert buf_hit[:3].sum().item() == 0 and buf_hit[3 + npix:].sum().item() == 0
+
+
+@pytest.mark.parametrize("size", [(317, 179), (322, 181)])
+def test_fp32_odd_widths_match_oracle(renderer, size):
+    # partial 16-pixel tiles at the right edge and bottom
+    w, h = size
+    print(f"[case] fp32 config1 {w}x{h}", flush=True)
+    _check_fp32(renderer, scenes.config1(w, h, frames=1), f"fp32 config1 {w}x{h}")
 
 
 def test_error_paths(renderer):
