@@ -1356,12 +1356,13 @@ int bh_time_flat(bh_ctx* ctx, const bh_flat_scene* sc, int iters, double* avg_ms
 
 // ---- frame ring ------------------------------------------------------------
 struct bh_ring {
-  struct Slot {
+  struct Slot {  // rgb plane then hit plane, one allocation each side
     uint8_t* d_rgb = nullptr;
-    uint8_t* d_hit = nullptr;
+    uint8_t* d_hit = nullptr;  // d_rgb + 3 npix
     uint8_t* h_rgb = nullptr;  // pinned
-    uint8_t* h_hit = nullptr;  // pinned
+    uint8_t* h_hit = nullptr;  // h_rgb + 3 npix
     hipEvent_t start = nullptr, done = nullptr;
+    hipEvent_t rendered = nullptr;  // the frame's kernels ended (copy stream waits on it)
     bool busy = false;  // queued or held by the host
   };
   bh_ctx* ctx = nullptr;
@@ -1374,11 +1375,10 @@ struct bh_ring {
 static void ring_free(bh_ring* r) {
   for (auto& s : r->slots) {
     if (s.d_rgb) (void)hipFree(s.d_rgb);
-    if (s.d_hit) (void)hipFree(s.d_hit);
     if (s.h_rgb) (void)hipHostFree(s.h_rgb);
-    if (s.h_hit) (void)hipHostFree(s.h_hit);
     if (s.start) (void)hipEventDestroy(s.start);
     if (s.done) (void)hipEventDestroy(s.done);
+    if (s.rendered) (void)hipEventDestroy(s.rendered);
   }
   delete r;
 }
@@ -1399,12 +1399,15 @@ int bh_ring_create(bh_ctx* ctx, int width, int height, int depth, bh_ring** out)
     r->slots.resize(depth);
     const size_t npix = (size_t)width * height;
     for (auto& s : r->slots) {
-      hipError_t e = hipMalloc(&s.d_rgb, npix * 3);
-      if (e == hipSuccess) e = hipMalloc(&s.d_hit, npix);
-      if (e == hipSuccess) e = hipHostMalloc(&s.h_rgb, npix * 3, hipHostMallocDefault);
-      if (e == hipSuccess) e = hipHostMalloc(&s.h_hit, npix, hipHostMallocDefault);
+      hipError_t e = hipMalloc(&s.d_rgb, npix * 4);
+      if (e == hipSuccess) e = hipHostMalloc(&s.h_rgb, npix * 4, hipHostMallocDefault);
+      if (e == hipSuccess) {
+        s.d_hit = s.d_rgb + npix * 3;
+        s.h_hit = s.h_rgb + npix * 3;
+      }
       if (e == hipSuccess) e = hipEventCreate(&s.start);
       if (e == hipSuccess) e = hipEventCreate(&s.done);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&s.rendered, hipEventDisableTiming);
       if (e != hipSuccess) {
         ring_free(r);
         return hip_fail(ctx, e, "frame ring allocation");
@@ -1425,14 +1428,19 @@ static bh_ring::Slot* ring_acquire(bh_ring* r, int* idx) {
   return s;
 }
 
+// The frame's copies to pinned memory run on the context's second stream,
+// after the frame's kernels, so the next frame's kernels overlap them.  A
+// slot's device planes are written again only after the host has released
+// the slot, i.e. after its copies completed.
 static int ring_finish(bh_ring* r, bh_ring::Slot* s, int idx, int rc, int* slot) {
   bh_ctx* ctx = r->ctx;
   if (rc) return rc;
   const size_t npix = (size_t)r->W * r->H;
-  hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, hipMemcpyAsync(s->h_rgb, s->d_rgb, npix * 3, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(s->h_hit, s->d_hit, npix, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipEventRecord(s->done, st));
+  hipStream_t cp = ctx->side;
+  HIP_TRY(ctx, hipEventRecord(s->rendered, ctx->stream));
+  HIP_TRY(ctx, hipStreamWaitEvent(cp, s->rendered, 0));
+  HIP_TRY(ctx, hipMemcpyAsync(s->h_rgb, s->d_rgb, npix * 4, hipMemcpyDeviceToHost, cp));
+  HIP_TRY(ctx, hipEventRecord(s->done, cp));
   {
     std::lock_guard<std::mutex> lk(r->m);
     s->busy = true;

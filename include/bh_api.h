@@ -368,7 +368,9 @@ int bh_ring_render(bh_ring* ring, const bh_scene* scene, const bh_options* opt, 
 int bh_ring_render_flat(bh_ring* ring, const bh_flat_scene* scene, int* slot);
 /* Block until slot's frame is on the host; rgb (H*W*3) and hit (H*W) point
  * into the slot's pinned memory until it is released; gpu_ms (optional) =
- * march plus copy time from HIP events. */
+ * time from the start of the frame's march to the end of its copy (HIP
+ * events; the copy runs on a second stream and may queue behind the
+ * previous frame's). */
 int bh_ring_wait(bh_ring* ring, int slot, const uint8_t** rgb, const uint8_t** hit,
                  float* gpu_ms);
 int bh_ring_release(bh_ring* ring, int slot);
