@@ -204,22 +204,10 @@ struct MarchOut {
 // The coeffs_* helpers give G at step n and the kappa of the chord's two
 // ends for each phase (the approach and outbound ones once to_outbound has
 // re-expanded the cubic); step_coeffs picks the phase from n.
-// x = (T)k for the wave-uniform step index 1 <= k < 2^24.  In FP32 the float
-// is assembled from the integer with integer operations only (exponent from
-// the leading bit, mantissa by shifting), so it stays in scalar registers and
-// costs no vector instruction.
-template <typename T> __device__ __forceinline__ T step_x(int k);
-template <> __device__ __forceinline__ float step_x<float>(int k) {
-#ifdef STEPX_SALU
-  const uint32_t u = (uint32_t)k;
-  const uint32_t e = 31u - (uint32_t)__builtin_clz(u);
-  const uint32_t m = (u << (23u - e)) & 0x7FFFFFu;
-  return __builtin_bit_cast(float, ((e + 127u) << 23) | m);
-#else
-  return (float)k;
-#endif
-}
-template <> __device__ __forceinline__ double step_x<double>(int k) { return (double)k; }
+// x = (T)k for the wave-uniform step index k (one conversion per step;
+// assembling the float from k with scalar integer operations was measured
+// slower: a dependent chain of 8 scalar instructions per step).
+template <typename T> __device__ __forceinline__ T step_x(int k) { return (T)k; }
 
 template <typename T>
 __device__ __forceinline__ T step_g(const Lane<T>& L, T x) {
