@@ -743,7 +743,8 @@ int bh_texture_set(bh_ctx* ctx, int slot, const bh_image* img) {
       return fail(ctx, BH_ERR_EMPTY_TEXTURE, "empty texture (Material::SetTexture)");
     if (img->channels != 3 && img->channels != 4)
       return fail(ctx, BH_ERR_INVALID_ARG, "texture channels must be 3 or 4");
-    if ((int64_t)img->rows * img->cols > ((int64_t)1 << 28))
+    if ((int64_t)img->rows * img->cols > ((int64_t)1 << 28) || img->rows > (1 << 24) ||
+        img->cols > (1 << 24))  // sides exact in FP32 texel addressing
       return fail(ctx, BH_ERR_INVALID_ARG, "texture too large");
     (void)hipSetDevice(ctx->device);
     const size_t n = (size_t)img->rows * img->cols;

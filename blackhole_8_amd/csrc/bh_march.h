@@ -395,6 +395,14 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
 done:;
 }
 
+// tex_index for an FP32 coordinate, without the FP64 round trip: the same
+// index for every float v (NaN -> 0; n - 1 < 2^24 is exact in FP32).
+__device__ __forceinline__ int tex_index32(float v, int n) {
+  if (!(v >= 0.f)) return 0;
+  if (v >= (float)(n - 1)) return n - 1;
+  return (int)v;
+}
+
 // Disc texel of a world point q (Rectangle::color, vector_object.h:163-179)
 // with r cos(theta) = s1.v/|s1| and r sin(theta) = |s1 x v|/|s1|.
 __device__ inline uint32_t disc_color32(const DevScene& s, float qx, float qy, float qz) {
@@ -405,7 +413,7 @@ __device__ inline uint32_t disc_color32(const DevScene& s, float qx, float qy, f
   const float cy = s.ts1[2] * vx - s.ts1[0] * vz;
   const float cz = s.ts1[0] * vy - s.ts1[1] * vx;
   const float b = __builtin_amdgcn_sqrtf(cx * cx + cy * cy + cz * cz) * s.tkb;
-  return texel(t, tex_index((double)a, t.cols), tex_index((double)b, t.rows));
+  return texel(t, tex_index32(a, t.cols), tex_index32(b, t.rows));
 }
 
 __device__ inline uint32_t equirect32(const DevTex& t, float dx, float dy, float dz) {
@@ -415,7 +423,7 @@ __device__ inline uint32_t equirect32(const DevTex& t, float dx, float dy, float
   const float lat = asinf(z);
   const float c = (lon + kPi) * (1.0f / (2.0f * kPi)) * (float)t.cols;
   const float r = (kPi / 2.0f - lat) * (1.0f / kPi) * (float)t.rows;
-  return texel(t, tex_index((double)floorf(c), t.cols), tex_index((double)floorf(r), t.rows));
+  return texel(t, tex_index32(floorf(c), t.cols), tex_index32(floorf(r), t.rows));
 }
 
 // Lane set-up in precision T from the FP64 quantities of the ray.
@@ -434,7 +442,10 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, doubl
   L.c3 = (T)(c3 * du2 * du);
   L.du = (T)du;
   L.h = (T)(0.5 * du);
-  L.u0 = (T)s.u0;
+  if constexpr (sizeof(T) == 4)
+    L.u0 = s.u0f;  // (float)u0, set with the scene
+  else
+    L.u0 = (T)s.u0;
   // smallest radius on the march: u = u0 + (nstep - 0.1) du (FP32: the guard
   // keeps a 1e-4 relative margin)
   const float x = s.Rhf * __builtin_fmaf(s.ktf, (float)du, s.u0f);

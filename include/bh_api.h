@@ -209,7 +209,8 @@ int bh_synchronize(bh_ctx* ctx);
 /* Textures — Material::SetTexture (material.h:29-35)                        */
 /* ======================================================================== */
 /* Upload img into slot (0..BH_MAX_TEXTURES-1); replaces the slot's content.
- * Empty image -> BH_ERR_EMPTY_TEXTURE (the reference calls std::terminate). */
+ * Empty image -> BH_ERR_EMPTY_TEXTURE (the reference calls std::terminate);
+ * more than 2^28 texels or a side over 2^24 -> BH_ERR_INVALID_ARG. */
 int bh_texture_set(bh_ctx* ctx, int slot, const bh_image* img);
 int bh_texture_clear(bh_ctx* ctx, int slot);
 
