@@ -73,6 +73,10 @@ def test_fp64_config1_matches_oracle(renderer):
 def _check_fp32(renderer, p, name):
     g = renderer.render_preset(p, ALL)
     r = oracle.render_preset(p, THREADS, ALL)
+    return _check_fp32_outputs(p, g, r, name)
+
+
+def _check_fp32_outputs(p, g, r, name):
     st = compare(g, r)
     _log(name, st)
     # RGB mismatches must all be disc texel straddles
@@ -212,6 +216,37 @@ def test_fp32_odd_widths_match_oracle(renderer, size):
     w, h = size
     print(f"[case] fp32 config1 {w}x{h}", flush=True)
     _check_fp32(renderer, scenes.config1(w, h, frames=1), f"fp32 config1 {w}x{h}")
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP32, abi.PREC_FP64])
+def test_tiny_frames_match_oracle(renderer, precision):
+    # frames of one row or column and fewer pixels than a wave: every lane
+    # outside the frame idles, the rest match the oracle
+    for w, h in ((1, 1), (1, 7), (7, 1), (2, 3), (3, 2), (17, 1)):
+        p = scenes.config1(w, h, frames=1, precision=precision)
+        print(f"[case] {w}x{h} precision {precision}", flush=True)
+        g = renderer.render_preset(p, ALL)
+        r = oracle.render_preset(p, 1, ALL)
+        assert np.array_equal(g["hit"], r["hit"]), (w, h)
+        if precision == abi.PREC_FP64:
+            assert np.array_equal(g["rgb"], r["rgb"]), (w, h)
+            assert np.array_equal(g["steps"], r["steps"]), (w, h)
+        st = compare(g, r)
+        assert st["phi_max_err"] <= (1e-9 if precision == abi.PREC_FP64 else 1e-5), (w, h)
+
+
+def test_fp32_8k_frame_matches_oracle(renderer):
+    # 7680x4320 (33 M rays in one launch, 270 tile rows): the oracle on three
+    # bands of rows, through the shadow, the photon ring and the disc edge
+    p = scenes.config1(7680, 4320, frames=1)
+    print("[case] fp32 config1 7680x4320", flush=True)
+    g = renderer.render_preset(p, ALL)
+    for y0, y1 in ((0, 24), (2100, 2220), (3000, 3048)):
+        r = oracle.render_preset(p, THREADS, ALL, rows=(y0, y1))
+        gs = {k: v[y0:y1] for k, v in g.items()}
+        rs = {k: v[y0:y1] for k, v in r.items()}
+        _check_fp32_outputs(p, gs, rs, f"fp32 8k rows {y0}..{y1}")
+    assert (g["hit"] == abi.HIT_DISC).sum() > 1_000_000
 
 
 def test_error_paths(renderer):
