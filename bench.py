@@ -146,7 +146,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--frames", type=int, default=120)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-frames", type=int, default=3)
+    ap.add_argument("--cpu-frames", type=int, default=12,
+                    help="frames of the path in the CPU baseline sample (about 25 "
+                         "core-seconds at 12)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -277,7 +279,7 @@ def main() -> int:
                "cores": threads, "kind": "port",
                "sample": f"frames {frames} of the 120-frame 1080p path, FP64 oracle "
                          f"(C restatement of the reference march), {threads} threads, "
-                         f"{cpu_model()}"}
+                         f"{cpu_s:.2f} s wall, {cpu_model()}"}
         g0 = r.render(p.with_camera(p.cameras[0]).scene, p.options, want)
         st = compare(g0, ref0)
         accuracy = {k: st[k] for k in ("class_mismatch_frac", "rgb_mismatch_frac",
