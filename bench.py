@@ -194,8 +194,8 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build renders on a single MI355X "
-              "(rays shard by frame or tile across GPUs with no exchange; see DESIGN.md). "
+        print("bench.py measures one GPU only: the renderer runs on a single MI355X "
+              "(rays would shard by frame or tile across GPUs with no exchange). "
               f"Requested --gpus {args.gpus}, WORLD_SIZE {world}.", file=sys.stderr)
         return 2
 
