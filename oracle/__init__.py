@@ -3,8 +3,8 @@
 Python binding of oracle/liboracle.so, the plain-C restatement of the
 reference march (include/blackhole/blackhole_solution_test.cc:161-298).  Only
 tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
-this package, and only as the checker; the product (blackhole_8_amd) never
-does.
+this package, and only to check results against; the product (blackhole_8_amd)
+never does.
 """
 from __future__ import annotations
 

@@ -1,7 +1,7 @@
 /*
  * bh_oracle.c — FP64 CPU oracle for the photon-geodesic march.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * TEST INFRASTRUCTURE ONLY (results are checked against it).  Loaded by tests/, by
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
  * product library.  Built with -O2 -ffp-contract=off so every product and sum
  * rounds exactly once, in the order written, like the reference's scalar code.

@@ -1,7 +1,7 @@
 /*
  * bh_oracle.h — FP64 CPU oracle for the photon-geodesic march.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  Results are checked against it: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
  * never as the thing measured or shipped.  The product path is the HIP
  * library under blackhole_8_amd/, which never links or calls this code.

@@ -97,7 +97,7 @@ def test_calls_without_a_device_or_handle_fail_with_status():
 
 
 def test_oracle_library_is_separate_from_the_product():
-    """The product library never links the oracle (the checker)."""
+    """The product library never links the oracle."""
     out = subprocess.run(["nm", "-D", str(abi.LIB_PATH)], capture_output=True, text=True,
                          check=True).stdout
     assert "or_render" not in out and "or_trace_pixel" not in out
