@@ -29,7 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3        # MI355X FP32 vector peak (published)
 FP32_NONPACKED_TFLOPS = 78.6    # one FMA per lane per clock
-TRAFFIC_FILE = ROOT / "profiles" / "march32_config1_traffic.json"
+TRAFFIC_FILE = ROOT / "profiles" / "march32_disc_orbit_traffic.json"
 
 
 def cpu_model() -> str:
@@ -59,7 +59,7 @@ def batch_timing(reps: int = 3) -> dict:
 
     r = Renderer(0)
     try:
-        items, tex = scenes.config3()
+        items, tex = scenes.uneven_batch()
         r.set_textures(tex)
         rays = sum(s.camera.width * s.camera.height for s, _ in items)
 
@@ -88,7 +88,7 @@ def batch_timing(reps: int = 3) -> dict:
                 "scene_ms_p90": round(float(np.quantile(ms, 0.9)), 4),
                 "scene_ms_max": round(float(ms.max()), 4)}
 
-    return {"workload": "config3: 512 scenes, side log-uniform 32..1024, r_s U[0.5,2], camera "
+    return {"workload": "uneven_batch: 512 scenes, side log-uniform 32..1024, r_s U[0.5,2], camera "
                         "U[6,50] r_s, inclination U[0,90] deg, disc with p 0.5, FP32, one "
                         "batch call (grid, persistent-tile or lane-refill mode, autotuned)",
             "scenes": len(items), "rays": int(rays), "algorithmic_gflop": round(flop / 1e9, 2),
@@ -108,12 +108,12 @@ def adjoint_timing(iters: int = 50) -> dict:
         diff.load_target_textures(r)
         dr = diff.DiffRenderer(r, diff.DiffConfig())
         fwd, fwd_adj = dr.time(diff.FIT_TRUE, iters)
-        pl = scenes.config4_plain(diff.FIT_TRUE, dr.cfg.width, dr.cfg.height, dr.cfg.fov,
+        pl = scenes.fit_plain(diff.FIT_TRUE, dr.cfg.width, dr.cfg.height, dr.cfg.fov,
                                   dr.cfg.r_outer)
         plain, plain_frame, _ = r.time_render(pl.scene, pl.options, iters)
     finally:
         r.close()
-    return {"workload": "config4: 512x512 hole + disc, r_s 1.2, inclination 72 deg, camera at "
+    return {"workload": "fit: 512x512 hole + disc, r_s 1.2, inclination 72 deg, camera at "
                         "18 r_s, r_inner 3.5 r_s, FP32; d/d(r_s, incl, dist, r_inner)",
             "fwd_ms": round(fwd, 5), "fwd_adj_ms": round(fwd_adj, 5),
             "ratio_vs_diff_fwd": round(fwd_adj / fwd, 3),
@@ -203,7 +203,7 @@ def main() -> int:
 
     from blackhole_8_amd import Renderer, abi, flops, scenes
 
-    p = scenes.config1(frames=args.frames)
+    p = scenes.disc_orbit(frames=args.frames)
     W, H, F = p.width, p.height, len(p.cameras)
     rays_per_step = W * H * F
 
@@ -311,7 +311,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "config1: hole + thin disc 3..12 r_s, 1920x1080, camera at "
+        "config": {"workload": "disc_orbit: hole + thin disc 3..12 r_s, 1920x1080, camera at "
                                "20 r_s, inclination 80 deg, 120-frame orbit path as one HIP "
                                "graph, FP32 march + FP64 set-up and photon-sphere band",
                    "width": W, "height": H, "frames": F, "rays_per_step": rays_per_step,

@@ -5,16 +5,20 @@ the oracle see byte-identical inputs.  Texture slot convention used by the
 presets: 0 background, 1 disc, 2.. spheres.
 
 Workloads (lengths in units of r_s = 2M = 1):
-  config0  plain hole, camera at 15 r_s, 60 deg, 320x180, FP64, escape at
-           100 r_s (cap 10000), 2048x1024 equirect noise+checker background
-  config1  hole + thin equatorial disc 3..12 r_s (1024x1024 RGBA radial
-           bands + noise), camera at 20 r_s and inclination 80 deg, 1920x1080,
-           FP32 with the FP64 photon-sphere band, 120-frame orbit at 3 deg/frame
-  config2  hole + 16 textured spheres (radii U[0.3,1.5], centres uniform in
-           the 5..30 r_s shell), config0 background and camera, 1280x720 FP32
-  shipped  the reference demo's own scene (blackhole_solution_test.cc:54-62,
-           103-112,148): 960x540, fov pi/2, camera (-2000,0,400), M=10, disc
-           100..1000 with p3.y = -0100 (octal -64), synthetic disc texture
+  plain_hole    hole alone, camera at 15 r_s, 60 deg, 320x180, FP64, escape at
+                100 r_s (cap 10000), 2048x1024 equirect noise+checker background
+  disc_orbit    hole + thin equatorial disc 3..12 r_s (1024x1024 RGBA radial
+                bands + noise), camera at 20 r_s and inclination 80 deg,
+                1920x1080, FP32 with the FP64 photon-sphere band, 120-frame orbit
+                at 3 deg/frame
+  spheres       hole + 16 textured spheres (radii U[0.3,1.5], centres uniform in
+                the 5..30 r_s shell), plain_hole's background and camera,
+                1280x720 FP32
+  uneven_batch  512 scenes of uneven size, mass, camera and disc in one batch
+  fit_plain     the reference-rule render of the 512x512 inverse-fit scene
+  shipped       the reference demo's own scene (blackhole_solution_test.cc:54-62,
+                103-112,148): 960x540, fov pi/2, camera (-2000,0,400), M=10, disc
+                100..1000 with p3.y = -0100 (octal -64), synthetic disc texture
 """
 from __future__ import annotations
 
@@ -163,31 +167,31 @@ def _scene(cam: abi.Camera, mass: float = 0.5, hole_pos=(0.0, 0.0, 0.0)) -> abi.
 # --------------------------------------------------------------------------
 # the workloads
 # --------------------------------------------------------------------------
-def config0(width: int = 320, height: int = 180) -> Preset:
+def plain_hole(width: int = 320, height: int = 180) -> Preset:
     cam = camera(width, height, math.pi / 3, position=(-15.0, 0.0, 0.0))
     s = _scene(cam)
     s.background = SLOT_BG
     o = options(precision=abi.PREC_FP64, escape=True, r_escape=100.0, step_cap=10000)
-    return Preset("config0", s, o, {SLOT_BG: gen_background()})
+    return Preset("plain_hole", s, o, {SLOT_BG: gen_background()})
 
 
-def config1_camera(frame: int, width: int = 1920, height: int = 1080) -> abi.Camera:
+def disc_orbit_camera(frame: int, width: int = 1920, height: int = 1080) -> abi.Camera:
     return orbit_camera(20.0, 80.0, 3.0 * frame, width, height, math.pi / 3)
 
 
-def config1(width: int = 1920, height: int = 1080, frames: int = 120,
+def disc_orbit(width: int = 1920, height: int = 1080, frames: int = 120,
             precision=abi.PREC_FP32) -> Preset:
-    s = _scene(config1_camera(0, width, height))
+    s = _scene(disc_orbit_camera(0, width, height))
     r_out, r_in = 12.0, 3.0
     s.has_disc = 1
     s.disc = annulus((r_out, r_out, 0.0), (-r_out, r_out, 0.0), (-r_out, -r_out, 0.0),
                      (r_out, -r_out, 0.0), r_out, r_in, SLOT_DISC)
     o = options(precision=precision, escape=False, band=1e-3)
-    cams = [config1_camera(k, width, height) for k in range(frames)]
-    return Preset("config1", s, o, {SLOT_DISC: gen_disc_texture()}, cams)
+    cams = [disc_orbit_camera(k, width, height) for k in range(frames)]
+    return Preset("disc_orbit", s, o, {SLOT_DISC: gen_disc_texture()}, cams)
 
 
-def config2(width: int = 1280, height: int = 720, precision=abi.PREC_FP32,
+def spheres(width: int = 1280, height: int = 720, precision=abi.PREC_FP32,
             n_spheres: int = 16) -> Preset:
     cam = camera(width, height, math.pi / 3, position=(-15.0, 0.0, 0.0))
     s = _scene(cam)
@@ -215,10 +219,10 @@ def config2(width: int = 1280, height: int = 720, precision=abi.PREC_FP32,
         placed += 1
     s.n_spheres = n_spheres
     o = options(precision=precision, escape=True, r_escape=100.0, step_cap=10000)
-    return Preset("config2", s, o, tex)
+    return Preset("spheres", s, o, tex)
 
 
-def config3(n: int = 512, seed: int = SEED_BASE + 0x300):
+def uneven_batch(n: int = 512, seed: int = SEED_BASE + 0x300):
     """Batched uneven renders: n independent hole (+ disc with probability
     1/2) scenes, square side log-uniform in [32, 1024], r_s ~ U[0.5, 2],
     camera at U[6, 50] r_s, inclination U[0, 90] deg, azimuth U[0, 360) deg,
@@ -244,7 +248,7 @@ def config3(n: int = 512, seed: int = SEED_BASE + 0x300):
     return items, {SLOT_DISC: gen_disc_texture()}
 
 
-def config4_plain(theta, width: int = 512, height: int = 512, fov: float = math.radians(70.0),
+def fit_plain(theta, width: int = 512, height: int = 512, fov: float = math.radians(70.0),
                   r_outer: float = 12.0, azim: float = 0.0,
                   precision=abi.PREC_FP32) -> Preset:
     """The inverse-fit scene at theta = (r_s, incl, dist, r_inner) for the
@@ -258,7 +262,7 @@ def config4_plain(theta, width: int = 512, height: int = 512, fov: float = math.
     s.disc = annulus((r_outer, r_outer, 0.0), (-r_outer, r_outer, 0.0), (-r_outer, -r_outer, 0.0),
                      (r_outer, -r_outer, 0.0), r_outer, r_in, SLOT_DISC)
     o = options(precision=precision, escape=False, band=1e-3)
-    return Preset("config4_plain", s, o, {SLOT_DISC: gen_disc_texture()})
+    return Preset("fit_plain", s, o, {SLOT_DISC: gen_disc_texture()})
 
 
 def flat_test(width: int = 800, height: int = 450):

@@ -3,7 +3,7 @@
   python tests/golden/make_golden.py
 
 renders.npz  small frames of the workloads (FP64 reference arithmetic):
-             config0 64x36, config1 96x54 frame 0, the shipped scene 96x54;
+             plain_hole 64x36, disc_orbit 96x54 frame 0, the shipped scene 96x54;
              per frame rgb, hit, steps, phi.
 solveg.npz   SolveG's 20-bisection turning point for a grid of (b, M).
 The oracle is itself pinned by closed forms (tests/test_oracle.py); these
@@ -22,8 +22,8 @@ from blackhole_8_amd import abi, scenes  # noqa: E402
 import oracle  # noqa: E402
 
 FRAMES = {
-    "config0": lambda: scenes.config0(64, 36),
-    "config1": lambda: scenes.config1(96, 54, frames=1, precision=abi.PREC_FP64),
+    "plain_hole": lambda: scenes.plain_hole(64, 36),
+    "disc_orbit": lambda: scenes.disc_orbit(96, 54, frames=1, precision=abi.PREC_FP64),
     "shipped": lambda: scenes.shipped(96, 54),
 }
 

@@ -287,7 +287,7 @@ def test_avi_output_holds_the_library_renders(renderer, tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("depth", [2, 3])
 def test_ring_returns_what_render_returns(renderer, depth):
-    p = scenes.config1(192, 108, frames=6)
+    p = scenes.disc_orbit(192, 108, frames=6)
     renderer.set_textures(p.textures)
     print(f"[case] frame ring depth {depth}", flush=True)
     with Ring(renderer, 192, 108, depth) as ring:
@@ -317,13 +317,13 @@ def test_ring_returns_what_render_returns(renderer, depth):
 def test_ring_errors(renderer):
     from blackhole_8_amd import BHError
 
-    p = scenes.config1(64, 36, frames=1)
+    p = scenes.disc_orbit(64, 36, frames=1)
     renderer.set_textures(p.textures)
     with Ring(renderer, 64, 36, 2) as ring:
         with pytest.raises(BHError) as e:
             ring.wait(0)  # holds no frame yet
         assert e.value.code == abi.BH_ERR_INVALID_ARG
-        big = scenes.config1(65, 36, frames=1)
+        big = scenes.disc_orbit(65, 36, frames=1)
         with pytest.raises(BHError) as e:
             ring.render((big.scene, big.options))
         assert e.value.code == abi.BH_ERR_INVALID_ARG

@@ -22,7 +22,7 @@ THREADS = min(16, os.cpu_count() or 1)
 
 @pytest.fixture(scope="module")
 def c3():
-    return scenes.config3()
+    return scenes.uneven_batch()
 
 
 def _small(items, limit=160, n=48):
@@ -40,7 +40,7 @@ def test_batch_equals_solo_renders(renderer, c3):
     items, tex = c3
     renderer.set_textures(tex)
     sub = _small(items)
-    print(f"[case] batch of {len(sub)} small config3 scenes", flush=True)
+    print(f"[case] batch of {len(sub)} small uneven_batch scenes", flush=True)
     res, ms, st = renderer.render_batch(sub, timing=True)
     assert st.rays == sum(s.camera.width * s.camera.height for s, _ in sub)
     assert st.solo_scenes == 0
@@ -67,10 +67,10 @@ def test_scheduling_knobs_do_not_change_pixels(renderer, c3, tuning):
         assert np.array_equal(a["rgb"], b["rgb"]) and np.array_equal(a["hit"], b["hit"])
 
 
-def test_full_config3_batch(renderer, c3):
+def test_full_uneven_batch_batch(renderer, c3):
     items, tex = c3
     renderer.set_textures(tex)
-    print(f"[case] full config3 batch: {len(items)} scenes", flush=True)
+    print(f"[case] full uneven_batch batch: {len(items)} scenes", flush=True)
     res, ms, st = renderer.render_batch(items, timing=True)
     rays = sum(s.camera.width * s.camera.height for s, _ in items)
     print(f"[batch] {rays} rays in {st.total_ms:.3f} ms ({rays / st.total_ms / 1e3:.0f} Mrays/s), "
@@ -98,7 +98,7 @@ def test_batch_scenes_match_oracle(renderer, c3):
 
 def test_batch_with_fp64_and_sphere_scenes(renderer, c3):
     items, tex = c3
-    p2 = scenes.config2(96, 54)
+    p2 = scenes.spheres(96, 54)
     renderer.set_textures(tex)
     renderer.set_textures(p2.textures)
     s64, o64 = items[0][0], scenes.options(precision=abi.PREC_FP64)

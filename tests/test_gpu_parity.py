@@ -29,13 +29,13 @@ def _log(name, st):
     print(f"[parity] {name}: {summary(st)}", flush=True)
 
 
-def test_fp64_config0_matches_oracle(renderer):
-    p = scenes.config0()
-    print("[case] fp64 config0 320x180", flush=True)
+def test_fp64_plain_hole_matches_oracle(renderer):
+    p = scenes.plain_hole()
+    print("[case] fp64 plain_hole 320x180", flush=True)
     g = renderer.render_preset(p, ALL)
     r = oracle.render_preset(p, THREADS, ALL)
     st = compare(g, r)
-    _log("fp64 config0", st)
+    _log("fp64 plain_hole", st)
     assert st["class_mismatch_frac"] <= 1e-4
     assert st["rgb_mismatch_frac"] <= 1e-4
     assert st["steps_mismatch"] == 0
@@ -58,13 +58,13 @@ def test_fp64_shipped_scene_matches_oracle(renderer):
     assert (r["hit"] == abi.HIT_DISC).sum() > 1000
 
 
-def test_fp64_config1_matches_oracle(renderer):
-    p = scenes.config1(480, 270, frames=1, precision=abi.PREC_FP64)
-    print("[case] fp64 config1 480x270", flush=True)
+def test_fp64_disc_orbit_matches_oracle(renderer):
+    p = scenes.disc_orbit(480, 270, frames=1, precision=abi.PREC_FP64)
+    print("[case] fp64 disc_orbit 480x270", flush=True)
     g = renderer.render_preset(p, ALL)
     r = oracle.render_preset(p, THREADS, ALL)
     st = compare(g, r)
-    _log("fp64 config1", st)
+    _log("fp64 disc_orbit", st)
     assert st["class_mismatch_frac"] <= 1e-4
     assert st["rgb_mismatch_frac"] <= 1e-4
     assert st["phi_max_err"] <= 1e-9
@@ -96,25 +96,25 @@ def _check_fp32_outputs(p, g, r, name):
 
 
 
-def test_fp32_config1_small_matches_oracle(renderer):
-    print("[case] fp32 config1 480x270", flush=True)
-    _check_fp32(renderer, scenes.config1(480, 270, frames=1), "fp32 config1 480x270")
+def test_fp32_disc_orbit_small_matches_oracle(renderer):
+    print("[case] fp32 disc_orbit 480x270", flush=True)
+    _check_fp32(renderer, scenes.disc_orbit(480, 270, frames=1), "fp32 disc_orbit 480x270")
 
 
-def test_fp32_config1_1080p_matches_oracle(renderer):
-    print("[case] fp32 config1 1920x1080 frame 0", flush=True)
-    st = _check_fp32(renderer, scenes.config1(frames=1), "fp32 config1 1080p")
+def test_fp32_disc_orbit_1080p_matches_oracle(renderer):
+    print("[case] fp32 disc_orbit 1920x1080 frame 0", flush=True)
+    st = _check_fp32(renderer, scenes.disc_orbit(frames=1), "fp32 disc_orbit 1080p")
     assert st["rgb_mismatch_frac"] <= 1e-3
 
 
-def test_fp32_config1_other_frames(renderer):
-    base = scenes.config1(640, 360, frames=120)
+def test_fp32_disc_orbit_other_frames(renderer):
+    base = scenes.disc_orbit(640, 360, frames=120)
     for k in (17, 59, 101):
-        print(f"[case] fp32 config1 640x360 frame {k}", flush=True)
-        _check_fp32(renderer, base.with_camera(base.cameras[k]), f"fp32 config1 640x360 frame {k}")
+        print(f"[case] fp32 disc_orbit 640x360 frame {k}", flush=True)
+        _check_fp32(renderer, base.with_camera(base.cameras[k]), f"fp32 disc_orbit 640x360 frame {k}")
 
 
-def _check_config2(renderer, p, name):
+def _check_spheres(renderer, p, name):
     g = renderer.render_preset(p, ALL)
     r = oracle.render_preset(p, THREADS, ALL)
     st = compare(g, r)
@@ -124,25 +124,25 @@ def _check_config2(renderer, p, name):
     return st
 
 
-def test_fp32_config2_spheres_match_oracle(renderer):
-    print("[case] fp32 config2 640x360 (16 spheres, escape, background)", flush=True)
-    st = _check_config2(renderer, scenes.config2(640, 360), "fp32 config2 640x360")
+def test_fp32_spheres_spheres_match_oracle(renderer):
+    print("[case] fp32 spheres 640x360 (16 spheres, escape, background)", flush=True)
+    st = _check_spheres(renderer, scenes.spheres(640, 360), "fp32 spheres 640x360")
     assert st["class_mismatch_frac"] <= 1e-4
     assert st["rgb_mismatch_frac"] <= 2e-3
     assert st["phi_max_err"] <= 1e-5
 
 
-def test_fp32_config2_720p_matches_oracle(renderer):
-    print("[case] fp32 config2 1280x720", flush=True)
-    st = _check_config2(renderer, scenes.config2(), "fp32 config2 1280x720")
+def test_fp32_spheres_720p_matches_oracle(renderer):
+    print("[case] fp32 spheres 1280x720", flush=True)
+    st = _check_spheres(renderer, scenes.spheres(), "fp32 spheres 1280x720")
     assert st["class_mismatch_frac"] <= 1e-4
     assert st["rgb_mismatch_frac"] <= 2e-3
 
 
-def test_fp64_config2_matches_oracle(renderer):
-    print("[case] fp64 config2 320x180", flush=True)
-    st = _check_config2(renderer, scenes.config2(320, 180, precision=abi.PREC_FP64),
-                        "fp64 config2 320x180")
+def test_fp64_spheres_matches_oracle(renderer):
+    print("[case] fp64 spheres 320x180", flush=True)
+    st = _check_spheres(renderer, scenes.spheres(320, 180, precision=abi.PREC_FP64),
+                        "fp64 spheres 320x180")
     assert st["class_mismatch_frac"] <= 1e-4
     assert st["rgb_mismatch_frac"] <= 1e-4
     assert st["phi_max_err"] <= 1e-9
@@ -153,7 +153,7 @@ def test_path_graph_matches_eager(renderer, precision):
     # FP32: one march launch over all frames and one band launch (the frames'
     # band queues must be drained and cleared between replays); FP64: a launch
     # per frame.
-    p = scenes.config1(320, 180, frames=6)
+    p = scenes.disc_orbit(320, 180, frames=6)
     o = scenes.options(precision=precision, band=1e-3)
     renderer.set_textures(p.textures)
     print(f"[case] path graph 6 frames 320x180 precision {precision}", flush=True)
@@ -186,7 +186,7 @@ def test_misaligned_device_planes(renderer):
     # the planes
     import torch
 
-    p = scenes.config1(640, 360, frames=1)
+    p = scenes.disc_orbit(640, 360, frames=1)
     o = abi.Options.from_buffer_copy(p.options)
     o.band = 1e-2  # a wide band: many pixels go through the band pass
     renderer.set_textures(p.textures)
@@ -214,8 +214,8 @@ def test_misaligned_device_planes(renderer):
 def test_fp32_odd_widths_match_oracle(renderer, size):
     # partial 16-pixel tiles at the right edge and bottom
     w, h = size
-    print(f"[case] fp32 config1 {w}x{h}", flush=True)
-    _check_fp32(renderer, scenes.config1(w, h, frames=1), f"fp32 config1 {w}x{h}")
+    print(f"[case] fp32 disc_orbit {w}x{h}", flush=True)
+    _check_fp32(renderer, scenes.disc_orbit(w, h, frames=1), f"fp32 disc_orbit {w}x{h}")
 
 
 @pytest.mark.parametrize("precision", [abi.PREC_FP32, abi.PREC_FP64])
@@ -223,7 +223,7 @@ def test_tiny_frames_match_oracle(renderer, precision):
     # frames of one row or column and fewer pixels than a wave: every lane
     # outside the frame idles, the rest match the oracle
     for w, h in ((1, 1), (1, 7), (7, 1), (2, 3), (3, 2), (17, 1)):
-        p = scenes.config1(w, h, frames=1, precision=precision)
+        p = scenes.disc_orbit(w, h, frames=1, precision=precision)
         print(f"[case] {w}x{h} precision {precision}", flush=True)
         g = renderer.render_preset(p, ALL)
         r = oracle.render_preset(p, 1, ALL)
@@ -238,8 +238,8 @@ def test_tiny_frames_match_oracle(renderer, precision):
 def test_fp32_8k_frame_matches_oracle(renderer):
     # 7680x4320 (33 M rays in one launch, 270 tile rows): the oracle on three
     # bands of rows, through the shadow, the photon ring and the disc edge
-    p = scenes.config1(7680, 4320, frames=1)
-    print("[case] fp32 config1 7680x4320", flush=True)
+    p = scenes.disc_orbit(7680, 4320, frames=1)
+    print("[case] fp32 disc_orbit 7680x4320", flush=True)
     g = renderer.render_preset(p, ALL)
     for y0, y1 in ((0, 24), (2100, 2220), (3000, 3048)):
         r = oracle.render_preset(p, THREADS, ALL, rows=(y0, y1))
@@ -250,7 +250,7 @@ def test_fp32_8k_frame_matches_oracle(renderer):
 
 
 def test_error_paths(renderer):
-    p = scenes.config1(64, 36, frames=1)
+    p = scenes.disc_orbit(64, 36, frames=1)
     renderer.set_textures(p.textures)
     s = abi.Scene.from_buffer_copy(p.scene)
     assert renderer.lib.bh_texture_clear(renderer.ctx, 63) == abi.BH_OK

@@ -50,7 +50,7 @@ def test_camera_default_frame_and_look_at():
     np.testing.assert_allclose(np.array([c.axis_x, c.axis_y, c.axis_z]),
                                [[1, 0, 0], [0, -1, 0], [0, 0, -1]], atol=1e-15)
     # orbit camera: looks at the hole, right-handed like the default frame
-    c = scenes.config1_camera(7)
+    c = scenes.disc_orbit_camera(7)
     x, y, z = (np.array(v) for v in (c.axis_x, c.axis_y, c.axis_z))
     pos = np.array(c.position)
     np.testing.assert_allclose(x, -pos / np.linalg.norm(pos), atol=1e-15)
@@ -102,8 +102,8 @@ def test_disc_texture_bands():
     assert (t[band == 1][:, 2] >= 190).all()
 
 
-def test_config2_spheres_placement():
-    p = scenes.config2(64, 36)
+def test_spheres_spheres_placement():
+    p = scenes.spheres(64, 36)
     s = p.scene
     assert s.n_spheres == 16
     cam = np.array(s.camera.position)

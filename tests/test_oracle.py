@@ -127,7 +127,7 @@ def test_inverse_3x3_matches_linear_algebra():
 # Orbit-plane frame (blackhole_solution_test.cc:167-193)
 # ---------------------------------------------------------------------------
 def test_ray_frame_identities():
-    p = scenes.config1(64, 36, frames=1)
+    p = scenes.disc_orbit(64, 36, frames=1)
     s, o = p.scene, p.options
     B = np.array(s.hole.position)
     C0 = np.array(s.camera.position)
@@ -159,7 +159,7 @@ def test_ray_frame_identities():
 
 
 def test_degenerate_ray_through_the_hole_is_captured_black():
-    p = scenes.config0()
+    p = scenes.plain_hole()
     px = oracle.trace_pixel(p.scene, p.options, p.textures, 160, 90)
     assert px["hit"] == abi.HIT_HORIZON and not px["rgb"].any()
 
@@ -328,7 +328,7 @@ def b_map(p):
 
 
 def test_plain_hole_frame_structure():
-    p = scenes.config0(160, 90)
+    p = scenes.plain_hole(160, 90)
     o = oracle.render_preset(p, 8, ("rgb", "hit", "steps"))
     hit, steps = o["hit"], o["steps"]
     b = b_map(p)
@@ -349,7 +349,7 @@ def test_plain_hole_frame_structure():
 
 
 def test_disc_frame_reference_mode_steps():
-    p = scenes.config1(96, 54, frames=1, precision=abi.PREC_FP64)
+    p = scenes.disc_orbit(96, 54, frames=1, precision=abi.PREC_FP64)
     o = oracle.render_preset(p, 8, ("rgb", "hit", "steps"))
     hit, steps = o["hit"], o["steps"]
     # reference rule: an unhit ray marches 2 nstep - 1 steps and stays black
