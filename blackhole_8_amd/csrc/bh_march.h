@@ -297,9 +297,9 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
 }
 
 // The march of one ray in precision T.  first_aa is the angle of the camera
-// in the (e1, e2) frame: 0 when Y0 > 0, pi when Y0 < 0 (quirk Q2: the
-// reference's phi0 = atan(b/Y0) then points away from the camera, and only
-// the first chord starts at the camera itself).
+// in the (e1, e2) frame: 0 when Y0 > 0, pi when Y0 < 0 (the reference's
+// phi0 = atan(b/Y0) then points away from the camera, and only the first
+// chord starts at the camera itself).
 //
 // Every lane of a wave is at the same step, so the loops run wave-uniform
 // trip counts and keep no per-lane flag: a ray that has ended is disarmed by
@@ -619,7 +619,8 @@ __device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t i
 }
 
 __device__ inline void degenerate_out(MarchOut& mo) {
-  // the pixel line passes through B (quirk Q13): captured / black
+  // the pixel line passes through B: the reference marches NaNs; defined
+  // here as captured / black
   mo.hit.cls = BH_HIT_HORIZON;
   mo.hit.obj = 0;
   mo.hit.ang = 0.0;

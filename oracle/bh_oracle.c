@@ -186,8 +186,8 @@ int or_ray_setup(const bh_scene* s, const bh_options* o, int x, int y, or_ray* r
   v3 Fp = matvec(ray->Rinv, F);
   memcpy(ray->Fp, Fp.v, sizeof Fp.v);
 
-  /* Degenerate frame (the pixel line passes through B, quirk Q13): the
-   * reference marches NaNs; defined here as captured / black. */
+  /* Degenerate frame (the pixel line passes through B): the reference
+   * marches NaNs; defined here as captured / black. */
   ray->degenerate = !ok || dot(zv, zv) == 0.0;
 
   double b = Fp.v[2];

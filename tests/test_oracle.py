@@ -178,8 +178,9 @@ def exact_sweep(b, M, u0, u1):
 @pytest.mark.parametrize("b", [2.7, 3.0, 5.0, 12.0, 18.0])
 def test_inbound_sweep_converges_to_the_integral(b):
     """As nstep grows, phi after the final approach step tends to
-    int_{u0}^{u_p} du/sqrt(G); the quirks (first weight 0, last step 0.9 du
-    weighted du/2, stop 0.1 du short of u_p) cost O(sqrt(du))."""
+    int_{u0}^{u_p} du/sqrt(G); the reference rule's departures from the
+    plain trapezoid (first weight 0, last step 0.9 du weighted du/2, stop
+    0.1 du short of u_p) cost O(sqrt(du))."""
     M, u0 = 0.5, 1 / 20
     up = L.or_solve_g(b, M, abi.ROOT_EXACT)
     exact = exact_sweep(b, M, u0, up)
