@@ -97,6 +97,58 @@ def batch_timing(reps: int = 3) -> dict:
                                                    for f, _ in best._fields_})}
 
 
+def cpu_side_by_side(iters: int = 20) -> dict:
+    """The other workloads' GPU time beside the FP64 oracle on the host's cores
+    for the same frame, and the oracle on one thread (the reference's serial
+    loop) for one frame of the 1080p disc orbit."""
+    import oracle
+    from blackhole_8_amd import Renderer, diff, scenes
+
+    threads = host_threads()
+
+    def timed(fn):
+        t = time.perf_counter()
+        fn()
+        return time.perf_counter() - t
+
+    out = {"cpu": cpu_model(), "cpu_threads": threads}
+    r = Renderer(0)
+    try:
+        for name, p, what in (
+                ("plain_hole", scenes.plain_hole(),
+                 "hole alone, 320x180, FP64 reference arithmetic, escape at 100 r_s"),
+                ("spheres", scenes.spheres(), "hole + 16 textured spheres, 1280x720, FP32")):
+            r.set_textures(p.textures)
+            _, gpu_ms, _ = r.time_render(p.scene, p.options, iters)
+            rays = p.scene.camera.width * p.scene.camera.height
+            cpu_s = timed(lambda: oracle.render_preset(p, threads))
+            out[name] = {"workload": what, "rays": rays, "gpu_ms": round(gpu_ms, 4),
+                         "cpu_ms": round(cpu_s * 1e3, 2),
+                         "gpu_mrays_per_s": round(rays / gpu_ms / 1e3, 1),
+                         "cpu_mrays_per_s": round(rays / cpu_s / 1e6, 3)}
+        tex = diff.load_target_textures(r)
+        dr = diff.DiffRenderer(r, diff.DiffConfig())
+        fwd_ms, _ = dr.time(diff.FIT_TRUE, iters)
+        rays = dr.cfg.width * dr.cfg.height
+        cpu_s = timed(lambda: oracle.render_diff(dr.cfg.params(diff.FIT_TRUE), tex, threads))
+        out["fit_forward"] = {"workload": "differentiable forward of the 512x512 inverse-fit "
+                                          "scene, FP32 (oracle in FP64)",
+                              "rays": rays, "gpu_ms": round(fwd_ms, 4),
+                              "cpu_ms": round(cpu_s * 1e3, 2),
+                              "gpu_mrays_per_s": round(rays / fwd_ms / 1e3, 1),
+                              "cpu_mrays_per_s": round(rays / cpu_s / 1e6, 3)}
+    finally:
+        r.close()
+    p = scenes.disc_orbit(frames=1)
+    rays = p.scene.camera.width * p.scene.camera.height
+    cpu_s = timed(lambda: oracle.render_preset(p, 1))
+    out["disc_orbit_one_thread"] = {"workload": "frame 0 of the 1080p disc orbit, FP64 oracle "
+                                                "on one thread",
+                                    "rays": rays, "cpu_ms": round(cpu_s * 1e3, 1),
+                                    "cpu_mrays_per_s": round(rays / cpu_s / 1e6, 3)}
+    return out
+
+
 def adjoint_timing(iters: int = 50) -> dict:
     """Forward+adjoint against the forward on the 512x512 inverse-fit scene
     (FP32): the differentiable forward, the fused forward+adjoint launch, and
@@ -160,6 +212,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the 512x512 forward / forward+adjoint and exit")
+    ap.add_argument("--side-only", action="store_true",
+                    help="print only the GPU-beside-CPU timings of the other workloads")
     ap.add_argument("--batch-only", action="store_true",
                     help="only time the 512-scene batch and exit")
     ap.add_argument("--no-flat", action="store_true")
@@ -190,6 +244,9 @@ def main() -> int:
         return 0
     if args.flat_only:
         print(json.dumps({"flat": flat_timing()}))
+        return 0
+    if args.side_only:
+        print(json.dumps({"cpu_side_by_side": cpu_side_by_side()}))
         return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -294,6 +351,7 @@ def main() -> int:
     path.close()
     r.close()
 
+    side = None if args.no_cpu_baseline else cpu_side_by_side()
     adjoint = None if args.no_adjoint else adjoint_timing()
     batch = None if args.no_batch else batch_timing()
     flat = None if args.no_flat else flat_timing()
@@ -319,6 +377,7 @@ def main() -> int:
                    "device": dev["name"], "cu": dev["cu"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_side_by_side": side,
         "accuracy_frame0": accuracy,
         "adjoint": adjoint,
         "batch": batch,
