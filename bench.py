@@ -289,10 +289,12 @@ def main() -> int:
     # per-pixel step counts and hit classes.
     march_ms, band_ms = path.time(iters=3)
     work, band = 0.0, 0
+    alg_bytes = 0  # framebuffer (3 B rgb + 1 B class) and one 4-byte texel per disc hit
     for k in range(F):
         sc = p.with_camera(p.cameras[k]).scene
         out = r.render(sc, p.options, ("hit", "steps"))
         work += flops.frame_flops(out["steps"], out["hit"], True)
+        alg_bytes += W * H * 4 + 4 * int((out["hit"] == abi.HIT_DISC).sum())
     single = [r.time_render(p.with_camera(p.cameras[k]).scene, p.options, iters=4)
               for k in (0, F // 2)]
     achieved = work / (march_ms * 1e-3) / 1e12
@@ -312,6 +314,10 @@ def main() -> int:
         "frac_nonpacked": round(achieved / FP32_NONPACKED_TFLOPS, 4),
         "traffic": traffic,
         "traffic_per": "launch (all frames of the path)",
+        "traffic_gbps": round(traffic / (march_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "algorithmic_bytes": alg_bytes,
+        "algorithmic_gbps": round(alg_bytes / (march_ms * 1e-3) / 1e9, 1),
+        "hbm_peak_gbps": 8000,
         "launch_ms": round(march_ms, 4),
         "avg_kernel_ms": round(march_ms / F, 5),
         "band_launch_ms": round(band_ms, 4),
