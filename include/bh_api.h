@@ -196,7 +196,9 @@ typedef struct bh_ctx bh_ctx;
  * object_manager.h:30-33).  device = HIP ordinal. */
 int bh_ctx_create(int device, bh_ctx** out);
 void bh_ctx_destroy(bh_ctx* ctx);
-/* Message of the last failing call on ctx (static string when ctx is NULL). */
+/* Message of the last failing call on ctx; with ctx NULL, the message of the
+ * last failing call on this thread that had no context (bh_ctx_create, ring
+ * waits and releases), or "" if none. */
 const char* bh_last_error(const bh_ctx* ctx);
 /* Library version and the device name / CU count the context runs on. */
 int bh_version(void);
