@@ -56,7 +56,7 @@ enum {
 /* ---- option enums ------------------------------------------------------ */
 enum {
   BH_PREC_FP64 = 0, /* every step in FP64, reference operation order */
-  BH_PREC_FP32 = 1  /* FP32 march, FP64 set-up, FP64 photon-sphere band and hits */
+  BH_PREC_FP32 = 1  /* FP32 march; FP64 set-up, swept angle and photon-sphere band */
 };
 enum { BH_FILTER_NEAREST = 0, BH_FILTER_BILINEAR = 1 };
 enum {
@@ -356,8 +356,9 @@ int bh_time_flat(bh_ctx* ctx, const bh_flat_scene* scene, int iters, double* avg
 /* ======================================================================== */
 /* depth (2..8) slots, each a device rgb/hit frame and a pinned host copy.
  * A render call queues the march of one frame into the next slot and the
- * copy of its planes to the slot's pinned memory, and returns at once
- * (waiting first while that slot is still held by the host).  wait gives the
+ * copy of its planes to the slot's pinned memory (on a second stream, so it
+ * overlaps the next frame's march), and returns at once (waiting first while
+ * that slot is still held by the host).  wait gives the
  * host planes once the frame is there; release hands the slot back.  Every
  * pixel is written by the march, so no clearing pass runs between frames.
  * bh_ring_wait and bh_ring_release may run on another thread than the render
