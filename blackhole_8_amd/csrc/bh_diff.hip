@@ -12,8 +12,9 @@
 // or full coverage.  One lane per ray.
 //
 // Adjoint (reverse mode, fused with the forward in one kernel):
-//   pass 1   forward march; each disc crossing records (psi_{j-1}, psi_j, T_k)
-//            in LDS and sets bit j of a chord mask;
+//   pass 1   forward march; each disc crossing records its chord's angles
+//            (FP32: psi_j alone, in fixed point), the transmittance T_k and
+//            four prefix sums in LDS and sets bit j of a chord mask;
 //   pass 2   reverse over the crossing records only.  The angle psi_m =
 //            h sum_{i <= m} (w_{i-1} + w_i) makes w_i enter every later angle,
 //            so the adjoint of w_i is h (A_i + A_{i+1}) with A_i the sum of
@@ -39,6 +40,8 @@
 // the gradient is deterministic.
 #include "bh_internal.h"
 #include "bh_math.h"
+
+#include <type_traits>
 
 namespace bh {
 namespace {
@@ -399,11 +402,22 @@ template <typename T> struct Acc {
 };
 
 // Per-lane crossing records in LDS, strided by the block size so that a
-// wave's lanes hit consecutive words.
+// wave's lanes hit consecutive words.  FP64 keeps psi_{j-1} and psi_j of
+// crossing k at psi[2k kThreads] and psi[(2k + 1) kThreads].  FP32 keeps
+// psi_j alone at psi[k kThreads], as a 32-bit fixed-point angle in units of
+// 2^-26 rad (error <= 2^-27 rad for psi_j < 64 rad; a crossing at a larger
+// angle is not recorded and counts as an overflow), and the adjoint
+// recovers psi_{j-1} = psi_j - (w_{j-1} + w_j) h from the step's closed
+// form.  That is 24 bytes a record, so three 256-lane blocks fit in a CU's
+// LDS.
+constexpr double kPsiUnit = 0x1p26;  // fixed-point units per radian
+constexpr double kPsiMax = 64.0;     // 2^32 units
 template <typename T> struct Rec {
-  double* psi;  // psi_{j-1} at psi[2k kThreads], psi_j at psi[(2k + 1) kThreads]
-  T* tr;        // T_k at tr[k kThreads]
-  T* ps;        // prefix sums through step j at ps[(4k + q) kThreads], q < 4
+  using Psi = std::conditional_t<sizeof(T) == 4, uint32_t, double>;
+  static constexpr int kPsi = sizeof(T) == 4 ? 1 : 2;  // angles per record
+  Psi* psi;
+  T* tr;  // T_k at tr[k kThreads]
+  T* ps;  // prefix sums through step j at ps[(4k + q) kThreads], q < 4
 };
 
 // Per-step factors of dw_j (w = G^(-1/2), dw = -w^3 dG / 2 with
@@ -493,9 +507,14 @@ __device__ inline void diff_march(const DiffScene& s, const DRay<T>& d, Rec<T> r
         T c[3];
         disc_texel<T, false>(s, q, c, nullptr, nullptr);
         if (RECORD) {
-          if (a.n < kMaxRec) {
-            rec.psi[(2 * a.n + 0) * kThreads] = psi_prev;
-            rec.psi[(2 * a.n + 1) * kThreads] = psi;
+          // psi >= 0; the negated test also catches a NaN angle
+          if (a.n < kMaxRec && (Rec<T>::kPsi == 2 || psi < kPsiMax)) {
+            if constexpr (Rec<T>::kPsi == 1) {
+              rec.psi[a.n * kThreads] = (uint32_t)rint(psi * kPsiUnit);
+            } else {
+              rec.psi[(2 * a.n + 0) * kThreads] = psi_prev;
+              rec.psi[(2 * a.n + 1) * kThreads] = psi;
+            }
             rec.tr[a.n * kThreads] = a.tr;
 #pragma unroll
             for (int q = 0; q < 4; ++q) rec.ps[(4 * a.n + q) * kThreads] = sx[q];
@@ -551,11 +570,11 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v) {
 }
 
 template <typename T, bool ADJ>
-__global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float* __restrict__ grad_rgb,
+__global__ __launch_bounds__(kThreads, (ADJ && sizeof(T) == 4) ? 3 : 1) void diff_kernel(DiffScene s, const float* __restrict__ grad_rgb,
                                                         float* __restrict__ rgb_out,
                                                         uint8_t* __restrict__ hit_out,
                                                         double* __restrict__ partial) {
-  __shared__ double rec_psi[ADJ ? 2 * kMaxRec * kThreads : 1];
+  __shared__ typename Rec<T>::Psi rec_psi[ADJ ? Rec<T>::kPsi * kMaxRec * kThreads : 1];
   __shared__ T rec_tr[ADJ ? kMaxRec * kThreads : 1];
   __shared__ T rec_ps[ADJ ? 4 * kMaxRec * kThreads : 1];
   __shared__ double red[ADJ ? kPartial * (kThreads / 64) : 1];
@@ -579,8 +598,8 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
       rgb_out[3 * idx + 2] = (float)a.rgb[2];
     }
     if (hit_out) hit_out[idx] = (uint8_t)a.cls;
+    if (ADJ) g_ov = a.overflow ? 1.0 : 0.0;
     if (ADJ && a.mask) {
-      g_ov = a.overflow ? 1.0 : 0.0;
       const T gb[3] = {(T)grad_rgb[3 * idx], (T)grad_rgb[3 * idx + 1], (T)grad_rgb[3 * idx + 2]};
       const int n = s.nstep;
       T B[3] = {(T)0, (T)0, (T)0};
@@ -590,13 +609,21 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
       T wx[4] = {(T)0, (T)0, (T)0, (T)0};
       const T c3x3 = (T)3 * d.c3, c2x2 = (T)2 * d.c2;
       const T ih = (T)1 / d.h;
-      int k = (a.n < kMaxRec ? a.n : kMaxRec) - 1;
+      int k = __popcll(a.mask) - 1;  // records 0 .. k, one per mask bit
       uint64_t mask = a.mask;
       while (mask) {
         const int j = 63 - __builtin_clzll(mask);
         mask &= ~(1ull << j);
-        const double ps_a = rec.psi[(2 * k + 0) * kThreads];
-        const double ps_b = rec.psi[(2 * k + 1) * kThreads];
+        const StepX<T> xj = step_x(d, j, n, c3x3, c2x2);
+        const StepX<T> xi = step_x(d, j > 1 ? j - 1 : 1, n, c3x3, c2x2);  // used for j > 1
+        double ps_a, ps_b;
+        if constexpr (Rec<T>::kPsi == 1) {
+          ps_b = (double)rec.psi[k * kThreads] * (1.0 / kPsiUnit);
+          ps_a = ps_b - (double)((xi.w + xj.w) * d.h);
+        } else {
+          ps_a = rec.psi[(2 * k + 0) * kThreads];
+          ps_b = rec.psi[(2 * k + 1) * kThreads];
+        }
         const T Tk = rec.tr[k * kThreads];
         T S0[4];
 #pragma unroll
@@ -654,7 +681,6 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
         point_adjoint(d, P2b, cb, sb, rb, e1b, e2b, &phb, &ub);
         u0b += ub;
         dub = fma_t(kappa<T>(j, n), ub, dub);
-        const StepX<T> xj = step_x(d, j, n, c3x3, c2x2);
         T S1[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
@@ -670,7 +696,6 @@ __global__ __launch_bounds__(kThreads) void diff_kernel(DiffScene s, const float
           point_adjoint(d, P1b, ca, sa, ra, e1b, e2b, &phb, &ub);
           u0b += ub;
           dub = fma_t(kappa<T>(j - 1, n), ub, dub);
-          const StepX<T> xi = step_x(d, j - 1, n, c3x3, c2x2);
 #pragma unroll
           for (int m = 0; m < 4; ++m) wx[m] = fma_t(phb, (T)2 * S1[m] - xi.x[m], wx[m]);
           hb = fma_t(phb, (T)ps_a * ih, hb);

@@ -150,15 +150,17 @@ def _pool(img: torch.Tensor, k: int) -> torch.Tensor:
 
 
 def fit(dr: DiffRenderer, target: torch.Tensor, theta0, iters: int = 200, lr: float = 0.2,
-        pyramid=(16, 8, 4, 2, 1), log=None):
+        pyramid=(32, 16, 8, 4, 2, 1), log=None):
     """Adam on the mean squared error to target, coarse to fine: the
     iterations are split evenly over the box-filter sizes in `pyramid`, and
-    each stage compares k x k averages of the render and the target (the
-    noisy disc texture makes the full-resolution loss rough in r_s and the
-    distance, so the coarse stages find the basin and the last stage the
-    minimum).  The learning rate is annealed on a cosine to 5% over the
-    iterations; theta = theta0 + scale * x, x from 0.  Returns (theta,
-    history of (full-resolution mse, theta))."""
+    the loss of stage i is the sum over pyramid[:i + 1] of the mean squared
+    error between k x k averages of the render and of the target.  The noisy
+    disc texture makes the full-resolution loss rough in r_s and the
+    distance: the coarse levels find the basin and, kept in the sum, hold
+    the finer stages in it while they find the minimum.  The learning rate
+    is annealed on a cosine to 5% over the iterations; theta = theta0 +
+    scale * x, x from 0.  Returns (theta, history of (full-resolution mse,
+    theta))."""
     scale = torch.tensor(FIT_SCALE, dtype=torch.float64)
     base = torch.tensor([float(v) for v in theta0], dtype=torch.float64)
     x = torch.zeros(4, dtype=torch.float64, requires_grad=True)
@@ -167,11 +169,11 @@ def fit(dr: DiffRenderer, target: torch.Tensor, theta0, iters: int = 200, lr: fl
     targets = {k: _pool(target, k) for k in pyramid}
     hist = []
     for it in range(iters):
-        k = pyramid[min(len(pyramid) - 1, it * len(pyramid) // iters)]
+        stage = min(len(pyramid) - 1, it * len(pyramid) // iters)
         opt.zero_grad()
         theta = base + scale * x
         img = render(theta, dr)
-        loss = ((_pool(img, k) - targets[k]) ** 2).mean()
+        loss = sum(((_pool(img, k) - targets[k]) ** 2).mean() for k in pyramid[:stage + 1])
         loss.backward()
         mse = float(((img.detach() - target) ** 2).mean())
         hist.append((mse, theta.detach().tolist()))

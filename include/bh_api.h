@@ -326,10 +326,12 @@ int bh_diff_render_device(bh_ctx* ctx, const bh_diff_params* p, int precision,
 /* Forward and adjoint in one launch: given dL/drgb (H*W*3 float), writes
  * dL/d(r_s, incl, dist, r_inner) to grad_out[4] (4 doubles).  rgb_out and
  * hit_out (optional) receive the forward render.  A ray records at most
- * BH_DIFF_MAX_CROSSINGS disc crossings for its adjoint; overflow_out
- * (optional, one uint32) receives the number of rays that had more, whose
- * later crossings are left out of the gradient.  All pointers are device
- * pointers; asynchronous on the context stream. */
+ * BH_DIFF_MAX_CROSSINGS disc crossings for its adjoint, and in FP32 only
+ * crossings at an orbit angle below 64 rad (ten turns and more; the angle is
+ * kept in 32-bit fixed point); overflow_out (optional, one uint32) receives
+ * the number of rays that had crossings beyond either limit, which are left
+ * out of the gradient.  All pointers are device pointers; asynchronous on
+ * the context stream. */
 int bh_diff_adjoint_device(bh_ctx* ctx, const bh_diff_params* p, int precision,
                            const float* grad_rgb, float* rgb_out, uint8_t* hit_out,
                            double* grad_out, uint32_t* overflow_out);
