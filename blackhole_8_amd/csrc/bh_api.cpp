@@ -281,15 +281,17 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
     d->sphf[i][3] = (float)d->sr[i];
   }
   d->fast_ok = d->precision == BH_PREC_FP32 && d->filter == BH_FILTER_NEAREST &&
-               d->nstep <= 4096 &&
+               d->nstep <= 4096 && d->u0 < d->Uc &&
                (!d->has_disc || (d->dv0[0] == d->B[0] && d->dv0[1] == d->B[1] &&
                                  d->dv0[2] == d->B[2]));
   return BH_OK;
 }
 
 // The FP32 event march assumes the disc is centred on the hole (its plane
-// passes through B, so every crossing is an angle in the orbit plane); other
-// scenes take the FP64 march.
+// passes through B, so every crossing is an angle in the orbit plane) and a
+// camera outside the photon sphere (r0 > 3M: then every ray's march runs
+// inward, du > 0, and its swept angle grows, which the event tests rely on);
+// other scenes take the FP64 march.
 bool fp32_eligible(const DevScene& d) { return d.fast_ok != 0; }
 
 // FP32 march then the FP64 band kernel, or the FP64 reference march.

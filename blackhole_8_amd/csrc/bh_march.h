@@ -24,8 +24,9 @@ namespace bh {
 //   G(u) = G(l) + s [2 l d + s (1 - 2 d) - 2 M s^2],  d = 1 - 3 M l,
 // a cubic in s whose terms do not cancel away from the photon sphere; the
 // coefficients come from FP64.  The reference's u_k = u0 + kappa_k du gives
-// s_k = sigma_k du with sigma_k = nstep - kappa_k exact, so a step is one
-// multiply, three FMAs, one rsqrt and the trapezoid update of a (FP64 sum).
+// s_k = sigma_k du with sigma_k = nstep - kappa_k exact, so a step is three
+// FMAs, one rsqrt and the trapezoid update of a (FP64 sum); in FP32 the
+// cubic carries the factor 1/h^2 of the trapezoid, so the update is one add.
 //
 // Events.  A chord can only reach the disc plane if sign(n . (P - B)) =
 // sign(alpha cos a + beta sin a) changes, i.e. a passes a crossing angle
@@ -42,7 +43,7 @@ struct Lane {
   //   approach G = c0 after to_outbound                  (sigma = 0.1)
   //   outbound x = j = n - nstep                         (sigma = 0.1 + j)
   // to_outbound re-expands the cubic about sigma = 0.1 once the inbound steps
-  // are done, so a step needs no per-lane s.
+  // are done, so a step needs no per-lane s.  FP32 holds G / h^2.
   T c0, c1, c2, c3;
   T du, h;            // du and du/2
   T u0;               // 1/|C - B|
@@ -261,7 +262,7 @@ __device__ __forceinline__ void step_coeffs(int n, int nstep, const Lane<T>& L, 
 
 // Running state of one ray's march.
 template <typename T> struct MarchState {
-  T w;       // w of the previous step
+  T w;       // w of the previous step (FP32: w h)
   double a;  // swept angle
   T d;       // last increment
 };
@@ -276,8 +277,11 @@ template <typename T, typename S, bool SPH = false>
 __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>& st,
                                            double first_aa, T G, T ka, T kb, bool force,
                                            HitRec& hit) {
-  const T wn = rsq<T>(G);
-  st.d = (st.w + wn) * L.h;
+  const T wn = rsq<T>(G);  // FP32: w h (setup_lane scales the cubic)
+  if constexpr (sizeof(T) == 4)
+    st.d = st.w + wn;
+  else
+    st.d = (st.w + wn) * L.h;
   st.w = wn;
   st.a += (double)st.d;
   const bool eh = (float)st.d > L.phi_safe;
@@ -436,10 +440,34 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, doubl
   const double c0 = __builtin_fma(l * l, __builtin_fma(s.M2, l, -1.0), ib2);
   const double c1 = 2.0 * l * dl, c2 = __builtin_fma(-2.0, dl, 1.0), c3 = -s.M2;
   const double du2 = du * du;
-  L.c0 = (T)c0;
-  L.c1 = (T)(c1 * du);
-  L.c2 = (T)(c2 * du2);
-  L.c3 = (T)(c3 * du2 * du);
+  if constexpr (sizeof(T) == 4) {
+    // FP32: the cubic is scaled by 1/h^2 = 4/du^2, so that its rsqrt is
+    // w h, the trapezoid half-step, and a step adds two of them with no
+    // multiply.  du > 0 on every FP32 ray (the camera is outside the photon
+    // sphere) save a ray leaving the camera at a right angle to the hole,
+    // whose turning point can round onto u0: it marches nowhere (G = +inf,
+    // every increment 0), as it does unscaled with du = 0 (and, to within
+    // 1e-30 w per step, with du below 1e-30).
+    if (du > 1e-30) {
+      // 1/du: FP32 reciprocal and one FP64 Newton step (relative error
+      // ~1e-14; an FP64 division here costs more than the multiply saved)
+      double r = (double)__builtin_amdgcn_rcpf((float)du);
+      r = r * __builtin_fma(-du, r, 2.0);
+      const double r4 = 4.0 * r;
+      L.c0 = (T)(c0 * (r4 * r));
+      L.c1 = (T)(c1 * r4);
+      L.c2 = (T)(c2 * 4.0);
+      L.c3 = (T)(c3 * (4.0 * du));
+    } else {
+      L.c0 = (T)__builtin_inf();
+      L.c1 = L.c2 = L.c3 = (T)0;
+    }
+  } else {
+    L.c0 = (T)c0;
+    L.c1 = (T)(c1 * du);
+    L.c2 = (T)(c2 * du2);
+    L.c3 = (T)(c3 * du2 * du);
+  }
   L.du = (T)du;
   L.h = (T)(0.5 * du);
   if constexpr (sizeof(T) == 4)

@@ -9,6 +9,7 @@ Bars (from the workload definitions):
               phi_end within 1e-5 rad on pixels with the same class and steps.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -233,6 +234,27 @@ def test_tiny_frames_match_oracle(renderer, precision):
             assert np.array_equal(g["steps"], r["steps"]), (w, h)
         st = compare(g, r)
         assert st["phi_max_err"] <= (1e-9 if precision == abi.PREC_FP64 else 1e-5), (w, h)
+
+
+def test_camera_inside_photon_sphere_takes_the_fp64_march(renderer):
+    # r0 = 1.3 < 3M = 1.5: every ray's march runs outward (du < 0) and its
+    # swept angle shrinks, which the FP32 event tests do not allow for, so an
+    # FP32 request is served by the FP64 reference march: both precisions
+    # give the same bytes in every plane, and the oracle's classes and angles
+    print("[case] camera inside the photon sphere 160x90", flush=True)
+    cam = scenes.orbit_camera(1.3, 80.0, 0.0, 160, 90, math.pi / 3)
+    out = {}
+    for prec in (abi.PREC_FP32, abi.PREC_FP64):
+        p = scenes.disc_orbit(160, 90, frames=1, precision=prec).with_camera(cam)
+        out[prec] = renderer.render_preset(p, ALL)
+    for k in ALL:
+        assert np.array_equal(out[abi.PREC_FP32][k], out[abi.PREC_FP64][k], equal_nan=True), k
+    r = oracle.render_preset(p, THREADS, ALL)
+    st = compare(out[abi.PREC_FP64], r)
+    _log("camera inside the photon sphere", st)
+    assert st["class_mismatch_frac"] == 0
+    assert st["steps_mismatch"] == 0
+    assert st["phi_max_err"] <= 1e-9
 
 
 def test_fp32_8k_frame_matches_oracle(renderer):
