@@ -56,7 +56,10 @@ enum {
 /* ---- option enums ------------------------------------------------------ */
 enum {
   BH_PREC_FP64 = 0, /* every step in FP64, reference operation order */
-  BH_PREC_FP32 = 1  /* FP32 march; FP64 set-up, swept angle and photon-sphere band */
+  BH_PREC_FP32 = 1  /* FP32 march; FP64 set-up, swept angle and photon-sphere band.
+                       Scenes the FP32 march does not cover render as BH_PREC_FP64:
+                       bilinear filter, a disc not centred on the hole, nstep > 4096,
+                       or the camera at or inside the photon sphere (|C - B| <= 3M) */
 };
 enum { BH_FILTER_NEAREST = 0, BH_FILTER_BILINEAR = 1 };
 enum {
