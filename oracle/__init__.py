@@ -54,6 +54,9 @@ def load():
         "or_trace_pixel": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.Options),
                                      C.POINTER(abi.Image), C.c_int, C.c_int, C.c_int,
                                      C.POINTER(abi.Outputs)]),
+        "or_trace_escape": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.Options),
+                                      C.POINTER(abi.Image), C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(abi.Outputs), dp]),
         "or_render": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.Options),
                                 C.POINTER(abi.Image), C.c_int, C.c_int,
                                 C.POINTER(abi.Outputs)]),
@@ -138,6 +141,22 @@ def trace_pixel(scene, opts, textures: dict, x: int, y: int):
     tex, n, keep = _textures(textures)
     lib.or_trace_pixel(C.byref(scene), C.byref(opts), tex, n, x, y, C.byref(o))
     return {k: v[y, x] for k, v in arrs.items()}
+
+
+def trace_escape(scene, opts, textures: dict, x: int, y: int):
+    """One pixel, plus the last two points of its march when it escapes:
+    (outputs of the pixel, chord as a (2, 3) array, NaN for a hit)."""
+    from blackhole_8_amd.render import alloc_outputs
+
+    lib = load()
+    w, h = scene.camera.width, scene.camera.height
+    arrs, o = alloc_outputs(w, h, ("rgb", "hit", "phi", "steps"))
+    tex, n, keep = _textures(textures)
+    chord = np.zeros(6)
+    lib.or_trace_escape(C.byref(scene), C.byref(opts), tex, n, x, y, C.byref(o),
+                        chord.ctypes.data_as(C.POINTER(C.c_double)))
+    del keep
+    return {k: v[y, x] for k, v in arrs.items()}, chord.reshape(2, 3)
 
 
 def render_flat(scene, textures: dict, want=("rgb", "hit")):

@@ -475,8 +475,11 @@ static void write_px(const bh_outputs* out, size_t idx, const uint8_t rgb[3], in
   if (out->steps) out->steps[idx] = steps;
 }
 
-int or_trace_pixel(const bh_scene* s, const bh_options* o, const bh_image* tex, int ntex,
-                   int x, int y, const bh_outputs* out) {
+/* The march of pixel (x, y).  chord, if not NULL, receives the last two
+ * points of the march (P_prev2 then P_prev) of a ray that hits nothing: the
+ * chord whose direction picks the background texel. */
+static int trace_pixel(const bh_scene* s, const bh_options* o, const bh_image* tex, int ntex,
+                       int x, int y, const bh_outputs* out, double chord[6]) {
   const int W = s->camera.width;
   size_t idx = (size_t)y * W + x;
   uint8_t black[3] = {0, 0, 0};
@@ -526,6 +529,11 @@ int or_trace_pixel(const bh_scene* s, const bh_options* o, const bh_image* tex, 
       st.u = un;
       if (march_step(s, &ray, &st, &h)) goto hit;
     }
+    if (chord)
+      for (int k = 0; k < 3; ++k) {
+        chord[k] = st.P_prev2.v[k];
+        chord[3 + k] = st.P_prev.v[k];
+      }
     const bh_image* bg = tex_at(tex, ntex, s->background);
     uint8_t c[3] = {0, 0, 0};
     if (bg) {
@@ -544,6 +552,17 @@ hit:
   hit_color(s, o, tex, ntex, &h, rgb);
   write_px(out, idx, rgb, h.cls, st.phi, &h.q, st.steps);
   return 0;
+}
+
+int or_trace_pixel(const bh_scene* s, const bh_options* o, const bh_image* tex, int ntex,
+                   int x, int y, const bh_outputs* out) {
+  return trace_pixel(s, o, tex, ntex, x, y, out, NULL);
+}
+
+int or_trace_escape(const bh_scene* s, const bh_options* o, const bh_image* tex, int ntex,
+                    int x, int y, const bh_outputs* out, double chord[6]) {
+  for (int k = 0; k < 6; ++k) chord[k] = NAN;
+  return trace_pixel(s, o, tex, ntex, x, y, out, chord);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -57,6 +57,11 @@ void or_equirect_texel(const double d[3], int rows, int cols, double* fc, double
 /* One pixel.  textures[slot] are host images.  Writes one pixel of out. */
 int or_trace_pixel(const bh_scene* s, const bh_options* o, const bh_image* textures,
                    int n_textures, int x, int y, const bh_outputs* out);
+/* or_trace_pixel that also returns, for a ray that escapes with the escape
+ * extension on, the last two points of its march (chord[0..2] = the point
+ * before the last, chord[3..5] = the last); NaN otherwise. */
+int or_trace_escape(const bh_scene* s, const bh_options* o, const bh_image* textures,
+                    int n_textures, int x, int y, const bh_outputs* out, double chord[6]);
 /* Whole frame on `threads` host threads (rows interleaved). */
 int or_render(const bh_scene* s, const bh_options* o, const bh_image* textures,
               int n_textures, int threads, const bh_outputs* out);

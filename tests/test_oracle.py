@@ -358,3 +358,161 @@ def test_disc_frame_reference_mode_steps():
     assert not o["rgb"][hit == abi.HIT_NONE].any()
     assert (hit == abi.HIT_DISC).sum() > 0.2 * hit.size
     assert (steps[hit == abi.HIT_DISC] <= 39).all()
+
+
+# ---------------------------------------------------------------------------
+# Escape extension: the outbound march continues with the same du and weight
+# until the next point would lie at r >= r_escape, then the background is
+# read along the last chord.  Pinned by the radii of the last two points, by
+# the flat limit (the chord tends to the pixel line C -> d) and by the weak
+# field (the chord turns toward the hole by the Schwarzschild deflection).
+# ---------------------------------------------------------------------------
+def _pixel_point(cam, x, y):
+    out = np.zeros(3)
+    L.or_pixel_vector(C.byref(cam), x, y, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def _line_dir(cam, x, y):
+    d = _pixel_point(cam, x, y) - np.array(cam.position)
+    return d / np.linalg.norm(d)
+
+
+def _chord_dir(chord):
+    d = chord[1] - chord[0]
+    return d / np.linalg.norm(d)
+
+
+def test_escape_stops_before_the_escape_radius():
+    """The last point lies inside r_escape and the next one the march would
+    take (one du further out) at or beyond it; the point before the last is
+    one du further in.  An escape one step early or late fails."""
+    p = scenes.plain_hole(40, 24)
+    s, o = p.scene, p.options
+    B = np.array(s.hole.position)
+    rng = np.random.default_rng(5)
+    checked = 0
+    for _ in range(60):
+        x, y = int(rng.integers(0, 40)), int(rng.integers(0, 24))
+        px, chord = oracle.trace_escape(s, o, p.textures, x, y)
+        if px["hit"] != abi.HIT_NONE:
+            assert np.isnan(chord).all()
+            continue
+        ray = oracle.ray_setup(s, o, x, y)
+        r_last = np.linalg.norm(chord[1] - B)
+        r_prev = np.linalg.norm(chord[0] - B)
+        u_last = 1.0 / r_last
+        assert r_last < o.r_escape
+        u_next = u_last - ray.du
+        assert u_next * o.r_escape <= 1.0 + 1e-12
+        assert 1.0 / r_prev == pytest.approx(u_last + ray.du, rel=1e-12)
+        assert px["steps"] < o.step_cap
+        # steps past the reference's outbound leg: u = u0 + (nstep - 0.1 - j) du
+        j = px["steps"] - (2 * o.nstep - 1)
+        assert j >= 0
+        assert u_last == pytest.approx(ray.u0 + (o.nstep - 0.1 - (j + o.nstep - 1)) * ray.du,
+                                       rel=1e-12)
+        checked += 1
+    assert checked > 40
+
+
+def test_escape_step_cap():
+    """With a cap below the outbound leg's length the march stops at the cap
+    and still reads the background along its last chord."""
+    p = scenes.plain_hole(40, 24)
+    o = abi.Options.from_buffer_copy(p.options)
+    o.step_cap = 45
+    for x, y in ((0, 0), (5, 20), (39, 23)):
+        px, chord = oracle.trace_escape(p.scene, o, p.textures, x, y)
+        assert px["hit"] == abi.HIT_NONE
+        assert px["steps"] == 45
+        assert np.isfinite(chord).all()
+
+
+def test_escape_background_is_read_along_the_last_chord():
+    """Every escaped pixel's colour is the equirect texel of the direction
+    from the point before the last to the last point."""
+    p = scenes.plain_hole(40, 24)
+    bg = p.textures[scenes.SLOT_BG]
+    fc, fr = C.c_double(), C.c_double()
+    col, row = C.c_int(), C.c_int()
+    n = 0
+    for y in range(0, 24, 3):
+        for x in range(0, 40, 3):
+            px, chord = oracle.trace_escape(p.scene, p.options, p.textures, x, y)
+            if px["hit"] != abi.HIT_NONE:
+                continue
+            d = _chord_dir(chord)
+            L.or_equirect_texel(dp(d), bg.shape[0], bg.shape[1], C.byref(fc), C.byref(fr),
+                                C.byref(col), C.byref(row))
+            assert tuple(px["rgb"]) == tuple(bg[row.value, col.value]), (x, y)
+            n += 1
+    assert n > 60
+
+
+def _flat_options(nstep):
+    # tiny mass: the exact turning point (20 bisections of [eps^(1/3), 1/(3M)]
+    # cannot resolve u_p ~ 1/b when 1/(3M) ~ 1e8)
+    return scenes.options(precision=abi.PREC_FP64, nstep=nstep, escape=True, r_escape=1e3,
+                          step_cap=10 ** 8, root=abi.ROOT_EXACT)
+
+
+def test_escape_chord_tends_to_the_pixel_line_in_flat_space():
+    """M -> 0: every escaping ray is the straight pixel line, so the last
+    chord points along C -> d.  The reference rule's trapezoid (first weight
+    0, a last approach of 0.9 du, stop 0.1 du short of the turning point)
+    misses O(sqrt(du)) of the swept angle, so the error falls as nstep^-1/2;
+    a reversed or misplaced chord stays O(1) off."""
+    p = scenes.plain_hole(64, 36)
+    s = abi.Scene.from_buffer_copy(p.scene)
+    s.hole.mass = 1e-9
+    pix = [(0, 0), (10, 5), (63, 35), (20, 30), (40, 2), (31, 17)]
+    errs = {}
+    for n in (200, 2000, 20000, 200000):
+        e = []
+        for x, y in pix:
+            px, chord = oracle.trace_escape(s, _flat_options(n), {}, x, y)
+            assert px["hit"] == abi.HIT_NONE
+            c = float(np.clip(_chord_dir(chord) @ _line_dir(s.camera, x, y), -1, 1))
+            e.append(math.acos(c))
+        errs[n] = np.array(e)
+        assert errs[n].max() < 0.1 / math.sqrt(n), n
+    # one decade of nstep shrinks the error by about sqrt(10)
+    assert (2.0 < errs[2000] / errs[20000]).all() and (errs[2000] / errs[20000] < 4.5).all()
+    assert (2.0 < errs[20000] / errs[200000]).all() and (errs[20000] / errs[200000] < 4.5).all()
+
+
+def test_escape_chord_turns_toward_the_hole_by_the_deflection():
+    """Weak field: seen from far away (r0 = 1e4 r_s, so the start of the march
+    is flat), the last chord of a ray with impact parameter b >> M turns
+    toward the hole, relative to the same ray at M -> 0 (which shares its
+    discretisation error), by 4M/b + (15 pi/4)(M/b)^2 + O((M/b)^3)."""
+    cam = scenes.camera(64, 36, 0.1, position=(-1e4, 0.0, 0.0))
+    nstep = 20000
+    o = scenes.options(precision=abi.PREC_FP64, nstep=nstep, escape=True, r_escape=1e8,
+                       step_cap=10 ** 9, root=abi.ROOT_EXACT)
+    C0 = np.array(cam.position)
+    checked = 0
+    for x, y in ((0, 0), (10, 5), (63, 35), (20, 30), (40, 2), (5, 30), (60, 3)):
+        dirs = {}
+        for m in (0.5, 1e-12):
+            s = abi.Scene()
+            s.camera = cam
+            s.hole.mass = m
+            s.background = -1
+            px, chord = oracle.trace_escape(s, o, {}, x, y)
+            assert px["hit"] == abi.HIT_NONE
+            dirs[m] = _chord_dir(chord)
+        line = _line_dir(cam, x, y)
+        b = np.linalg.norm(np.cross(line, -C0))
+        if b < 15.0:
+            continue
+        closest = C0 + (-C0 @ line) * line
+        toward = -closest / np.linalg.norm(closest)
+        alpha = math.atan2((dirs[0.5] - dirs[1e-12]) @ toward, dirs[0.5] @ dirs[1e-12])
+        q = 0.5 / b
+        assert alpha > 0
+        assert alpha == pytest.approx(4 * q + 15 * math.pi / 4 * q * q,
+                                      abs=60 * q ** 3 + 0.05 / math.sqrt(nstep)), (x, y, b)
+        checked += 1
+    assert checked >= 4
