@@ -71,7 +71,8 @@ class Scene(C.Structure):
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("nstep", C.c_int32), ("escape", C.c_int32),
                 ("step_cap", C.c_int32), ("r_escape", C.c_double), ("filter", C.c_int32),
-                ("root", C.c_int32), ("band", C.c_double), ("edge_width", C.c_double)]
+                ("root", C.c_int32), ("band", C.c_double), ("edge_width", C.c_double),
+                ("edge_guard", C.c_double)]
 
 
 class Outputs(C.Structure):

@@ -42,7 +42,7 @@ struct bh_ctx {
   DeviceTexture tex[BH_MAX_TEXTURES];
   DevBuf scene, rgb, hit, phi, hitpt, steps, qitems, qcount;
   DevBuf dpart, dgrad, drgb, dhit, dgin;  // differentiable render
-  DevBuf bscenes, bouts, boff, bctl, bband, bend, btile;  // batches
+  DevBuf bscenes, bouts, boff, bctl, bband, bbandc, bend, btile;  // batches
   int wall_khz = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   char name[256] = {0};
@@ -55,9 +55,9 @@ struct bh_path {
   DevScene* d_scenes = nullptr;
   uint8_t* d_rgb = nullptr;
   uint8_t* d_hit = nullptr;
-  uint32_t* d_qitems = nullptr;
+  uint64_t* d_qitems = nullptr;
   uint32_t* d_qcount = nullptr;
-  uint32_t qcap = 0;                 // queue slice per frame
+  uint32_t sub_cap = 0;              // FP64 queue of each frame: kQueueSubs x sub_cap items
   bool fused = false;                // all frames FP32: one march + one band launch
   bool esc = false, sph = false;     // scene class of the fused launches
   std::vector<hipEvent_t> events;
@@ -174,6 +174,8 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
   if (o.escape && (!(o.r_escape > 0.0) || o.step_cap <= 0 || o.step_cap > 1000000))
     return fail(ctx, BH_ERR_INVALID_ARG, "escape needs r_escape > 0 and step_cap in 1..1e6");
   if (!(o.band >= 0.0)) return fail(ctx, BH_ERR_INVALID_ARG, "band must be >= 0");
+  if (!std::isfinite(o.edge_guard) || o.edge_guard > 0.01)
+    return fail(ctx, BH_ERR_INVALID_ARG, "edge_guard must be finite and <= 0.01");
   if (sc->n_spheres < 0 || sc->n_spheres > BH_MAX_SPHERES)
     return fail(ctx, BH_ERR_INVALID_ARG, "n_spheres out of range");
 
@@ -280,6 +282,28 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
     for (int k = 0; k < 3; ++k) d->sphf[i][k] = (float)(d->sc[i][k] - d->B[k]);
     d->sphf[i][3] = (float)d->sr[i];
   }
+  // Texel-edge guards from the relative error bound g of an FP32 hit point
+  // (the largest measured: 1.4e-6 of the disc's outer radius, 1080p frames
+  // of the disc orbit) and of an FP32 escape direction: the disc's in texel
+  // units, the spheres' and the background's as angles of the direction that
+  // picks the texel.
+  const double g = o.edge_guard == 0.0 ? 2e-6 : (o.edge_guard > 0.0 ? o.edge_guard : 0.0);
+  if (d->has_disc) {
+    double l1 = 0, l2 = 0;
+    for (int i = 0; i < 3; ++i) {
+      l1 += (d->dv2[i] - d->dv1[i]) * (d->dv2[i] - d->dv1[i]);
+      l2 += (d->dv4[i] - d->dv1[i]) * (d->dv4[i] - d->dv1[i]);
+    }
+    const double E = g * std::fabs(d->r_out);
+    d->edge_a = (float)(E * d->disc_tex.cols / std::sqrt(l1));
+    d->edge_b = (float)(E * d->disc_tex.rows / std::sqrt(l2));
+  }
+  d->edge_bg = (float)g;
+  for (int i = 0; i < d->n_spheres; ++i) {
+    double c2 = 0;
+    for (int k = 0; k < 3; ++k) c2 += (d->sc[i][k] - d->B[k]) * (d->sc[i][k] - d->B[k]);
+    d->sph_edge[i] = (float)(g * (std::sqrt(c2) + d->sr[i]) / d->sr[i]);
+  }
   d->fast_ok = d->precision == BH_PREC_FP32 && d->filter == BH_FILTER_NEAREST &&
                d->nstep <= 4096 && d->u0 < d->Uc &&
                (!d->has_disc || (d->dv0[0] == d->B[0] && d->dv0[1] == d->B[1] &&
@@ -300,7 +324,7 @@ int launch_frame(bh_ctx* ctx, const DevScene& host_scene, DevScene* d_scene, con
   const int W = host_scene.W, H = host_scene.H;
   if (fp32_eligible(host_scene)) {
     // the FP32 march appends its band rays to q
-    if (!q.items || !q.count || q.capacity == 0)
+    if (!q.items || !q.count || q.sub_cap == 0)
       return fail(ctx, BH_ERR_INVALID_ARG, "FP32 march launched without a band queue");
     const bool esc = host_scene.escape != 0;
     const bool sph = host_scene.n_spheres > 0;
@@ -312,18 +336,18 @@ int launch_frame(bh_ctx* ctx, const DevScene& host_scene, DevScene* d_scene, con
   return BH_OK;
 }
 
-// Queue of npix items with a zeroed count pair (the band kernel keeps it
-// zeroed between frames).
-int queue_buffers(bh_ctx* ctx, uint32_t** items, uint32_t** count, size_t npix) {
+// The context's FP64 queue for frames up to W x H, with zeroed count pairs
+// (the FP64 pass keeps them zeroed between frames).
+int queue_buffers(bh_ctx* ctx, int W, int H, DevQueue* q) {
   const bool fresh = ctx->qcount.p == nullptr;
-  int rc = ensure(ctx, ctx->qitems, npix * sizeof(uint32_t));
+  int rc = ensure(ctx, ctx->qitems, bh::queue_items(W, H) * sizeof(uint64_t));
   if (rc) return rc;
-  if ((rc = ensure(ctx, ctx->qcount, 256))) return rc;
+  if ((rc = ensure(ctx, ctx->qcount, bh::kQueueCountWords * sizeof(uint32_t)))) return rc;
   if (fresh) {
-    HIP_TRY(ctx, bh::launch_zero_u32((uint32_t*)ctx->qcount.p, 64, ctx->stream));
+    HIP_TRY(ctx, bh::launch_zero_u32((uint32_t*)ctx->qcount.p, (int)bh::kQueueCountWords,
+                                     ctx->stream));
   }
-  *items = (uint32_t*)ctx->qitems.p;
-  *count = (uint32_t*)ctx->qcount.p;
+  *q = DevQueue{(uint64_t*)ctx->qitems.p, (uint32_t*)ctx->qcount.p, bh::queue_sub_cap(W, H)};
   return BH_OK;
 }
 
@@ -425,8 +449,12 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   if (t.mode == BH_BATCH_GRID &&
       (rc = ensure(ctx, ctx->btile, ((size_t)(p.n_rays >> 6) + 1) * sizeof(unsigned long long))))
     return rc;
-  const uint32_t cap = std::max<uint32_t>(p.n_rays / 16u, 4096u);
-  if ((rc = ensure(ctx, ctx->bband, (size_t)cap * sizeof(uint32_t)))) return rc;
+  // FP64 queue: every ray id is queued at most once (band or texel edge), in
+  // the sub-queue of its 64-ray tile
+  const uint32_t n_tiles = p.n_rays >> 6;
+  const uint32_t sub_cap = std::max<uint32_t>((n_tiles + bh::kQueueSubs - 1) / bh::kQueueSubs, 1u) * 64u;
+  if ((rc = ensure(ctx, ctx->bband, (size_t)sub_cap * bh::kQueueSubs * sizeof(uint64_t)))) return rc;
+  if ((rc = ensure(ctx, ctx->bbandc, bh::kQueueCountWords * sizeof(uint32_t)))) return rc;
   hipStream_t st = ctx->stream;
   // the persistent march reads the FP32 scenes by position in the plan
   std::vector<DevScene> fs;
@@ -452,14 +480,12 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   b.ray_off = (const uint32_t*)ctx->boff.p;
   b.n_scenes = (uint32_t)p.fast.size();
   b.n_rays = p.n_rays;
-  // control block: four queue heads, the band count, the start time
+  // control block: four queue heads, the start time
   b.head = (uint32_t*)ctx->bctl.p;
-  b.band_count = (uint32_t*)ctx->bctl.p + 4;
   b.t_start = (unsigned long long*)((char*)ctx->bctl.p + 24);
   b.tile_lo = 0;
   b.tile_hi = p.n_rays >> 6;
-  b.band_items = (uint32_t*)ctx->bband.p;
-  b.band_capacity = cap;
+  b.band = DevQueue{(uint64_t*)ctx->bband.p, (uint32_t*)ctx->bbandc.p, sub_cap};
   b.refill_steps = t.refill_steps;
   b.refill_idle = t.refill_idle;
   b.mode = t.mode;
@@ -491,14 +517,17 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   // solo scenes: the per-frame kernels, with the context's band queue (the
   // band kernel clears it after each scene)
   if (!p.solo.empty()) {
-    size_t maxpix = 0;
-    for (int i : p.solo) maxpix = std::max(maxpix, (size_t)p.scenes[i].W * p.scenes[i].H);
-    uint32_t *qi, *qc;
-    if ((rc = queue_buffers(ctx, &qi, &qc, maxpix))) return rc;
+    int maxw = 0, maxh = 0;
+    for (int i : p.solo) {
+      maxw = std::max(maxw, p.scenes[i].W);
+      maxh = std::max(maxh, p.scenes[i].H);
+    }
+    DevQueue q;  // large enough for every solo scene: sub_cap grows with W and H
+    if ((rc = queue_buffers(ctx, maxw, maxh, &q))) return rc;
     for (size_t k = 0; k < p.solo.size(); ++k) {
       const int i = p.solo[k];
       const DevScene& d = p.scenes[i];
-      DevQueue q{qi, qc, (uint32_t)((size_t)d.W * d.H)};
+      q.sub_cap = bh::queue_sub_cap(d.W, d.H);
       if ((rc = launch_frame(ctx, d, d_all + p.fast.size() + k, p.outs[i], q, st))) return rc;
     }
   }
@@ -609,11 +638,11 @@ constexpr int kFramesPerLaunch = 1024;
 
 hipError_t path_march(const bh_path* p, hipStream_t st) {
   const DevOut o{p->d_rgb, p->d_hit, nullptr, nullptr, nullptr};
-  const DevQueue q{p->d_qitems, p->d_qcount, p->qcap};
+  const DevQueue q{p->d_qitems, p->d_qcount, p->sub_cap};
   for (int f0 = 0; f0 < p->n_frames; f0 += kFramesPerLaunch) {
     const int nf = std::min(kFramesPerLaunch, p->n_frames - f0);
-    const hipError_t e = bh::launch_march_frames(p->d_scenes, o, p->W, p->H, f0, nf, q, p->qcap,
-                                                 st, p->esc, p->sph);
+    const hipError_t e = bh::launch_march_frames(p->d_scenes, o, p->W, p->H, f0, nf, q, st,
+                                                 p->esc, p->sph);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -621,11 +650,11 @@ hipError_t path_march(const bh_path* p, hipStream_t st) {
 
 hipError_t path_band(const bh_path* p, hipStream_t st) {
   const DevOut o{p->d_rgb, p->d_hit, nullptr, nullptr, nullptr};
-  const DevQueue q{p->d_qitems, p->d_qcount, p->qcap};
+  const DevQueue q{p->d_qitems, p->d_qcount, p->sub_cap};
   for (int f0 = 0; f0 < p->n_frames; f0 += kFramesPerLaunch) {
     const int nf = std::min(kFramesPerLaunch, p->n_frames - f0);
-    const hipError_t e = bh::launch_band_frames(p->d_scenes, o, p->W, p->H, f0, nf, q, p->qcap,
-                                                st, p->esc, p->sph);
+    const hipError_t e = bh::launch_band_frames(p->d_scenes, o, p->W, p->H, f0, nf, q, st,
+                                                p->esc, p->sph);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -803,13 +832,11 @@ int bh_render_device(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
     if (rc) return rc;
     (void)hipSetDevice(ctx->device);
     if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
-    uint32_t *qi, *qc;
-    const size_t npix = (size_t)d.W * d.H;
-    if ((rc = queue_buffers(ctx, &qi, &qc, npix))) return rc;
+    DevQueue q;
+    if ((rc = queue_buffers(ctx, d.W, d.H, &q))) return rc;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
     DevOut o{out_device->rgb, out_device->hit, out_device->phi_end, out_device->hit_point,
              out_device->steps};
-    DevQueue q{qi, qc, (uint32_t)npix};
     return launch_frame(ctx, d, (DevScene*)ctx->scene.p, o, q, ctx->stream);
   });
 }
@@ -855,13 +882,12 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
     if ((rc = ensure(ctx, ctx->rgb, npix * 3))) return rc;
     if ((rc = ensure(ctx, ctx->hit, npix))) return rc;
     if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
-    uint32_t *qi, *qc;
-    if ((rc = queue_buffers(ctx, &qi, &qc, npix))) return rc;
+    DevQueue q;
+    if ((rc = queue_buffers(ctx, d.W, d.H, &q))) return rc;
     hipStream_t st = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, st));
     DevOut o{(uint8_t*)ctx->rgb.p, (uint8_t*)ctx->hit.p, nullptr, nullptr, nullptr};
     DevScene* ds = (DevScene*)ctx->scene.p;
-    DevQueue q{qi, qc, (uint32_t)npix};
     const bool fast = fp32_eligible(d);
     // warm-up
     if ((rc = launch_frame(ctx, d, ds, o, q, st))) return rc;
@@ -869,7 +895,7 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
     // main kernel: iters back-to-back launches between one event pair (the
     // queue is cleared afterwards); whole frame: iters march + band launches
     float main_ms = 0, total_ms = 0;
-    uint32_t nband = 0;
+    std::vector<uint32_t> cnt(bh::kQueueCountWords, 0u);
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     for (int it = 0; it < iters; ++it) {
       if (fast) {
@@ -880,8 +906,8 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     if (fast) {
-      HIP_TRY(ctx, hipMemcpyAsync(&nband, qc, 4, hipMemcpyDeviceToHost, st));
-      HIP_TRY(ctx, bh::launch_zero_u32(qc, 2, st));
+      HIP_TRY(ctx, hipMemcpyAsync(cnt.data(), q.count, cnt.size() * 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(ctx, bh::launch_zero_u32(q.count, (int)bh::kQueueCountWords, st));
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev2, st));
     for (int it = 0; it < iters; ++it)
@@ -893,7 +919,9 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
     HIP_TRY(ctx, hipEventElapsedTime(&main_ms, ctx->ev0, ctx->ev1));
     HIP_TRY(ctx, hipEventElapsedTime(&total_ms, ctx->ev2, end));
     (void)hipEventDestroy(end);
-    if (fast) nband /= (uint32_t)iters;
+    int64_t nband = 0;
+    for (uint32_t k = 0; k < bh::kQueueSubs; ++k) nband += cnt[16 * k] + cnt[16 * k + 2];
+    if (fast) nband /= iters;
     if (avg_ms_main) *avg_ms_main = main_ms / iters;
     if (avg_ms_total) *avg_ms_total = total_ms / iters;
     if (band_rays) *band_rays = nband;
@@ -935,17 +963,18 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
     if ((e = hipMalloc(&p->d_scenes, sizeof(DevScene) * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc scenes");
     if ((e = hipMalloc(&p->d_rgb, npix * 3 * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc rgb");
     if ((e = hipMalloc(&p->d_hit, npix * n_frames)) != hipSuccess) return cleanup(e, "hipMalloc hit");
-    // one queue slice and count pair per frame, so that the FP64 band kernel of
-    // frame i (side stream) overlaps the FP32 march of frame i+1
-    const size_t qcap = npix / 16 > 1024 ? npix / 16 : 1024;
-    p->qcap = (uint32_t)qcap;
+    // one FP64 queue per frame, so that the FP64 pass of frame i (side
+    // stream) overlaps the FP32 march of frame i+1
+    p->sub_cap = bh::queue_sub_cap(p->W, p->H);
+    const size_t qitems = bh::queue_items(p->W, p->H);
+    const size_t qwords = bh::kQueueCountWords;
     p->fused = true;
     for (const DevScene& f : frames) p->fused = p->fused && fp32_eligible(f);
     p->esc = frames[0].escape != 0;
     p->sph = frames[0].n_spheres > 0;
-    if ((e = hipMalloc(&p->d_qitems, qcap * n_frames * sizeof(uint32_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
-    if ((e = hipMalloc(&p->d_qcount, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMalloc queue count");
-    if ((e = hipMemset(p->d_qcount, 0, 64 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMemset queue count");
+    if ((e = hipMalloc(&p->d_qitems, qitems * n_frames * sizeof(uint64_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
+    if ((e = hipMalloc(&p->d_qcount, qwords * 4 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMalloc queue count");
+    if ((e = hipMemset(p->d_qcount, 0, qwords * 4 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMemset queue count");
     if ((e = hipMemcpy(p->d_scenes, frames.data(), sizeof(DevScene) * n_frames, hipMemcpyHostToDevice)) != hipSuccess)
       return cleanup(e, "hipMemcpy scenes");
     p->events.resize((size_t)n_frames + 1, nullptr);
@@ -962,7 +991,7 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
     }
     for (int i = 0; i < n_frames && rc == BH_OK && !p->fused; ++i) {
       DevOut o{p->d_rgb + npix * 3 * i, p->d_hit + npix * i, nullptr, nullptr, nullptr};
-      DevQueue q{p->d_qitems + qcap * i, p->d_qcount + 16 * i, (uint32_t)qcap};
+      DevQueue q{p->d_qitems + qitems * i, p->d_qcount + qwords * i, p->sub_cap};
       const DevScene& f = frames[(size_t)i];
       if (fp32_eligible(f)) {
         const bool esc = f.escape != 0;
@@ -1171,15 +1200,18 @@ int bh_render_batch_device(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_
       float m = 0, tot = 0;
       HIP_TRY(ctx, hipEventElapsedTime(&m, ctx->ev0, ctx->ev1));
       HIP_TRY(ctx, hipEventElapsedTime(&tot, ctx->ev0, ctx->ev2));
-      uint32_t nb = 0;
-      HIP_TRY(ctx, hipMemcpy(&nb, b.band_count, 4, hipMemcpyDeviceToHost));
+      std::vector<uint32_t> cnt(bh::kQueueCountWords);
+      HIP_TRY(ctx, hipMemcpy(cnt.data(), b.band.count, cnt.size() * 4, hipMemcpyDeviceToHost));
+      int64_t nb = 0;
+      for (uint32_t k = 0; k < bh::kQueueSubs; ++k)
+        nb += std::min(cnt[16 * k] + cnt[16 * k + 2], b.band.sub_cap);
       stats->march_ms = m;
       stats->band_ms = tot - m;
       stats->total_ms = tot;
       int64_t px = 0;
       for (const auto& d : p.scenes) px += (int64_t)d.W * d.H;
       stats->rays = px;
-      stats->band_rays = std::min(nb, b.band_capacity);
+      stats->band_rays = nb;
       stats->solo_scenes = (int32_t)p.solo.size();
       stats->pad_ = 0;
     }
@@ -1282,6 +1314,11 @@ static int build_flat(bh_ctx* ctx, const bh_flat_scene* sc, bh::DevFlat* dp) {
     if (!finite3(p.position) || !finite3(p.axis_x) || !finite3(p.axis_y) || !finite3(p.axis_z) ||
         !std::isfinite(p.pattern_size))
       return fail(ctx, BH_ERR_INVALID_ARG, "plane must be finite");
+    // ChessPattern2D(int pattern_size) (pattern.h:20): a whole number; 0 = black
+    if (p.pattern_size != 0.0 &&
+        !(p.pattern_size >= 1.0 && p.pattern_size <= 536870912.0 &&
+          p.pattern_size == std::floor(p.pattern_size)))
+      return fail(ctx, BH_ERR_INVALID_ARG, "plane pattern_size must be 0 or a whole number in 1..2^29");
     copy3(d.pp[i], p.position);
     copy3(d.px[i], p.axis_x);
     copy3(d.py[i], p.axis_y);

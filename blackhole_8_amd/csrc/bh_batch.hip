@@ -29,9 +29,11 @@
 // per range, ranges shrinking as the queue empties), so the single counter is
 // touched a few times per wave rather than once per tile.  Photon-sphere band
 // rays are appended (same ballot/prefix-sum scheme) to a queue for the FP64
-// pass that follows.  A scene's completion time is the latest time a wave
-// finished one of its rays in the march (wall clock; one atomic max when a
-// wave moves on to another scene); band rays are finished by the FP64 pass.
+// pass that follows, and so are the rays whose texel lies within the edge
+// guard of a texel edge (the FP64 pass marches them again in the reference's
+// order).  A scene's completion time is the latest time a wave finished one
+// of its rays in the march (wall clock; one atomic max when a wave moves on
+// to another scene); queued rays are finished by the FP64 pass.
 #include "bh_internal.h"
 #include "bh_march.h"
 
@@ -125,21 +127,20 @@ struct StepScene {
 struct RayRun {
   Lane<float> L;
   MarchState<float> st;
-  MarchOut o;
+  MarchOut<float> o;
   RaySetup r;
   StepScene k;
-  uint32_t sc, idx;
+  uint32_t sc, idx, id;  // scene, pixel index in the scene, ray id
   int n;  // next step
 };
 
-// Finish a ray: output planes of its scene.
-__device__ inline void finish(const BatchDev& b, const RayRun& R) {
+// Finish a ray: output planes of its scene.  Returns true when its texel lies
+// within the edge guard of a texel edge (the ray goes to the FP64 pass).
+__device__ inline bool finish(const BatchDev& b, const RayRun& R) {
   const DevScene& s = b.scenes[R.sc];
   const DevOut out = b.outs[R.sc];
-  if (s.escape)
-    finish_ray<true>(s, out, R.idx, R.r, R.o);
-  else
-    finish_ray<false>(s, out, R.idx, R.r, R.o);
+  if (s.escape) return finish_ray<true>(s, out, R.idx, R.r, R.o);
+  return finish_ray<false>(s, out, R.idx, R.r, R.o);
 }
 
 // Advance one ray by one step of the reference rule (the per-frame march's
@@ -202,6 +203,7 @@ __device__ inline bool start(const BatchDev& b, uint32_t id, RayRun& R, bool* ba
   *band = false;
   *fin = false;
   R.sc = scene_of(b, id);
+  R.id = id;
   const DevScene& s = b.scenes[R.sc];
   int x, y;
   if (!pixel_of(s, id - b.ray_off[R.sc], &x, &y)) return false;
@@ -213,7 +215,7 @@ __device__ inline bool start(const BatchDev& b, uint32_t id, RayRun& R, bool* ba
   }
   if (R.r.degenerate) {
     degenerate_out(R.o);
-    finish(b, R);
+    (void)finish(b, R);  // class 1, black: no texel
     *fin = true;
     return false;
   }
@@ -221,22 +223,48 @@ __device__ inline bool start(const BatchDev& b, uint32_t id, RayRun& R, bool* ba
   return true;
 }
 
-// Append the band rays of this wave to the band queue (one atomic); a full
-// queue marches them here in FP32, as the per-frame kernel does.
-__device__ inline bool push_band(const BatchDev& b, bool band, uint32_t id) {
-  const uint64_t m = __ballot(band);
+// Position of slot in sub-queue k: band rays from the front, texel-edge rays
+// (back) from the back.
+__device__ __forceinline__ size_t queue_pos(const DevQueue& q, uint32_t k, uint32_t slot,
+                                            bool back) {
+  return (size_t)k * q.sub_cap + (back ? q.sub_cap - 1u - slot : slot);
+}
+
+// Append the lanes with flag set, whose ray ids all lie in one tile, to the
+// tile's FP64 sub-queue (one atomic per wave): band rays to the front,
+// texel-edge rays with their outcome to the back.  Returns true for a lane
+// whose ray did not fit; a sub-queue holds every ray id of its tiles, so that
+// does not happen.
+__device__ inline bool push_band(const BatchDev& b, bool flag, uint32_t id, uint32_t outcome,
+                                 bool back) {
+  const uint64_t m = __ballot(flag);
   if (!m) return false;
-  uint32_t base = 0;
   const int leader = __builtin_ctzll(m);
-  if (lane_id() == leader) base = atomicAdd(b.band_count, (uint32_t)__popcll(m));
+  const uint32_t k =
+      ((uint32_t)__builtin_amdgcn_readlane((int)id, leader) >> 6) % kQueueSubs;
+  uint32_t* count = b.band.count + 16 * k + (back ? 2 : 0);
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
   base = (uint32_t)__shfl((int)base, leader);
-  if (!band) return false;
+  if (!flag) return false;
   const uint32_t slot = base + lane_rank(m);
-  if (slot < b.band_capacity) {
-    b.band_items[slot] = id;
+  if (slot < b.band.sub_cap) {
+    b.band.items[queue_pos(b.band, k, slot, back)] = (uint64_t)id | ((uint64_t)outcome << 32);
     return false;
   }
   return true;  // overflow: march here
+}
+
+// The same for lanes holding rays of different tiles (lane-refill mode): one
+// atomic per lane on its own tile's sub-queue.
+__device__ inline bool push_band_lane(const BatchDev& b, bool flag, uint32_t id,
+                                      uint32_t outcome, bool back) {
+  if (!flag) return false;
+  const uint32_t k = (id >> 6) % kQueueSubs;
+  const uint32_t slot = atomicAdd(b.band.count + 16 * k + (back ? 2 : 0), 1u);
+  if (slot >= b.band.sub_cap) return true;
+  b.band.items[queue_pos(b.band, k, slot, back)] = (uint64_t)id | ((uint64_t)outcome << 32);
+  return false;
 }
 
 // Lane-refill mode: every lane its own ray and step counter.  Ray ids come
@@ -272,7 +300,7 @@ __device__ inline void run_lanes(const BatchDev& b, uint32_t waves) {
       }
       res += min(nidle, avail);
       note_finish(b, f, fin, R.sc);
-      if (push_band(b, band, id)) {
+      if (push_band_lane(b, band, id, 0u, false)) {
         arm(b.scenes[R.sc], R);  // band queue full: march this ray in FP32 here
         live = true;
       }
@@ -281,15 +309,17 @@ __device__ inline void run_lanes(const BatchDev& b, uint32_t waves) {
       if (drained) break;
       continue;
     }
-    bool fin = false;
+    bool fin = false, redo = false;
     for (int k = 0; k < b.refill_steps; ++k) {
       if (live && advance(R)) {
-        finish(b, R);
+        redo = finish(b, R);
         live = false;
         fin = true;
       }
     }
     note_finish(b, f, fin, R.sc);
+    // a lane ends at most one ray per refill round, so R is that ray
+    push_band_lane(b, redo, R.id, queue_outcome(R.o.hit.cls, R.o.steps), true);
   }
   flush(b, f);
 }
@@ -303,15 +333,15 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
                                   uint32_t id) {
   int x, y;
   if (!pixel_of(s, id - b.ray_off[sc], &x, &y)) {
-    push_band(b, false, id);
+    push_band(b, false, id, 0u, false);
     return false;
   }
   const size_t idx = (size_t)y * s.W + x;
   RaySetup r;
   setup_ray(s, x, y, false, r);
   const bool band = r.band && !r.degenerate;
-  if (!push_band(b, band, id) && band) return false;
-  MarchOut mo;
+  if (!push_band(b, band, id, 0u, false) && band) return false;
+  MarchOut<float> mo;
   if (r.degenerate) {
     degenerate_out(mo);
   } else {
@@ -320,7 +350,8 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
     march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
   }
-  finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo);
+  const bool redo = finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo);
+  push_band(b, redo, id, queue_outcome(mo.hit.cls, mo.steps), true);
   return true;
 }
 
@@ -396,36 +427,36 @@ __global__ __launch_bounds__(256) void batch_lane_kernel(BatchDev b) {
   run_lanes(b, (gridDim.x * blockDim.x) >> 6);
 }
 
-// FP64 march of the queued band rays (the per-frame band kernel's body with
-// the scene of each ray id).
+// FP64 pass over the queued rays (the per-frame FP64 pass's body with the
+// scene of each ray id): workgroup blk takes part blk / kQueueSubs of
+// sub-queue blk mod kQueueSubs.
 __global__ __launch_bounds__(64) void batch_band_kernel(BatchDev b) {
-  const uint32_t n = min(b.band_count[0], b.band_capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t id = b.band_items[i];
+  const uint32_t k = blockIdx.x % kQueueSubs, part = blockIdx.x / kQueueSubs;
+  const uint32_t parts = gridDim.x / kQueueSubs;
+  const uint32_t nb = min(b.band.count[16 * k], b.band.sub_cap);
+  const uint32_t ne = min(b.band.count[16 * k + 2], b.band.sub_cap - nb);
+  const uint64_t* items = b.band.items + (size_t)k * b.band.sub_cap;
+  // band rays, then texel-edge rays (from the back)
+#pragma unroll 1
+  for (uint32_t i = part * blockDim.x + threadIdx.x; i < nb + ne; i += parts * blockDim.x) {
+    const uint64_t it = items[i < nb ? i : b.band.sub_cap - 1u - (i - nb)];
+    const uint32_t id = (uint32_t)it, outcome = (uint32_t)(it >> 32);
     const uint32_t sc = scene_of(b, id);
     const DevScene& s = b.scenes[sc];
     int x, y;
     pixel_of(s, id - b.ray_off[sc], &x, &y);
     const size_t idx = (size_t)y * s.W + x;
-    RaySetup r;
-    setup_ray(s, x, y, true, r);
-    MarchOut mo;
-    Lane<double> L;
-    const double du = (r.l - s.u0) * s.inv_nstep;
-    setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
     const bool sph = s.n_spheres > 0;
     if (s.escape) {
       if (sph)
-        march<double, true, true>(s, L, r.captured, r.first_aa, du, mo);
+        fp64_ray<true, true>(s, b.outs[sc], idx, x, y, outcome);
       else
-        march<double, true, false>(s, L, r.captured, r.first_aa, du, mo);
-      finish_ray<true>(s, b.outs[sc], idx, r, mo);
+        fp64_ray<true, false>(s, b.outs[sc], idx, x, y, outcome);
     } else {
       if (sph)
-        march<double, false, true>(s, L, r.captured, r.first_aa, du, mo);
+        fp64_ray<false, true>(s, b.outs[sc], idx, x, y, outcome);
       else
-        march<double, false, false>(s, L, r.captured, r.first_aa, du, mo);
-      finish_ray<false>(s, b.outs[sc], idx, r, mo);
+        fp64_ray<false, false>(s, b.outs[sc], idx, x, y, outcome);
     }
   }
 }
@@ -434,16 +465,17 @@ __global__ void batch_clear_kernel(BatchDev b) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < b.n_scenes) b.scene_end[i] = 0ull;
   if (i < 4) b.head[i] = 0u;
-  if (i == 0) {
-    b.band_count[0] = 0u;
-    b.t_start[0] = ~0ull;
+  if (i < kQueueSubs) {
+    b.band.count[16 * i] = 0u;
+    b.band.count[16 * i + 2] = 0u;
   }
+  if (i == 0) b.t_start[0] = ~0ull;
 }
 
 }  // namespace
 
 hipError_t launch_batch_clear(const BatchDev& b, hipStream_t stream) {
-  const uint32_t n = b.n_scenes > 0 ? b.n_scenes : 1;
+  const uint32_t n = b.n_scenes > kQueueSubs ? b.n_scenes : kQueueSubs;
   hipLaunchKernelGGL(batch_clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, b);
   return hipGetLastError();
 }
@@ -499,8 +531,9 @@ hipError_t launch_batch_grid_finish(const BatchDev& b, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// 32 one-wave workgroups per sub-queue.
 hipError_t launch_batch_band(const BatchDev& b, hipStream_t stream) {
-  hipLaunchKernelGGL(batch_band_kernel, dim3(256), dim3(64), 0, stream, b);
+  hipLaunchKernelGGL(batch_band_kernel, dim3(32 * kQueueSubs), dim3(64), 0, stream, b);
   return hipGetLastError();
 }
 

@@ -46,9 +46,13 @@ __device__ inline bool plane_collide(const DevFlat& s, int i, d3 p1, d3 p2, d3* 
 }
 
 // ChessPattern2D::mod (pattern.h:44-47): |a| minus whole multiples of
-// 2 size, with C++ integer division of truncated values.
+// 2 size, with C++ integer division of truncated values; size is a whole
+// number >= 1 (checked on the host).  Beyond the int range, where the
+// reference's (int) cast is undefined, the same truncated quotient is taken
+// in double arithmetic.
 __device__ inline double chess_mod(double a, double ps) {
   const double b = fabs(a);
+  if (!(b < 2147483648.0)) return b - floor(floor(b) / (ps * 2)) * (ps * 2);
   return b - (double)((int)b / (int)(ps * 2)) * (ps * 2);
 }
 

@@ -419,6 +419,64 @@ __device__ inline void trace64(const DevScene& s, int x, int y, Pixel64& px) {
   px.steps = steps;
 }
 
+// The reference march of pixel (x, y) replayed for a known outcome: the march
+// ends at step n with a disc hit (cls BH_HIT_DISC) or with no hit (escape
+// background along the last chord).  The swept angle of steps 1..n follows
+// trace64's loop to the bit (the same updates of u, w and phi in the same
+// order); only the last chord's two end points are placed, and the colour is
+// taken as trace64 takes it, without the per-step positions and object tests.
+// Returns false when the last chord does not cross the disc after all (the
+// outcome was not the reference's); px is then unset.
+__device__ inline bool replay64(const DevScene& s, int x, int y, int n, int cls, Pixel64& px) {
+  Frame64 f;
+  frame64(s, x, y, f);
+  if (f.degenerate || n < 1) return false;
+  const d3 B = ld3(s.B);
+  const double M = s.M;
+  const int nstep = s.nstep;
+  double u = f.u0, w_prev = 0, phi = f.phi0;
+  double u_a = u, phi_a = phi;  // state at the start of the last chord
+  for (int k = 1; k <= n; ++k) {
+    u_a = u;
+    phi_a = phi;
+    if (k < nstep)
+      u += f.du;
+    else if (k == nstep)
+      u += f.du * 0.9;
+    else
+      u -= f.du;
+    double w = 1.0 / sqrt(G64(u, f.b, M));
+    phi += (w_prev + w) * f.du_h;
+    w_prev = w;
+  }
+  auto place = [&](double uu, double ph) {
+    double r = 1. / uu;
+    d3 lv = mk3(0, r * cos(ph), r * sin(ph));
+    return matrow(f.R, lv) + B;
+  };
+  const d3 Pb = place(u, phi);
+  const d3 Pa = n == 1 ? ld3(s.C) : place(u_a, phi_a);
+  const double nan = __builtin_nan("");
+  px.phi = phi;
+  px.steps = n;
+  if (cls == BH_HIT_DISC) {
+    Hit64 h;
+    h.cls = BH_HIT_DISC;
+    h.index = -1;
+    if (!s.has_disc || !annulus64(s, Pa, Pb, &h.q)) return false;
+    px.rgb = hit_color64(s, h);
+    px.cls = BH_HIT_DISC;
+    px.q = h.q;
+    return true;
+  }
+  if (cls != BH_HIT_NONE) return false;
+  px.rgb = 0;
+  px.cls = BH_HIT_NONE;
+  px.q = mk3(nan, nan, nan);
+  if (s.escape && s.has_bg) px.rgb = equirect64(s.bg, normalize(Pb - Pa));
+  return true;
+}
+
 __device__ inline void store_pixel64(const DevOut& o, size_t idx, const Pixel64& p) {
   if (o.rgb) {
     o.rgb[3 * idx + 0] = (uint8_t)(p.rgb & 0xff);

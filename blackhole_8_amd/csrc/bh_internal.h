@@ -65,6 +65,11 @@ struct DevScene {
   float tv1[3], ts1[3];   // disc texture square: p1 and s1 = p2 - p1
   float tka, tkb;         // cols / |s1|^2 and rows / (|s1| |s2|)
   float sphf[BH_MAX_SPHERES][4];  // sphere centre relative to B, radius
+  // texel-edge guards of the FP32 march: a ray whose texel coordinate lies
+  // within these of a texel edge is marched again in FP64 (0: never)
+  float edge_a, edge_b;             // disc texel coordinates (texel units)
+  float edge_bg;                    // background direction (radians)
+  float sph_edge[BH_MAX_SPHERES];   // sphere i's normal direction (radians)
   int32_t fast_ok;        // the FP32 march applies to this scene
   int32_t pad1;
 };
@@ -79,14 +84,49 @@ struct DevOut {
   int32_t* steps;
 };
 
-// Photon-sphere band queue: the FP32 march appends the pixel indices of rays
-// with |b/b_c - 1| < band; the FP64 band kernel drains it and its last
-// workgroup clears count and done for the next frame.
+// FP64 queue: the FP32 march appends the rays it leaves to the FP64 pass —
+// the photon-sphere band (|b/b_c - 1| < band) and the rays whose texel lies
+// within the edge guard of a texel edge.  An item is the pixel index (low 32
+// bits) and, for a texel-edge ray, the outcome of its FP32 march (high 32
+// bits: queue_outcome; 0 for a band ray, which the FP32 march left alone).  It is
+// split into kQueueSubs sub-queues so that the appends of a frame spread over
+// that many counters (one counter serialises every append of the frame):
+// sub-queue k takes the pixels of the march's 16x16-pixel workgroups (bx, by)
+// with (bx + 37 by) mod kQueueSubs = k (a column or a row of workgroups meets
+// every sub-queue, so a line of texel ties is spread over all of them) and
+// holds items [k sub_cap, (k + 1) sub_cap): band rays from the front, counted
+// by count[16 k], texel-edge rays from the back, counted by count[16 k + 2]
+// (the FP64 pass runs the two kinds in separate loops, so that a wave does
+// not mix the band rays' full march with the edge rays' short one);
+// count[16 k + 1] counts the finished workgroups of the FP64 pass, whose last
+// one clears the counts for the next frame.  A pixel is appended at most once
+// and sub_cap covers every pixel of the sub-queue's workgroups (at most
+// ceil(gx / kQueueSubs) per row of workgroups), so the two ends never meet.
+constexpr uint32_t kQueueSubs = 64;
+constexpr uint32_t kQueueCountWords = 16 * kQueueSubs;  // count words per queue
 struct DevQueue {
-  uint32_t* items;
-  uint32_t* count;  // count[0] = items, count[1] = finished workgroups
-  uint32_t capacity;
+  uint64_t* items;
+  uint32_t* count;
+  uint32_t sub_cap;
 };
+
+// Outcome word of a queue item: hit class (BH_HIT_*) and the step it ended
+// at (steps < 2^28; the step cap is at most 10^6).
+__host__ __device__ inline uint32_t queue_outcome(int cls, int steps) {
+  return ((uint32_t)cls << 28) | ((uint32_t)steps & 0x0fffffffu);
+}
+
+// Sub-queue of workgroup (bx, by) of a frame.
+__host__ __device__ inline uint32_t queue_sub(uint32_t bx, uint32_t by) {
+  return (bx + 37u * by) % kQueueSubs;
+}
+// sub_cap of the queue of a W x H frame: 256 pixels for each of the
+// gy ceil(gx / kQueueSubs) workgroups a sub-queue can take.
+inline uint32_t queue_sub_cap(int W, int H) {
+  const uint64_t gx = (uint64_t)((W + 15) / 16), gy = (uint64_t)((H + 15) / 16);
+  return (uint32_t)(gy * ((gx + kQueueSubs - 1) / kQueueSubs) * 256u);
+}
+inline size_t queue_items(int W, int H) { return (size_t)kQueueSubs * queue_sub_cap(W, H); }
 
 hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, int H,
                              hipStream_t stream);
@@ -94,12 +134,14 @@ hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, in
                              const DevQueue& q, hipStream_t stream, bool escape, bool spheres);
 hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,
                        hipStream_t stream, bool escape, bool spheres);
+// Frames [frame0, frame0 + nf) of a path: frame f's queue is q shifted by f
+// queues of queue_items(W, H) items and kQueueCountWords count words.
 hipError_t launch_march_frames(const DevScene* scenes, const DevOut& out, int W, int H,
-                               int frame0, int nf, const DevQueue& q, uint32_t qstride,
-                               hipStream_t stream, bool escape, bool spheres);
+                               int frame0, int nf, const DevQueue& q, hipStream_t stream,
+                               bool escape, bool spheres);
 hipError_t launch_band_frames(const DevScene* scenes, const DevOut& out, int W, int H,
-                              int frame0, int nf, const DevQueue& q, uint32_t qstride,
-                              hipStream_t stream, bool escape, bool spheres);
+                              int frame0, int nf, const DevQueue& q, hipStream_t stream,
+                              bool escape, bool spheres);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
@@ -115,9 +157,9 @@ struct BatchDev {
   uint32_t n_scenes, n_rays;
   uint32_t tile_lo, tile_hi; // tile-mode launch: the tiles of one scene class
   uint32_t* head;            // queue head of this launch (four slots, cleared together)
-  uint32_t* band_items;      // band rays (ray ids) for the FP64 pass
-  uint32_t* band_count;      // [0] queued band rays
-  uint32_t band_capacity;
+  DevQueue band;             // FP64 pass: items hold ray ids (low 32 bits) and
+                             // outcomes; sub-queue k takes the tiles k, k +
+                             // kQueueSubs, ... (sub_cap = 64 per tile)
   int32_t mode;              // BH_BATCH_GRID, _TILES or _LANES
   int32_t refill_steps;      // steps between refill checks
   int32_t refill_idle;       // refill once at least this many lanes of 64 idle

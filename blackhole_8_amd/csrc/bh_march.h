@@ -57,11 +57,12 @@ struct Lane {
   uint32_t smask;       // spheres that cut the orbit plane
 };
 
+template <typename T>
 struct HitRec {
   int cls;       // BH_HIT_*
   int obj;       // sphere index of an object hit
   double ang;    // rotation of the frame the hit point is expressed in
-  float hx, hy;  // hit point relative to B in that frame
+  T hx, hy;      // hit point relative to B in that frame
 };
 
 // Exact tests of the chord from (kappa_a, a_a) to (kappa_b, a_b), where
@@ -69,7 +70,8 @@ struct HitRec {
 // S is DevScene or any type with its fields Rh, r_in2 and r_out2.
 template <typename T, typename S>
 __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa, T kb,
-                                   double ab, T da, bool th, bool td, HitRec& hit, T* d2out) {
+                                   double ab, T da, bool th, bool td, HitRec<T>& hit,
+                                   T* d2out) {
   const T ra = rcp<T>(fma_t<T>(ka, L.du, L.u0));
   const T rb = rcp<T>(fma_t<T>(kb, L.du, L.u0));
   bool found = false;
@@ -92,8 +94,8 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
           best = t * t * qq;
           hit.cls = BH_HIT_HORIZON;
           hit.ang = aa;
-          hit.hx = (float)(ra + t * qx);
-          hit.hy = (float)(t * qy);
+          hit.hx = ra + t * qx;
+          hit.hy = t * qy;
         }
       }
     }
@@ -122,8 +124,8 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
           best = d2;
           hit.cls = BH_HIT_DISC;
           hit.ang = chik;
-          hit.hx = (float)px;
-          hit.hy = (float)py;
+          hit.hx = px;
+          hit.hy = py;
         }
       }
     }
@@ -138,16 +140,11 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
 // of the plane and the centre's projection c', so the reference's
 // segment/sphere test (blackhole_solution.h:65-88, the same test as the
 // horizon's) of an in-plane chord is exactly this circle test in the plane.
-// The chord from (ka, aa) to (kb, aa + da) is tested in the frame rotated by
-// aa; a hit nearer to the chord start than best (found) replaces hit.
+// The chord from P_a = (ra, 0) to P_b = (bx, by), in the frame rotated by aa,
+// is tested; a hit nearer to the chord start than best (found) replaces hit.
 template <typename T>
-__device__ inline bool sphere_chord(const DevScene& s, const Lane<T>& L, T ka, double aa, T kb,
-                                    T da, bool found, T best, HitRec& hit) {
-  const T ra = rcp<T>(fma_t<T>(ka, L.du, L.u0));
-  const T rb = rcp<T>(fma_t<T>(kb, L.du, L.u0));
-  T sd, cd;
-  sincos_t<T>(da, &sd, &cd);
-  const T bx = rb * cd, by = rb * sd;
+__device__ inline bool sphere_chord_pts(const DevScene& s, const Lane<T>& L, T ra, double aa,
+                                        T bx, T by, bool found, T best, HitRec<T>& hit) {
   const T qx = bx - ra, qy = by;
   const T qq = qx * qx + qy * qy;
   float sa, ca;
@@ -178,23 +175,52 @@ __device__ inline bool sphere_chord(const DevScene& s, const Lane<T>& L, T ka, d
       hit.cls = BH_HIT_OBJECT;
       hit.obj = i;
       hit.ang = aa;
-      hit.hx = (float)(ra + t * qx);
-      hit.hy = (float)(t * qy);
+      hit.hx = ra + t * qx;
+      hit.hy = t * qy;
     }
   }
   return found;
+}
+
+// The step chord from (ka, aa) to (kb, aa + da) against the spheres.
+template <typename T>
+__device__ inline bool sphere_chord(const DevScene& s, const Lane<T>& L, T ka, double aa, T kb,
+                                    T da, bool found, T best, HitRec<T>& hit) {
+  const T ra = rcp<T>(fma_t<T>(ka, L.du, L.u0));
+  const T rb = rcp<T>(fma_t<T>(kb, L.du, L.u0));
+  T sd, cd;
+  sincos_t<T>(da, &sd, &cd);
+  return sphere_chord_pts<T>(s, L, ra, aa, rb * cd, rb * sd, found, best, hit);
+}
+
+// A captured ray's last chord, from its last point (kappa = nstep - 0.1, at
+// the swept angle a) straight to B (blackhole_solution_test.cc:264-272): it
+// enters the horizon at distance r - R_h, unless a sphere is met before.
+template <typename T>
+__device__ inline void captured_chord(const DevScene& s, const Lane<T>& L, double a,
+                                      HitRec<T>& hit) {
+  hit.cls = BH_HIT_HORIZON;
+  hit.ang = a;
+  hit.hx = (T)s.Rh;
+  hit.hy = (T)0;
+  if (L.smask) {
+    const T ra = rcp<T>(fma_t<T>((T)s.nstep - (T)0.1, L.du, L.u0));
+    const T dh = ra - (T)s.Rh;
+    sphere_chord_pts<T>(s, L, ra, a, (T)0, (T)0, true, dh * dh, hit);
+  }
 }
 
 __device__ __forceinline__ double next_event(double a, double chi0, float guard) {
   return chi0 + (floor((a - (double)guard - chi0) * kInvPiD) + 1.0) * kPiD - (double)guard;
 }
 
+template <typename T>
 struct MarchOut {
-  HitRec hit;
+  HitRec<T> hit;
   int steps;     // steps taken
   double a;      // swept angle after the last step taken
   double du64;   // du of the ray
-  float d;       // last increment taken (for the escape direction)
+  T d;           // last increment taken (for the escape direction)
 };
 
 // Step n of the reference rule has u = u0 + kappa_n du and s = l - u =
@@ -276,7 +302,7 @@ template <typename T> struct MarchState {
 template <typename T, typename S, bool SPH = false>
 __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>& st,
                                            double first_aa, T G, T ka, T kb, bool force,
-                                           HitRec& hit) {
+                                           HitRec<T>& hit) {
   const T wn = rsq<T>(G);  // FP32: w h (setup_lane scales the cubic)
   if constexpr (sizeof(T) == 4)
     st.d = st.w + wn;
@@ -313,22 +339,22 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
 // loop to a handful of scalar instructions.
 template <typename T, bool ESCAPE, bool SPH = false>
 __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, double first_aa,
-                             double du64, MarchOut& o) {
+                             double du64, MarchOut<T>& o) {
   const int nstep = s.nstep;
   MarchState<T> st{(T)0, 0.0, (T)0};
   o.hit.cls = BH_HIT_NONE;
   o.hit.obj = 0;
   o.hit.ang = 0.0;
-  o.hit.hx = o.hit.hy = 0.f;
+  o.hit.hx = o.hit.hy = (T)0;
   o.du64 = du64;
   o.steps = 0;
   o.a = 0.0;
-  o.d = 0.f;
+  o.d = (T)0;
   const float kInf = __builtin_inff();
   auto end_ray = [&](int n) {
     o.steps = n;
     o.a = st.a;
-    o.d = (float)st.d;
+    o.d = st.d;
     L.phi_safe = kInf;
     L.a_ev = 1e308;
     L.smask = 0u;  // no sphere tests either once ended
@@ -367,10 +393,14 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     // captured: chord to the centre; the horizon is always entered
     // (r > 3M > 2M) and the disc test sees n.(B - B) = 0 (:264-272)
     if (captured && armed()) {
-      o.hit.cls = BH_HIT_HORIZON;
-      o.hit.ang = st.a;
-      o.hit.hx = s.Rhf;
-      o.hit.hy = 0.f;
+      if constexpr (SPH) {
+        captured_chord<T>(s, L, st.a, o.hit);
+      } else {
+        o.hit.cls = BH_HIT_HORIZON;
+        o.hit.ang = st.a;
+        o.hit.hx = (T)s.Rh;
+        o.hit.hy = (T)0;
+      }
       end_ray(nstep);
     }
     if (!__any(armed())) goto done;
@@ -407,9 +437,18 @@ __device__ __forceinline__ int tex_index32(float v, int n) {
   return (int)v;
 }
 
+// True when v lies within tol of an integer, a texel edge: the nearest-texel
+// index of a point within tol of v may then differ from v's (false for NaN).
+__device__ __forceinline__ bool near_edge(float v, float tol) {
+  const float f = v - __builtin_floorf(v);
+  return f < tol || f > 1.f - tol;
+}
+
 // Disc texel of a world point q (Rectangle::color, vector_object.h:163-179)
-// with r cos(theta) = s1.v/|s1| and r sin(theta) = |s1 x v|/|s1|.
-__device__ inline uint32_t disc_color32(const DevScene& s, float qx, float qy, float qz) {
+// with r cos(theta) = s1.v/|s1| and r sin(theta) = |s1 x v|/|s1|.  *redo:
+// the texel coordinates lie within the scene's guard of a texel edge.
+__device__ inline uint32_t disc_color32(const DevScene& s, float qx, float qy, float qz,
+                                        bool* redo) {
   const DevTex& t = s.disc_tex;
   const float vx = qx - s.tv1[0], vy = qy - s.tv1[1], vz = qz - s.tv1[2];
   const float a = (s.ts1[0] * vx + s.ts1[1] * vy + s.ts1[2] * vz) * s.tka;
@@ -417,16 +456,25 @@ __device__ inline uint32_t disc_color32(const DevScene& s, float qx, float qy, f
   const float cy = s.ts1[2] * vx - s.ts1[0] * vz;
   const float cz = s.ts1[0] * vy - s.ts1[1] * vx;
   const float b = __builtin_amdgcn_sqrtf(cx * cx + cy * cy + cz * cz) * s.tkb;
+  *redo = near_edge(a, s.edge_a) || near_edge(b, s.edge_b);
   return texel(t, tex_index32(a, t.cols), tex_index32(b, t.rows));
 }
 
-__device__ inline uint32_t equirect32(const DevTex& t, float dx, float dy, float dz) {
+// Equirectangular texel of a unit direction.  *redo: a direction within eps
+// radians may fall in another texel (a longitude error is the angle over
+// cos(latitude); asin's conditioning near the poles is covered by the same
+// factor on the rows).
+__device__ inline uint32_t equirect32(const DevTex& t, float dx, float dy, float dz, float eps,
+                                      bool* redo) {
   const float kPi = 3.14159265358979323846f;
   const float z = fminf(1.f, fmaxf(-1.f, dz));
   const float lon = atan2f(dy, dx);
   const float lat = asinf(z);
   const float c = (lon + kPi) * (1.0f / (2.0f * kPi)) * (float)t.cols;
   const float r = (kPi / 2.0f - lat) * (1.0f / kPi) * (float)t.rows;
+  const float e = eps * __builtin_amdgcn_rsqf(fmaxf(__builtin_fmaf(-z, z, 1.f), 1e-12f));
+  *redo = near_edge(c, e * (float)t.cols * (1.0f / (2.0f * kPi))) ||
+          near_edge(r, e * (float)t.rows * (1.0f / kPi));
   return texel(t, tex_index32(floorf(c), t.cols), tex_index32(floorf(r), t.rows));
 }
 
@@ -591,12 +639,15 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
   if (bisect) r.l = solve_g64(sqrt(cc / dd), s.M, s.l0, s.root);
 }
 
-// Hit point, colour and output planes of one ray.
+// Hit point, colour and output planes of one ray.  Returns true when the
+// ray's texel coordinate lies within the scene's edge guard of a texel edge:
+// its texel is then settled by the FP64 pass.
 template <bool ESCAPE>
-__device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t idx,
-                                  const RaySetup& r, const MarchOut& mo) {
+__device__ inline bool finish_ray(const DevScene& s, const DevOut& out, size_t idx,
+                                  const RaySetup& r, const MarchOut<float>& mo) {
   const float e1x = r.sgn * s.e1f[0], e1y = r.sgn * s.e1f[1], e1z = r.sgn * s.e1f[2];
   uint32_t rgb = 0;
+  bool redo = false;
   float qx = __builtin_nanf(""), qy = qx, qz = qx;
   if (mo.hit.cls != BH_HIT_NONE) {
     if (!r.degenerate) {
@@ -608,13 +659,14 @@ __device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t i
       qy = __builtin_fmaf(Y, r.e2y, __builtin_fmaf(X, e1y, s.Bf[1]));
       qz = __builtin_fmaf(Y, r.e2z, __builtin_fmaf(X, e1z, s.Bf[2]));
       if (mo.hit.cls == BH_HIT_DISC) {
-        rgb = disc_color32(s, qx, qy, qz);
+        rgb = disc_color32(s, qx, qy, qz, &redo);
       } else if (mo.hit.cls == BH_HIT_OBJECT) {
         // sphere texel: equirectangular lookup of the outward normal
         const float* c = s.sphf[mo.hit.obj];
         const float iR = 1.0f / c[3];
         rgb = equirect32(s.stex[mo.hit.obj], (qx - s.Bf[0] - c[0]) * iR,
-                         (qy - s.Bf[1] - c[1]) * iR, (qz - s.Bf[2] - c[2]) * iR);
+                         (qy - s.Bf[1] - c[1]) * iR, (qz - s.Bf[2] - c[2]) * iR,
+                         s.sph_edge[mo.hit.obj], &redo);
       }
     }
   } else if (ESCAPE && s.has_bg) {
@@ -629,7 +681,7 @@ __device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t i
     const float X = rb * cb - ra * ca, Y = rb * sb - ra * sa;
     const float dx = X * e1x + Y * r.e2x, dy = X * e1y + Y * r.e2y, dz = X * e1z + Y * r.e2z;
     const float inv = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
-    rgb = equirect32(s.bg, dx * inv, dy * inv, dz * inv);
+    rgb = equirect32(s.bg, dx * inv, dy * inv, dz * inv, s.edge_bg, &redo);
   }
   if (out.rgb) {
     out.rgb[3 * idx + 0] = (uint8_t)(rgb & 0xff);
@@ -644,19 +696,231 @@ __device__ inline void finish_ray(const DevScene& s, const DevOut& out, size_t i
     out.hitpt[3 * idx + 2] = (double)qz;
   }
   if (out.phi) out.phi[idx] = r.degenerate ? mo.a : atan(sqrt(r.cc) / r.dF) + mo.a;
+  return redo;
 }
 
-__device__ inline void degenerate_out(MarchOut& mo) {
+template <typename T>
+__device__ inline void degenerate_out(MarchOut<T>& mo) {
   // the pixel line passes through B: the reference marches NaNs; defined
   // here as captured / black
   mo.hit.cls = BH_HIT_HORIZON;
   mo.hit.obj = 0;
   mo.hit.ang = 0.0;
-  mo.hit.hx = mo.hit.hy = 0.f;
+  mo.hit.hx = mo.hit.hy = (T)0;
   mo.steps = 0;
   mo.a = __builtin_nan("");
   mo.du64 = 0.0;
-  mo.d = 0.f;
+  mo.d = (T)0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 finish of the FP64 event march (the FP64 pass over queued rays)
+// ---------------------------------------------------------------------------
+// Texel coordinates within this of a texel edge count as ties: the FP64 event
+// march keeps a hit point to ~1e-12 relative, far inside it, so only hit
+// points whose exact position lies on a texel edge (under a mirror symmetry
+// of the camera and the texture) are ties, and their texel is decided by the
+// rounding of the reference's own operation order.
+constexpr double kTieTexel = 1e-8;
+
+__device__ __forceinline__ bool near_edge64(double v, double tol) {
+  const double f = v - floor(v);
+  return f < tol || f > 1.0 - tol;
+}
+
+// Equirectangular texel of a unit direction in FP64 (equirect64's mapping);
+// *tie: a coordinate lies within kTieTexel of a texel edge.
+__device__ inline uint32_t equirect64_tie(const DevTex& t, double dx, double dy, double dz,
+                                          bool* tie) {
+  const double z = dz < -1.0 ? -1.0 : (dz > 1.0 ? 1.0 : dz);
+  const double c = (atan2(dy, dx) + kPiD) * (0.5 * kInvPiD) * t.cols;
+  const double r = (kPiD / 2.0 - asin(z)) * kInvPiD * t.rows;
+  *tie = near_edge64(c, kTieTexel) || near_edge64(r, kTieTexel);
+  return texel(t, tex_index(floor(c), t.cols), tex_index(floor(r), t.rows));
+}
+
+// Orbit-plane basis of pixel (x, y) in FP64 (setup_ray keeps e2 in FP32):
+// e1 = sign(Y0) F/|F|, e2 = sign(Y0) ((dv.F) F - r0^2 dv) / (r0 |pv x F|).
+__device__ inline void basis64(const DevScene& s, int x, int y, const RaySetup& r, double e1[3],
+                               double e2[3]) {
+  const double sx = s.W / 2.0 - x, sy = s.H / 2.0 - y;
+  const double pv[3] = {__builtin_fma(-s.vz[0], sy, __builtin_fma(-s.vy[0], sx, s.fvB[0])),
+                        __builtin_fma(-s.vz[1], sy, __builtin_fma(-s.vy[1], sx, s.fvB[1])),
+                        __builtin_fma(-s.vz[2], sy, __builtin_fma(-s.vy[2], sx, s.fvB[2]))};
+  const double sg = (double)r.sgn;
+  const double ke = sg * s.u0 / sqrt(r.cc);
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = sg * s.F[i] * s.u0;
+    e2[i] = __builtin_fma(r.dF, s.F[i], -s.r02 * (s.F[i] - pv[i])) * ke;
+  }
+}
+
+// The disc crossing angles of an FP64 lane from the FP64 basis (setup_lane
+// finds them with an FP32 atan2, which would limit the crossing point to
+// FP32 accuracy): n.(P - B) ~ al cos a + be sin a vanishes at
+// a = atan2(be, al) + pi/2 (mod pi).
+__device__ inline void disc_events64(const DevScene& s, const double e1[3], const double e2[3],
+                                     Lane<double>& L) {
+  L.chi0 = 1e300;
+  L.a_ev = 1e300;
+  if (!s.has_disc) return;
+  const double al = s.dn[0] * e1[0] + s.dn[1] * e1[1] + s.dn[2] * e1[2];
+  const double be = s.dn[0] * e2[0] + s.dn[1] * e2[1] + s.dn[2] * e2[2];
+  if (!(al * al + be * be > 0.0)) return;
+  L.chi0 = atan2(be, al) + kPiD / 2.0;
+  L.a_ev = next_event(0.0, L.chi0, L.guard);
+}
+
+// Hit point, colour and output planes of pixel idx marched in FP64, with the
+// ray's FP64 basis.  Returns true for a tie (a texel coordinate on a texel
+// edge to FP64 accuracy): the caller then renders the pixel with the
+// reference march.
+template <bool ESCAPE>
+__device__ inline bool finish_ray64(const DevScene& s, const DevOut& out, size_t idx,
+                                    const double e1[3], const double e2[3], const RaySetup& r,
+                                    const MarchOut<double>& mo) {
+  uint32_t rgb = 0;
+  bool tie = false;
+  const double nan = __builtin_nan("");
+  double q[3] = {nan, nan, nan};
+  if (mo.hit.cls != BH_HIT_NONE) {
+    double sa, ca;
+    sincos_small64(mo.hit.ang, &sa, &ca);
+    const double X = mo.hit.hx * ca - mo.hit.hy * sa;
+    const double Y = mo.hit.hx * sa + mo.hit.hy * ca;
+    for (int i = 0; i < 3; ++i) q[i] = __builtin_fma(Y, e2[i], __builtin_fma(X, e1[i], s.B[i]));
+    if (mo.hit.cls == BH_HIT_DISC) {
+      // Rectangle::color (vector_object.h:163-179) in closed form
+      const DevTex& t = s.disc_tex;
+      double s1[3], s2[3], v[3];
+      for (int i = 0; i < 3; ++i) {
+        s1[i] = s.dv2[i] - s.dv1[i];
+        s2[i] = s.dv4[i] - s.dv1[i];
+        v[i] = q[i] - s.dv1[i];
+      }
+      const double l11 = s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2];
+      const double l22 = s2[0] * s2[0] + s2[1] * s2[1] + s2[2] * s2[2];
+      const double cx = s1[1] * v[2] - s1[2] * v[1];
+      const double cy = s1[2] * v[0] - s1[0] * v[2];
+      const double cz = s1[0] * v[1] - s1[1] * v[0];
+      const double a = (s1[0] * v[0] + s1[1] * v[1] + s1[2] * v[2]) / l11 * t.cols;
+      const double b = sqrt((cx * cx + cy * cy + cz * cz) / (l11 * l22)) * t.rows;
+      tie = near_edge64(a, kTieTexel) || near_edge64(b, kTieTexel);
+      rgb = texel(t, tex_index(a, t.cols), tex_index(b, t.rows));
+    } else if (mo.hit.cls == BH_HIT_OBJECT) {
+      const double iR = 1.0 / s.sr[mo.hit.obj];
+      const double* c = s.sc[mo.hit.obj];
+      rgb = equirect64_tie(s.stex[mo.hit.obj], (q[0] - c[0]) * iR, (q[1] - c[1]) * iR,
+                           (q[2] - c[2]) * iR, &tie);
+    }
+  } else if (ESCAPE && s.has_bg) {
+    // direction of the last chord; outbound points are one du apart, the
+    // last one at sigma = 0.1 + steps - nstep
+    const double kb = (double)s.nstep - (0.1 + (double)(mo.steps - s.nstep));
+    const double rb = 1.0 / __builtin_fma(kb, mo.du64, s.u0);
+    const double ra = 1.0 / __builtin_fma(kb + 1.0, mo.du64, s.u0);
+    double sb, cb, sa, ca;
+    sincos_small64(mo.a, &sb, &cb);
+    sincos_small64(mo.a - mo.d, &sa, &ca);
+    const double X = rb * cb - ra * ca, Y = rb * sb - ra * sa;
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = X * e1[i] + Y * e2[i];
+    const double inv = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    rgb = equirect64_tie(s.bg, d[0] * inv, d[1] * inv, d[2] * inv, &tie);
+  }
+  if (out.rgb) {
+    out.rgb[3 * idx + 0] = (uint8_t)(rgb & 0xff);
+    out.rgb[3 * idx + 1] = (uint8_t)((rgb >> 8) & 0xff);
+    out.rgb[3 * idx + 2] = (uint8_t)((rgb >> 16) & 0xff);
+  }
+  if (out.hit) out.hit[idx] = (uint8_t)mo.hit.cls;
+  if (out.steps) out.steps[idx] = mo.steps;
+  if (out.hitpt) {
+    out.hitpt[3 * idx + 0] = q[0];
+    out.hitpt[3 * idx + 1] = q[1];
+    out.hitpt[3 * idx + 2] = q[2];
+  }
+  if (out.phi) out.phi[idx] = atan(sqrt(r.cc) / r.dF) + mo.a;
+  return tie;
+}
+
+// FP64 refinement of a texel-edge ray whose FP32 march ended at step n with
+// class cls (a disc hit, or no hit in an escape scene): the swept angle of
+// steps 1..n in FP64 with no object tests (the same work on every lane, so
+// the rays of a wave, taken from all over the frame, do not diverge), then
+// the last chord's disc test in FP64.  Returns false when that outcome does
+// not hold in FP64 (the caller then runs the FP64 event march).  The earlier
+// chords are taken to hit nothing, as the FP32 march found: an FP32 miss of
+// an earlier hit needs a crossing within FP32 rounding of the disc's rim,
+// independent of the texel-edge condition that queued the ray.
+template <bool ESCAPE>
+__device__ inline bool refine64(const DevScene& s, Lane<double> L, const RaySetup& r, double du,
+                                int cls, int n, MarchOut<double>& o) {
+  const int nstep = s.nstep;
+  if (n < 1 || (r.captured && n > nstep)) return false;
+  if (cls != BH_HIT_DISC && !(ESCAPE && cls == BH_HIT_NONE)) return false;
+  double w = 0.0, a = 0.0, a_prev = 0.0, d = 0.0, ka = 0.0, kb = 0.0;
+  for (int k = 1; k <= n; ++k) {
+    if (k == nstep) to_outbound<double>(L);
+    double G;
+    step_coeffs<double>(k, nstep, L, &G, &ka, &kb);
+    const double wn = rsq<double>(G);
+    d = (w + wn) * L.h;
+    w = wn;
+    a_prev = a;
+    a += d;
+  }
+  o.steps = n;
+  o.a = a;
+  o.d = d;
+  o.du64 = du;
+  o.hit.cls = BH_HIT_NONE;
+  o.hit.obj = 0;
+  o.hit.ang = 0.0;
+  o.hit.hx = o.hit.hy = 0.0;
+  if (cls == BH_HIT_NONE) return true;
+  // the first chord of a ray with Y0 < 0 starts at the camera (angle pi)
+  const double aa = n == 1 ? r.first_aa : a_prev;
+  double d2;
+  return event_chord<double, DevScene>(s, L, ka, aa, kb, a, a - aa, false, true, o.hit, &d2) &&
+         o.hit.cls == BH_HIT_DISC;
+}
+
+// FP64 march of one queued ray: a texel-edge ray's FP32 outcome refined in
+// FP64 (refine64), or, for a band ray and when the refinement fails, the event
+// march in FP64; both with an FP64 basis, crossing angles and finish.  A texel
+// tie, whose texel the reference's rounding order decides, is replayed in
+// that order (replay64) for the outcome found.  The reference march itself
+// (march64_kernel's per-pixel code) serves a degenerate ray, a replay that
+// disagrees, and scenes with spheres (the event march tests spheres with an
+// FP32 table).
+template <bool ESCAPE, bool SPH>
+__device__ inline void fp64_ray(const DevScene& s, const DevOut& out, size_t idx, int x, int y,
+                                uint32_t outcome) {
+  Pixel64 px;
+  if (!SPH) {
+    RaySetup r;
+    setup_ray(s, x, y, false, r);
+    if (!r.degenerate) {
+      double e1[3], e2[3];
+      basis64(s, x, y, r, e1, e2);
+      Lane<double> L;
+      const double du = (r.l - s.u0) * s.inv_nstep;
+      setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
+      disc_events64(s, e1, e2, L);
+      MarchOut<double> mo;
+      if (!(outcome != 0u && refine64<ESCAPE>(s, L, r, du, (int)(outcome >> 28),
+                                              (int)(outcome & 0x0fffffffu), mo)))
+        march<double, ESCAPE, false>(s, L, r.captured, r.first_aa, du, mo);
+      if (!finish_ray64<ESCAPE>(s, out, idx, e1, e2, r, mo)) return;
+      if (replay64(s, x, y, mo.steps, mo.hit.cls, px)) {
+        store_pixel64(out, idx, px);
+        return;
+      }
+    }
+  }
+  trace64(s, x, y, px);
+  store_pixel64(out, idx, px);
 }
 
 }  // namespace bh

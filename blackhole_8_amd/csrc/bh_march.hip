@@ -4,9 +4,11 @@
 //                  (include/blackhole/blackhole_solution_test.cc:161-298).
 //  march_kernel    FP32 event march: FP64 set-up, a 3-FMA + 1-rsqrt step in
 //                  the orbit plane, exact object tests only on the steps where
-//                  an object can be reached; rays in the photon-sphere band
-//                  are queued for band_kernel.
-//  band_kernel     the same event march in FP64 for the queued band rays.
+//                  an object can be reached; rays in the photon-sphere band,
+//                  and rays whose texel lies within the edge guard of a texel
+//                  edge, are queued for band_kernel.
+//  band_kernel     the FP64 reference march (march64_kernel's per-pixel code)
+//                  of the queued rays.
 //  march_frames_kernel / band_frames_kernel
 //                  the same two passes over every frame of a camera path in
 //                  one launch each.
@@ -28,22 +30,36 @@ __global__ __launch_bounds__(256) void march64_kernel(const DevScene* __restrict
   store_pixel64(out, (size_t)y * s.W + x, px);
 }
 
-// One ray of the FP32 march: pixel (x, y) of frame s.
+// Append band ray idx to the front of sub-queue k (wave-uniform; the compiler
+// turns the lanes' increments into one atomic per wave).  False if it did not
+// fit, which the sub-queue sizes rule out.
+__device__ __forceinline__ bool queue_push_band(const DevQueue& q, uint32_t k, uint32_t idx) {
+  const uint32_t slot = atomicAdd(q.count + 16 * k, 1u);
+  if (slot >= q.sub_cap) return false;
+  q.items[(size_t)k * q.sub_cap + slot] = (uint64_t)idx;
+  return true;
+}
+
+// Append texel-edge ray idx and its FP32 outcome to the back of sub-queue k.
+__device__ __forceinline__ void queue_push_edge(const DevQueue& q, uint32_t k, uint32_t idx,
+                                                uint32_t outcome) {
+  const uint32_t slot = atomicAdd(q.count + 16 * k + 2, 1u);
+  if (slot < q.sub_cap)
+    q.items[(size_t)k * q.sub_cap + (q.sub_cap - 1u - slot)] =
+        (uint64_t)idx | ((uint64_t)outcome << 32);
+}
+
+// One ray of the FP32 march: pixel (x, y) of frame s, marched by a workgroup
+// of the frame whose queue is sub-queue k.
 template <bool ESCAPE, bool SPH>
 __device__ __forceinline__ void march_pixel(const DevScene& s, const DevOut& out,
-                                            const DevQueue& q, int x, int y) {
+                                            const DevQueue& q, uint32_t k, int x, int y) {
   const size_t idx = (size_t)y * s.W + x;
   RaySetup r;
   setup_ray(s, x, y, false, r);
-  if (r.band && !r.degenerate) {
-    // queued for the FP64 march; a full queue marches the ray here in FP32
-    const uint32_t slot = atomicAdd(q.count, 1u);
-    if (slot < q.capacity) {
-      q.items[slot] = (uint32_t)idx;
-      return;
-    }
-  }
-  MarchOut mo;
+  // queued for the FP64 march (a full queue would march the ray here)
+  if (r.band && !r.degenerate && queue_push_band(q, k, (uint32_t)idx)) return;
+  MarchOut<float> mo;
   if (r.degenerate) {
     degenerate_out(mo);
   } else {
@@ -52,7 +68,10 @@ __device__ __forceinline__ void march_pixel(const DevScene& s, const DevOut& out
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
     march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
   }
-  finish_ray<ESCAPE>(s, out, idx, r, mo);
+  // the texel lies within the FP32 error of a texel edge: the FP64 pass
+  // refines the ray and overwrites what was just stored
+  if (finish_ray<ESCAPE>(s, out, idx, r, mo))
+    queue_push_edge(q, k, (uint32_t)idx, queue_outcome(mo.hit.cls, mo.steps));
 }
 
 template <bool ESCAPE, bool SPH>
@@ -62,22 +81,21 @@ __global__ __launch_bounds__(256) void march_kernel(const DevScene* __restrict__
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= s.W || y >= s.H) return;
-  march_pixel<ESCAPE, SPH>(s, out, q, x, y);
+  march_pixel<ESCAPE, SPH>(s, out, q, queue_sub(blockIdx.x, blockIdx.y), x, y);
 }
 
-// Per-frame planes and queue slice of frame f of a camera path: planes at
-// offset f npix, queue slice f of qstride items and its count pair.
+// Per-frame planes and queue of frame f of a camera path: planes at offset
+// f npix, the queue kQueueSubs sub-queues further per frame.
 __device__ __forceinline__ void frame_slice(const DevOut& out, const DevQueue& q, int f,
-                                            size_t npix, uint32_t qstride, DevOut* o,
-                                            DevQueue* qf) {
+                                            size_t npix, DevOut* o, DevQueue* qf) {
   o->rgb = out.rgb ? out.rgb + (size_t)f * npix * 3 : nullptr;
   o->hit = out.hit ? out.hit + (size_t)f * npix : nullptr;
   o->phi = out.phi ? out.phi + (size_t)f * npix : nullptr;
   o->hitpt = out.hitpt ? out.hitpt + (size_t)f * npix * 3 : nullptr;
   o->steps = out.steps ? out.steps + (size_t)f * npix : nullptr;
-  qf->items = q.items + (size_t)f * qstride;
-  qf->count = q.count + 16 * f;
-  qf->capacity = q.capacity;
+  qf->items = q.items + (size_t)f * kQueueSubs * q.sub_cap;
+  qf->count = q.count + (size_t)f * kQueueCountWords;
+  qf->sub_cap = q.sub_cap;
 }
 
 // Every frame of a camera path in one launch (blockIdx.z = frame - frame0):
@@ -85,7 +103,7 @@ __device__ __forceinline__ void frame_slice(const DevOut& out, const DevQueue& q
 template <bool ESCAPE, bool SPH>
 __global__ __launch_bounds__(256) void march_frames_kernel(const DevScene* __restrict__ sp,
                                                            DevOut out, DevQueue q, size_t npix,
-                                                           uint32_t qstride, int frame0) {
+                                                           int frame0) {
   const int f = frame0 + (int)blockIdx.z;
   const DevScene& s = sp[f];
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
@@ -93,60 +111,99 @@ __global__ __launch_bounds__(256) void march_frames_kernel(const DevScene* __res
   if (x >= s.W || y >= s.H) return;
   DevOut o;
   DevQueue qf;
-  frame_slice(out, q, f, npix, qstride, &o, &qf);
-  march_pixel<ESCAPE, SPH>(s, o, qf, x, y);
+  frame_slice(out, q, f, npix, &o, &qf);
+  march_pixel<ESCAPE, SPH>(s, o, qf, queue_sub(blockIdx.x, blockIdx.y), x, y);
 }
 
-// FP64 march of the photon-sphere band rays queued by the FP32 march.  The
-// last of the nblk workgroups serving the queue clears it for the next frame.
+// FP64 pass over one frame's queue, by one-wave workgroups of two kinds, so
+// that no wave mixes the band rays' full FP64 march with the texel-edge rays'
+// short refinement: workgroups blk < kQueueSubs * R take the texel-edge rays
+// (the back of the sub-queues; blk takes part blk / kQueueSubs of sub-queue
+// blk mod kQueueSubs), the nband workgroups after them the band rays of all
+// sub-queues packed together (the front segments, indexed through their
+// prefix sum).  The last workgroup to finish clears the frame's counts for
+// the next frame; count[1] of sub-queue 0 counts the finished workgroups.
 template <bool ESCAPE, bool SPH>
 __device__ __forceinline__ void band_rays(const DevScene& s, const DevOut& out, const DevQueue& q,
-                                          uint32_t blk, uint32_t nblk) {
-  const uint32_t n = min(q.count[0], q.capacity);
-  for (uint32_t i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
-    const uint32_t idx = q.items[i];
-    const int x = (int)(idx % (uint32_t)s.W), y = (int)(idx / (uint32_t)s.W);
-    RaySetup r;
-    setup_ray(s, x, y, true, r);
-    MarchOut mo;
-    Lane<double> L;
-    const double du = (r.l - s.u0) * s.inv_nstep;
-    setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
-    march<double, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
-    finish_ray<ESCAPE>(s, out, idx, r, mo);
+                                          uint32_t blk, uint32_t R, uint32_t nband) {
+  __shared__ uint32_t pre[kQueueSubs + 1];
+  __shared__ uint32_t last;
+  const uint32_t lane = threadIdx.x;
+  if (blk < kQueueSubs * R) {
+    const uint32_t k = blk % kQueueSubs, part = blk / kQueueSubs;
+    const uint32_t* count = q.count + 16 * k;
+    const uint64_t* items = q.items + (size_t)k * q.sub_cap;
+    const uint32_t nb = min(count[0], q.sub_cap);
+    const uint32_t ne = min(count[2], q.sub_cap - nb);
+#pragma unroll 1
+    for (uint32_t i = part * 64 + lane; i < ne; i += R * 64) {
+      const uint64_t it = items[q.sub_cap - 1u - i];
+      const uint32_t idx = (uint32_t)it;
+      fp64_ray<ESCAPE, SPH>(s, out, idx, (int)(idx % (uint32_t)s.W),
+                            (int)(idx / (uint32_t)s.W), (uint32_t)(it >> 32));
+    }
+  } else {
+    // pre[k] = band rays of sub-queues 0 .. k-1
+    pre[lane + 1] = min(q.count[16 * lane], q.sub_cap);
+    __syncthreads();
+    if (lane == 0) {
+      pre[0] = 0;
+      for (uint32_t k = 1; k <= kQueueSubs; ++k) pre[k] += pre[k - 1];
+    }
+    __syncthreads();
+    const uint32_t total = pre[kQueueSubs];
+#pragma unroll 1
+    for (uint32_t j = (blk - kQueueSubs * R) * 64 + lane; j < total; j += nband * 64) {
+      uint32_t lo = 0, hi = kQueueSubs;  // the k with pre[k] <= j < pre[k + 1]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= j)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const uint32_t idx = (uint32_t)q.items[(size_t)lo * q.sub_cap + (j - pre[lo])];
+      fp64_ray<ESCAPE, SPH>(s, out, idx, (int)(idx % (uint32_t)s.W),
+                            (int)(idx / (uint32_t)s.W), 0u);
+    }
   }
-  // Every lane's read of count[0] has returned before the barrier, so the
-  // reset cannot race with it; the queue is read again only by later
+  // Every lane's reads of the counts have returned before the barrier, so
+  // the reset cannot race with them; the queue is read again only by later
   // launches.  (No device-scope fence here: on this multi-XCD part it writes
   // back and invalidates the XCD's L2, full of framebuffer lines, in every
   // workgroup.)
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t done = atomicAdd(q.count + 1, 1u);
-    if (done == nblk - 1) {
-      atomicExch(q.count, 0u);
-      atomicExch(q.count + 1, 0u);
-    }
+  if (lane == 0) last = atomicAdd(q.count + 1, 1u) == kQueueSubs * R + nband - 1u ? 1u : 0u;
+  __syncthreads();
+  if (last) {
+    atomicExch(q.count + 16 * lane, 0u);
+    atomicExch(q.count + 16 * lane + 2, 0u);
+    if (lane == 0) atomicExch(q.count + 1, 0u);
   }
 }
+
+// Workgroups of the FP64 pass of one frame: edge workgroups per sub-queue and
+// band workgroups.
+constexpr uint32_t kEdgeParts1 = 16, kBandWaves1 = 16;     // one frame alone
+constexpr uint32_t kEdgePartsN = 2, kBandWavesN = 8;       // each frame of a path
 
 template <bool ESCAPE, bool SPH>
 __global__ __launch_bounds__(64) void band_kernel(const DevScene* __restrict__ sp, DevOut out,
                                                   DevQueue q) {
-  band_rays<ESCAPE, SPH>(*sp, out, q, blockIdx.x, gridDim.x);
+  band_rays<ESCAPE, SPH>(*sp, out, q, blockIdx.x, kEdgeParts1, kBandWaves1);
 }
 
-// The band rays of every frame of a path: blockIdx.y = frame - frame0, the
-// gridDim.x workgroups of a row serve that frame's queue slice.
+// The queued rays of every frame of a path: blockIdx.y = frame - frame0, the
+// gridDim.x workgroups of a row serve that frame's queue.
 template <bool ESCAPE, bool SPH>
 __global__ __launch_bounds__(64) void band_frames_kernel(const DevScene* __restrict__ sp,
                                                          DevOut out, DevQueue q, size_t npix,
-                                                         uint32_t qstride, int frame0) {
+                                                         int frame0) {
   const int f = frame0 + (int)blockIdx.y;
   DevOut o;
   DevQueue qf;
-  frame_slice(out, q, f, npix, qstride, &o, &qf);
-  band_rays<ESCAPE, SPH>(sp[f], o, qf, blockIdx.x, gridDim.x);
+  frame_slice(out, q, f, npix, &o, &qf);
+  band_rays<ESCAPE, SPH>(sp[f], o, qf, blockIdx.x, kEdgePartsN, kBandWavesN);
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, int n) {
@@ -181,68 +238,69 @@ hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, in
   return hipGetLastError();
 }
 
-// Band grid: 64 workgroups of one wave; every workgroup reaches the queue
-// reset, so the queue is clear when the launch completes.
+// FP64 pass of one frame; the last workgroup clears the queue, so it is clear
+// when the launch completes.
 hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,
                        hipStream_t stream, bool escape, bool spheres) {
+  const dim3 g(kEdgeParts1 * kQueueSubs + kBandWaves1), b(64);
   if (spheres) {
     if (escape)
-      hipLaunchKernelGGL((band_kernel<true, true>), dim3(64), dim3(64), 0, stream, scene, out, q);
+      hipLaunchKernelGGL((band_kernel<true, true>), g, b, 0, stream, scene, out, q);
     else
-      hipLaunchKernelGGL((band_kernel<false, true>), dim3(64), dim3(64), 0, stream, scene, out, q);
+      hipLaunchKernelGGL((band_kernel<false, true>), g, b, 0, stream, scene, out, q);
   } else {
     if (escape)
-      hipLaunchKernelGGL((band_kernel<true, false>), dim3(64), dim3(64), 0, stream, scene, out, q);
+      hipLaunchKernelGGL((band_kernel<true, false>), g, b, 0, stream, scene, out, q);
     else
-      hipLaunchKernelGGL((band_kernel<false, false>), dim3(64), dim3(64), 0, stream, scene, out, q);
+      hipLaunchKernelGGL((band_kernel<false, false>), g, b, 0, stream, scene, out, q);
   }
   return hipGetLastError();
 }
 
 // Frames [frame0, frame0 + nf) of a camera path, nf <= 65535.
 hipError_t launch_march_frames(const DevScene* scenes, const DevOut& out, int W, int H,
-                               int frame0, int nf, const DevQueue& q, uint32_t qstride,
-                               hipStream_t stream, bool escape, bool spheres) {
+                               int frame0, int nf, const DevQueue& q, hipStream_t stream,
+                               bool escape, bool spheres) {
   const dim3 grid((W + 15) / 16, (H + 15) / 16, nf);
   const size_t npix = (size_t)W * H;
   if (spheres) {
     if (escape)
       hipLaunchKernelGGL((march_frames_kernel<true, true>), grid, dim3(256), 0, stream, scenes,
-                         out, q, npix, qstride, frame0);
+                         out, q, npix, frame0);
     else
       hipLaunchKernelGGL((march_frames_kernel<false, true>), grid, dim3(256), 0, stream, scenes,
-                         out, q, npix, qstride, frame0);
+                         out, q, npix, frame0);
   } else {
     if (escape)
       hipLaunchKernelGGL((march_frames_kernel<true, false>), grid, dim3(256), 0, stream, scenes,
-                         out, q, npix, qstride, frame0);
+                         out, q, npix, frame0);
     else
       hipLaunchKernelGGL((march_frames_kernel<false, false>), grid, dim3(256), 0, stream, scenes,
-                         out, q, npix, qstride, frame0);
+                         out, q, npix, frame0);
   }
   return hipGetLastError();
 }
 
-// Band grid: 64 one-wave workgroups per frame.
+// FP64 pass of a path: one row of workgroups per frame.
 hipError_t launch_band_frames(const DevScene* scenes, const DevOut& out, int W, int H,
-                              int frame0, int nf, const DevQueue& q, uint32_t qstride,
-                              hipStream_t stream, bool escape, bool spheres) {
-  const dim3 grid(64, nf);
+                              int frame0, int nf, const DevQueue& q, hipStream_t stream,
+                              bool escape, bool spheres) {
+  const dim3 g(kEdgePartsN * kQueueSubs + kBandWavesN, nf), b(64);
   const size_t npix = (size_t)W * H;
   if (spheres) {
     if (escape)
-      hipLaunchKernelGGL((band_frames_kernel<true, true>), grid, dim3(64), 0, stream, scenes, out,
-                         q, npix, qstride, frame0);
+      hipLaunchKernelGGL((band_frames_kernel<true, true>), g, b, 0, stream, scenes, out, q, npix,
+                         frame0);
     else
-      hipLaunchKernelGGL((band_frames_kernel<false, true>), grid, dim3(64), 0, stream, scenes,
-                         out, q, npix, qstride, frame0);
+      hipLaunchKernelGGL((band_frames_kernel<false, true>), g, b, 0, stream, scenes, out, q, npix,
+                         frame0);
   } else {
     if (escape)
-      hipLaunchKernelGGL((band_frames_kernel<true, false>), grid, dim3(64), 0, stream, scenes,
-                         out, q, npix, qstride, frame0);
+      hipLaunchKernelGGL((band_frames_kernel<true, false>), g, b, 0, stream, scenes, out, q, npix,
+                         frame0);
     else
-      hipLaunchKernelGGL((band_frames_kernel<false, false>), grid, dim3(64), 0, stream, scenes,
-                         out, q, npix, qstride, frame0);
+      hipLaunchKernelGGL((band_frames_kernel<false, false>), g, b, 0, stream, scenes, out, q,
+                         npix, frame0);
   }
   return hipGetLastError();
 }

@@ -111,10 +111,11 @@ def annulus(p1, p2, p3, p4, r_outer: float, r_inner: float, texture: int) -> abi
 
 def options(precision=abi.PREC_FP64, nstep=20, escape=False, r_escape=0.0, step_cap=10000,
             filter=abi.FILTER_NEAREST, root=abi.ROOT_BISECT20, band=1e-3,
-            edge_width=0.0) -> abi.Options:
+            edge_width=0.0, edge_guard=0.0) -> abi.Options:
     o = abi.Options()
     o.precision, o.nstep, o.escape, o.step_cap = precision, nstep, int(escape), step_cap
     o.r_escape, o.filter, o.root, o.band, o.edge_width = r_escape, filter, root, band, edge_width
+    o.edge_guard = edge_guard
     return o
 
 

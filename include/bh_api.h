@@ -138,6 +138,12 @@ typedef struct bh_options {
   int32_t root;      /* BH_ROOT_*                                           */
   double band;       /* FP32 mode: rays with |b/b_c - 1| < band march in FP64 */
   double edge_width; /* bilinear mode only: soft disc-edge width (0 = hard) */
+  double edge_guard; /* FP32 mode: bound on the relative error of an FP32 hit
+                        point or escape direction.  A ray whose disc, sphere or
+                        background texel coordinate lies within that error of a
+                        texel edge is marched again in FP64 in the reference's
+                        operation order, so its texel is the FP64 march's.
+                        0: the default 2e-6; negative: no re-march.           */
 } bh_options;
 
 /* ---- per-pixel outputs (NULL = not wanted) ------------------------------ */
@@ -177,7 +183,8 @@ typedef struct bh_rect {
 typedef struct bh_plane {
   double position[3];
   double axis_x[3], axis_y[3], axis_z[3];
-  double pattern_size; /* ChessPattern2D(size) (pattern.h:14-50); 0 = black */
+  double pattern_size; /* ChessPattern2D(size) (pattern.h:14-50), a whole
+                          number in 1..2^29 (the reference's int); 0 = black */
 } bh_plane;
 
 typedef struct bh_flat_scene {

@@ -926,8 +926,12 @@ static int plane_collide(const bh_plane* pl, v3 p1, v3 p2, v3* inter) {
 }
 
 /* ChessPattern2D (pattern.h:14-50) */
+/* ChessPattern2D::mod (pattern.h:44-47).  Beyond the int range, where the
+ * reference's (int) cast is undefined, the same truncated quotient is taken
+ * in double arithmetic. */
 static double chess_mod(double a, double ps) {
   double b = fabs(a);
+  if (!(b < 2147483648.0)) return b - floor(floor(b) / (ps * 2)) * (ps * 2);
   return b - ((int)(b) / (int)(ps * 2)) * (ps * 2);
 }
 

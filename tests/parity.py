@@ -55,13 +55,15 @@ def disc_texel_coords(hitpt: np.ndarray, vertex: np.ndarray, rows: int, cols: in
     return proj / l1 * cols, perp / np.linalg.norm(s2) * rows
 
 
-def straddles(a: np.ndarray, b: np.ndarray, tol: float = 1e-3) -> np.ndarray:
-    """True where the span between two continuous texel coordinates, widened
-    by tol texels, contains a texel edge: the nearest-texel lookups of the two
-    points may then legitimately differ."""
+def straddles(a: np.ndarray, b: np.ndarray, tol: float = 1e-3,
+              max_span: float = 0.01) -> np.ndarray:
+    """True where two continuous texel coordinates at most max_span texels
+    apart lie on either side of a texel edge (the span widened by tol): a
+    nearest-texel difference between two nearly equal points.  Points further
+    apart are never a straddle."""
     lo = np.minimum(a, b) - tol
     hi = np.maximum(a, b) + tol
-    return np.floor(hi) > np.floor(lo)
+    return (np.floor(hi) > np.floor(lo)) & (np.abs(a - b) <= max_span)
 
 
 def oracle_disc_texel(hitpt: np.ndarray, vertex, rows: int, cols: int):

@@ -4,9 +4,16 @@ Bars (from the workload definitions):
   FP64 march  hit class and RGB identical except a stated fraction <= 1e-4 of
               pixels; phi_end within 1e-9 rad; step counts identical.
   FP32 march  hit class identical except <= 1e-4 of pixels; RGB within 1/255
-              except pixels whose disc texel coordinate straddles a texel edge
-              between the two renders (nearest-texel lookup of a noise texture);
-              phi_end within 1e-5 rad on pixels with the same class and steps.
+              except <= 1e-4 of pixels (no exemption: rays whose FP32 texel
+              coordinate lies near a texel edge are re-marched in FP64);
+              phi_end within 1e-5 rad on pixels with the same class and steps;
+              hit points within 1e-4 (world units, scenes of r_s = 1) on
+              pixels with the same class.
+The FP64 march and the oracle both follow the reference's operation order, but
+the device's sin/cos/acos may differ from the host libm's in the last bit, which
+decides the texel of the few hit points that lie exactly on a texel edge (the
+texture's centre lines, under the camera's mirror symmetry): a few pixels per
+million.
 """
 import ctypes as C
 import math
@@ -77,22 +84,34 @@ def _check_fp32(renderer, p, name):
     return _check_fp32_outputs(p, g, r, name)
 
 
+RGB_FRAC = 1e-4      # RGB mismatches (same class), fraction of pixels
+HITPT_ERR = 1e-4     # hit point error on pixels of the same class
+
+
+def _allowed(n_pixels, frac=RGB_FRAC):
+    """Mismatch count allowed by a fraction of a frame: at least one pixel,
+    since a texel tie on a centre line can fall either way."""
+    return max(1, int(frac * n_pixels))
+
+
 def _check_fp32_outputs(p, g, r, name):
     st = compare(g, r)
     _log(name, st)
-    # RGB mismatches must all be disc texel straddles
     bad = st["rgb_bad_mask"]
-    if bad.any():
+    if bad.any() and p.scene.has_disc:
+        # diagnostics only: how many are straddles of nearly equal points
         tex = p.textures[scenes.SLOT_DISC]
         vert = np.array(p.scene.disc.vertex)
-        ga, gb = disc_texel_coords(g["hitpt"][bad], vert, tex.shape[0], tex.shape[1])
-        ra, rb = oracle_disc_texel(r["hitpt"][bad], vert, tex.shape[0], tex.shape[1])
-        explained = (straddles(ga, ra) | straddles(gb, rb)) & (g["hit"][bad] == abi.HIT_DISC)
-        print(f"[parity] {name}: rgb mismatches {int(bad.sum())}, "
-              f"texel straddles {int(explained.sum())}", flush=True)
-        assert explained.all()
-    assert st["class_mismatch_frac"] <= 1e-4
+        sel = bad & (g["hit"] == abi.HIT_DISC)
+        ga, gb = disc_texel_coords(g["hitpt"][sel], vert, tex.shape[0], tex.shape[1])
+        ra, rb = oracle_disc_texel(r["hitpt"][sel], vert, tex.shape[0], tex.shape[1])
+        near = straddles(ga, ra) | straddles(gb, rb)
+        print(f"[parity] {name}: rgb mismatches {int(bad.sum())}, disc texel straddles "
+              f"within 0.01 texel {int(near.sum())}", flush=True)
+    assert st["class_mismatch"] <= _allowed(st["pixels"])
+    assert st["rgb_mismatch"] <= _allowed(st["pixels"])
     assert st["phi_max_err"] <= 1e-5
+    assert st["hit_point_max_err"] <= HITPT_ERR
     return st
 
 
@@ -105,7 +124,64 @@ def test_fp32_disc_orbit_small_matches_oracle(renderer):
 def test_fp32_disc_orbit_1080p_matches_oracle(renderer):
     print("[case] fp32 disc_orbit 1920x1080 frame 0", flush=True)
     st = _check_fp32(renderer, scenes.disc_orbit(frames=1), "fp32 disc_orbit 1080p")
-    assert st["rgb_mismatch_frac"] <= 1e-3
+    assert st["rgb_mismatch_frac"] <= 1e-4
+
+
+def test_fp32_path_frames_1080p_match_oracle(renderer):
+    # the benchmarked workload itself: the 120-frame 1080p path as one graph
+    # (one march launch over all frames, one FP64 launch over their queues);
+    # frames 0, 59 and 119 of its output against the oracle
+    p = scenes.disc_orbit(frames=120)
+    renderer.set_textures(p.textures)
+    print("[case] fp32 disc_orbit path 120 frames 1920x1080", flush=True)
+    path = renderer.path(p.scene, p.options, p.cameras)
+    try:
+        path.launch()
+        path.launch()  # a replay leaves the same frames
+        for k in (0, 59, 119):
+            print(f"[case] path frame {k}", flush=True)
+            rgb, hit = path.read(k)
+            r = oracle.render_preset(p.with_camera(p.cameras[k]), THREADS, ("rgb", "hit"))
+            st = compare({"rgb": rgb, "hit": hit}, r)
+            _log(f"path frame {k}", st)
+            assert st["class_mismatch_frac"] <= 1e-4
+            assert st["rgb_mismatch_frac"] <= 1e-4
+    finally:
+        path.close()
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP32, abi.PREC_FP64])
+def test_camera_turned_away_matches_oracle(renderer, precision):
+    # quirk Q2: phi0 = atan(b / Y0) with Y0 < 0 for pixel lines that point
+    # away from the hole (blackhole_solution_test.cc:193).  The camera, 1.5
+    # above the disc at 8 from the hole, looks 100 degrees away from it, so
+    # most of its 60-degree frame has Y0 < 0 (phi0 then points away from the
+    # camera, and those rays end mostly on the horizon, as in the reference);
+    # the FP32 march starts their first chord at the camera (angle pi in its
+    # frame) and forces the first step's tests.  The rest of the frame sees
+    # the disc.
+    import math
+
+    cam = scenes.look_at(480, 270, math.pi / 3, (-8.0, 0.0, 1.5),
+                         target=(-8.0 + 20 * math.cos(math.radians(100)),
+                                 20 * math.sin(math.radians(100)), 1.5))
+    p = scenes.disc_orbit(480, 270, frames=1, precision=precision).with_camera(cam)
+    print(f"[case] camera turned away 480x270 precision {precision}", flush=True)
+    g = renderer.render_preset(p, ALL)
+    r = oracle.render_preset(p, THREADS, ALL)
+    y0neg = sum(oracle.ray_setup(p.scene, p.options, x, y).Fp[1] < 0
+                for y in range(0, 270, 15) for x in range(0, 480, 15))
+    assert 100 < y0neg < 18 * 32 - 100  # both signs of Y0
+    assert (r["hit"] == abi.HIT_DISC).sum() > 5000
+    if precision == abi.PREC_FP64:
+        st = compare(g, r)
+        _log("camera turned away fp64", st)
+        assert st["class_mismatch"] <= _allowed(st["pixels"])
+        assert st["rgb_mismatch"] <= _allowed(st["pixels"])
+        assert st["steps_mismatch"] == 0
+        assert st["phi_max_err"] <= 1e-9
+    else:
+        _check_fp32_outputs(p, g, r, "fp32 camera turned away")
 
 
 def test_fp32_disc_orbit_other_frames(renderer):
@@ -128,16 +204,41 @@ def _check_spheres(renderer, p, name):
 def test_fp32_spheres_spheres_match_oracle(renderer):
     print("[case] fp32 spheres 640x360 (16 spheres, escape, background)", flush=True)
     st = _check_spheres(renderer, scenes.spheres(640, 360), "fp32 spheres 640x360")
-    assert st["class_mismatch_frac"] <= 1e-4
-    assert st["rgb_mismatch_frac"] <= 2e-3
+    assert st["class_mismatch"] <= _allowed(st["pixels"])
+    assert st["rgb_mismatch"] <= _allowed(st["pixels"])
     assert st["phi_max_err"] <= 1e-5
+    assert st["hit_point_max_err"] <= HITPT_ERR
 
 
 def test_fp32_spheres_720p_matches_oracle(renderer):
     print("[case] fp32 spheres 1280x720", flush=True)
     st = _check_spheres(renderer, scenes.spheres(), "fp32 spheres 1280x720")
-    assert st["class_mismatch_frac"] <= 1e-4
-    assert st["rgb_mismatch_frac"] <= 2e-3
+    assert st["class_mismatch"] <= _allowed(st["pixels"])
+    assert st["rgb_mismatch"] <= _allowed(st["pixels"])
+    assert st["hit_point_max_err"] <= HITPT_ERR
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP32, abi.PREC_FP64])
+def test_sphere_on_the_captured_chord(renderer, precision):
+    # a sphere between the photon sphere and the horizon: captured rays end
+    # with a straight chord from r ~ 3M to B (blackhole_solution_test.cc:
+    # 264-272), and the reference tests every object on it, so the sphere is
+    # hit before the horizon by the rays whose chord passes through it
+    p = scenes.spheres(320, 180, precision=precision)
+    s = abi.Scene.from_buffer_copy(p.scene)
+    s.n_spheres = 1
+    s.spheres[0].center[:] = (-0.9, 0.35, 0.0)
+    s.spheres[0].radius = 0.3
+    p.scene = s
+    print(f"[case] sphere on the captured chord 320x180 precision {precision}", flush=True)
+    g = renderer.render_preset(p, ALL)
+    r = oracle.render_preset(p, THREADS, ALL)
+    st = compare(g, r)
+    _log(f"sphere on the captured chord {precision}", st)
+    # the sphere is met on the chord to the centre (steps = nstep)
+    assert ((r["hit"] == abi.HIT_OBJECT) & (r["steps"] == 20)).sum() > 100
+    assert st["class_mismatch"] <= _allowed(st["pixels"])
+    assert st["rgb_mismatch"] <= _allowed(st["pixels"])
 
 
 def test_fp64_spheres_matches_oracle(renderer):
@@ -229,8 +330,8 @@ def test_tiny_frames_match_oracle(renderer, precision):
         g = renderer.render_preset(p, ALL)
         r = oracle.render_preset(p, 1, ALL)
         assert np.array_equal(g["hit"], r["hit"]), (w, h)
+        assert np.array_equal(g["rgb"], r["rgb"]), (w, h)
         if precision == abi.PREC_FP64:
-            assert np.array_equal(g["rgb"], r["rgb"]), (w, h)
             assert np.array_equal(g["steps"], r["steps"]), (w, h)
         st = compare(g, r)
         assert st["phi_max_err"] <= (1e-9 if precision == abi.PREC_FP64 else 1e-5), (w, h)

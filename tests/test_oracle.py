@@ -516,3 +516,30 @@ def test_escape_chord_turns_toward_the_hole_by_the_deflection():
                                       abs=60 * q ** 3 + 0.05 / math.sqrt(nstep)), (x, y, b)
         checked += 1
     assert checked >= 4
+
+
+def test_captured_chord_meets_a_sphere_before_the_horizon():
+    """A captured ray ends with the straight chord from its last point
+    (r ~ 3M) to B, and every object is tested on it (blackhole_solution_test.cc:
+    264-272): with a sphere straddling that chord, the pixels whose chord
+    passes through it hit the sphere's surface between the horizon (r = 2M)
+    and r = 3M, on the segment toward B, instead of the horizon."""
+    p = scenes.spheres(160, 90, precision=abi.PREC_FP64)
+    s = abi.Scene.from_buffer_copy(p.scene)
+    s.n_spheres = 1
+    c = np.array([-0.9, 0.35, 0.0])
+    s.spheres[0].center[:] = c
+    s.spheres[0].radius = 0.3
+    p.scene = s
+    o = oracle.render_preset(p, 8, ("rgb", "hit", "hitpt", "steps"))
+    on_chord = (o["hit"] == abi.HIT_OBJECT) & (o["steps"] == 20)
+    assert on_chord.sum() > 20
+    q = o["hitpt"][on_chord]
+    np.testing.assert_allclose(np.linalg.norm(q - c, axis=1), 0.3, rtol=1e-12)
+    rq = np.linalg.norm(q, axis=1)
+    assert (rq > 1.0).all() and (rq < 1.5).all()
+    # the same pixels hit the horizon when the sphere is gone
+    s.n_spheres = 0
+    o2 = oracle.render_preset(p, 8, ("hit", "steps"))
+    assert (o2["hit"][on_chord] == abi.HIT_HORIZON).all()
+    assert (o2["steps"][on_chord] == 20).all()

@@ -50,6 +50,8 @@ def color_at(pl, q):
     (10, 10, True),    # boundary: <= on both sides
     (0, 5, False),     # x*y = 0 takes the flipped branch
     (30, 30, True),    # 30 mod 20 = 10 -> boundary, in
+    (3e9 + 5, 5, True),      # beyond the int range: 3000000005 mod 20 = 5
+    (3e9 + 15.5, 5, False),  # truncated, then mod 20: 15 in the second half
 ])
 def test_chess_pattern_known_answers(x, y, white):
     assert color_at(plane(), (x, y, 0.0)) == (255 if white else 0)
