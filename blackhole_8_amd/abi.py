@@ -128,6 +128,8 @@ SIGNATURES = {
     "bh_scene_check": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options)]),
     "bh_render": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options), C.POINTER(Outputs)]),
     "bh_render_device": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options), C.POINTER(Outputs)]),
+    "bh_render_device_noalloc": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options),
+                                           C.POINTER(Outputs)]),
     "bh_path_create": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options), C.POINTER(Camera),
                                  C.c_int, C.POINTER(P)]),
     "bh_path_launch": (C.c_int, [P, C.POINTER(C.c_float)]),

@@ -44,6 +44,7 @@ struct bh_ctx {
   DevBuf dpart, dgrad, drgb, dhit, dgin;  // differentiable render
   DevBuf bscenes, bouts, boff, bctl, bband, bbandc, bend, btile;  // batches
   int wall_khz = 0;
+  void* noalloc = nullptr;  // this device's allocation-free scratch array
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   char name[256] = {0};
   int n_cu = 0, clock_khz = 0;
@@ -699,6 +700,9 @@ int bh_ctx_create(int device, bh_ctx** out) {
     }
     if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
       c->wall_khz = 0;
+    // this device's allocation-free scratch array (device is current); the
+    // entry point reports its absence if the lookup fails
+    if (bh::noalloc_scratch(&c->noalloc) != hipSuccess) c->noalloc = nullptr;
     *out = c;
     return BH_OK;
   });
@@ -838,6 +842,31 @@ int bh_render_device(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
     DevOut o{out_device->rgb, out_device->hit, out_device->phi_end, out_device->hit_point,
              out_device->steps};
     return launch_frame(ctx, d, (DevScene*)ctx->scene.p, o, q, ctx->stream);
+  });
+}
+
+int bh_render_device_noalloc(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                             const bh_outputs* out_device) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!out_device) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
+    if (!ctx->noalloc)
+      return fail(ctx, BH_ERR_HIP, "the allocation-free scratch array is not available on this device");
+    DevScene d;
+    int rc = build_scene(ctx, scene, opt, &d);
+    if (rc) return rc;
+    if (fp32_eligible(d) && bh::queue_items(d.W, d.H) > bh::kNoallocQueueItems)
+      return fail(ctx, BH_ERR_INVALID_ARG,
+                  "frame too large for bh_render_device_noalloc (its FP64 queue holds " +
+                      std::to_string(bh::kNoallocQueueItems) + " items, this frame needs " +
+                      std::to_string(bh::queue_items(d.W, d.H)) + ")");
+    (void)hipSetDevice(ctx->device);
+    const bh::NoallocLayout l = bh::noalloc_layout(ctx->noalloc);
+    HIP_TRY(ctx, bh::launch_noalloc_setup(d, l, ctx->stream));
+    DevOut o{out_device->rgb, out_device->hit, out_device->phi_end, out_device->hit_point,
+             out_device->steps};
+    const DevQueue q{l.items, l.count, bh::queue_sub_cap(d.W, d.H)};
+    return launch_frame(ctx, d, l.scene, o, q, ctx->stream);
   });
 }
 

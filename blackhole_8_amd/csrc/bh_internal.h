@@ -128,6 +128,23 @@ inline uint32_t queue_sub_cap(int W, int H) {
 }
 inline size_t queue_items(int W, int H) { return (size_t)kQueueSubs * queue_sub_cap(W, H); }
 
+// The allocation-free entry point's file-scope device array (bh_noalloc.hip):
+// a scene record, the queue's count words and at most kNoallocQueueItems
+// queue items (24 MiB), which take frames up to queue_items(W, H) of that,
+// e.g. 2560 x 1080.
+constexpr size_t kNoallocQueueItems = size_t(3) << 20;
+struct NoallocLayout {
+  DevScene* scene;
+  uint32_t* count;
+  uint64_t* items;
+};
+// Device address of this device's array (the current device; a host-side
+// stand-in of the array's name is no device address).
+hipError_t noalloc_scratch(void** ptr);
+NoallocLayout noalloc_layout(void* base);
+// Scene record into the array, queue counts cleared; on stream.
+hipError_t launch_noalloc_setup(const DevScene& s, const NoallocLayout& l, hipStream_t stream);
+
 hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, int H,
                              hipStream_t stream);
 hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, int H,

@@ -241,6 +241,19 @@ int bh_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
 int bh_render_device(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                      const bh_outputs* out_device);
 
+/* bh_render_device without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops).  The scene record and the FP64 queue live in a
+ * file-scope device array of the library, 24 MiB on every GPU it runs on,
+ * held until the process ends; the caller passes the output planes and binds
+ * the textures beforehand (bh_texture_set).  Frames whose FP32 march needs a
+ * queue of more than 3 x 2^20 items (larger than about 2560 x 1080) return
+ * BH_ERR_INVALID_ARG.  All calls on one GPU share its array: calls that
+ * overlap on one GPU (other streams, other contexts) race; calls on one
+ * context's stream are ordered. */
+int bh_render_device_noalloc(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                             const bh_outputs* out_device);
+
 /* ======================================================================== */
 /* Camera paths — one frame per camera, replayed as one HIP graph            */
 /* ======================================================================== */
