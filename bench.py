@@ -4,16 +4,21 @@
 Workload (one "step"): the 1920x1080 hole + thin-disc scene (disc 3..12 r_s,
 camera at 20 r_s, inclination 80 deg, fov 60 deg) rendered along its
 120-frame circular camera path (3 deg per frame), replayed as one HIP graph:
-248,832,000 rays per step, FP32 march with FP64 set-up and the FP64
-photon-sphere band.  Inputs are resident in HBM before the timed region;
-outputs stay in HBM.
+248,832,000 rays per step, FP32 march with FP64 set-up, then the FP64 pass
+over the photon-sphere band and the rays whose FP32 texel lies within the
+FP32 error of a texel edge.  Inputs are resident in HBM before the timed
+region; outputs stay in HBM.
 
 Prints one JSON line.  `roofline` covers the dominant kernel (the FP32
 march, every frame of the path in one launch): algorithmic FLOPs of the
 path (blackhole_8_amd/flops.py, from each frame's own per-pixel step counts)
 over that launch's HIP-event duration on the context stream, against the
-FP32 vector peak.  `cpu_baseline` is the FP64
-oracle (a C restatement of the reference march) on this host's cores.
+FP32 vector peak; beside it the FMA throughput a plain (not packed) FP32
+instruction stream reaches on this device, measured here (bh_fma_issue_rate),
+and the march's VALU busy fraction from rocprof counters (profiles/).
+`accuracy` compares frames 0, 59 and 119 of the timed graph's own output
+with the FP64 oracle.  `cpu_baseline` is the FP64 oracle (a C restatement
+of the reference march) on this host's cores.
 """
 from __future__ import annotations
 
@@ -28,8 +33,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3        # MI355X FP32 vector peak (published)
-FP32_NONPACKED_TFLOPS = 78.6    # one FMA per lane per clock
 TRAFFIC_FILE = ROOT / "profiles" / "march32_disc_orbit_traffic.json"
+VALU_FILE = ROOT / "profiles" / "march32_disc_orbit_valu.json"
 
 
 def cpu_model() -> str:
@@ -274,20 +279,26 @@ def main() -> int:
         path.launch()
     r.lib.bh_synchronize(r.ctx)
     t0 = time.perf_counter()
-    gpu_ms = 0.0
+    gpu_ms = march_sum = band_sum = 0.0
     for _ in range(args.steps):
         gpu_ms += path.launch()
+        m_ms, b_ms = path.last_ms()  # the graph's own timing events around its kernels
+        march_sum += m_ms
+        band_sum += b_ms
     r.lib.bh_synchronize(r.ctx)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     value = rays_per_step * args.steps / wall / 1e6
+    # frames of the last timed replay, for the accuracy check below
+    acc_frames = [0, F // 2 - 1, F - 1] if F >= 3 else list(range(F))
+    path_frames = {k: path.read(k) for k in acc_frames}
 
     # ---- dominant kernel: the fused march launch and algorithmic work -------
     # All frames of the path march in one launch (march_frames_kernel); its
-    # duration comes from HIP events around eager launches of that kernel on
-    # the context stream (bh_path_time), the work from each frame's own
-    # per-pixel step counts and hit classes.
-    march_ms, band_ms = path.time(iters=3)
+    # duration is the mean over the timed replays of the HIP events the graph
+    # records on the context stream around that kernel (bh_path_last_ms), the
+    # work comes from each frame's own per-pixel step counts and hit classes.
+    march_ms, band_ms = march_sum / args.steps, band_sum / args.steps
     work, band = 0.0, 0
     alg_bytes = 0  # framebuffer (3 B rgb + 1 B class) and one 4-byte texel per disc hit
     for k in range(F):
@@ -298,12 +309,22 @@ def main() -> int:
     single = [r.time_render(p.with_camera(p.cameras[k]).scene, p.options, iters=4)
               for k in (0, F // 2)]
     achieved = work / (march_ms * 1e-3) / 1e12
+    import ctypes as C
+
+    fma32, fma64 = C.c_double(), C.c_double()
+    r._check(r.lib.bh_fma_issue_rate(r.ctx, C.byref(fma32), C.byref(fma64)))
     traffic = None
     if TRAFFIC_FILE.exists():
         try:
             traffic = json.loads(TRAFFIC_FILE.read_text()).get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
+    valu = None
+    if VALU_FILE.exists():
+        try:
+            valu = json.loads(VALU_FILE.read_text()).get("valu_busy")
+        except (OSError, ValueError):
+            valu = None
     roofline = {
         "bound": "valu",
         "kernel": "march_frames_kernel<false,false> (FP32 event march, all frames in one launch)",
@@ -311,7 +332,10 @@ def main() -> int:
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-        "frac_nonpacked": round(achieved / FP32_NONPACKED_TFLOPS, 4),
+        "fp32_fma_issue_tflops": round(fma32.value, 2),
+        "fp64_fma_issue_tflops": round(fma64.value, 2),
+        "frac_of_fma_issue": round(achieved / fma32.value, 4),
+        "valu_busy": valu,
         "traffic": traffic,
         "traffic_per": "launch (all frames of the path)",
         "traffic_gbps": round(traffic / (march_ms * 1e-3) / 1e9, 1) if traffic else None,
@@ -326,9 +350,10 @@ def main() -> int:
         "band_rays_per_frame": round(sum(n for _, _, n in single) / len(single), 1),
     }
 
-    # ---- accuracy of frame 0 and the CPU baseline ----------------------------
+    # ---- accuracy of the timed graph's frames and the CPU baseline ----------
     cpu = None
     accuracy = None
+
     if not args.no_cpu_baseline:
         import oracle
         from tests.parity import compare
@@ -350,10 +375,22 @@ def main() -> int:
                "sample": f"frames {frames} of the 120-frame 1080p path, FP64 oracle "
                          f"(C restatement of the reference march), {threads} threads, "
                          f"{cpu_s:.2f} s wall, {cpu_model()}"}
+        # frames of the graph's output (the timed launch's planes) against the
+        # oracle; frame 0's angle and hit-point errors from a render of the
+        # same march kernel with every plane
+        accuracy = {"path_frames": {}}
+        for k in acc_frames:
+            rk = ref0 if k == 0 else oracle.render_preset(p.with_camera(p.cameras[k]),
+                                                          threads, ("rgb", "hit"))
+            rgb, hit = path_frames[k]
+            st = compare({"rgb": rgb, "hit": hit}, rk)
+            accuracy["path_frames"][str(k)] = {
+                "class_mismatch_frac": st["class_mismatch_frac"],
+                "rgb_mismatch_frac": st["rgb_mismatch_frac"]}
         g0 = r.render(p.with_camera(p.cameras[0]).scene, p.options, want)
         st = compare(g0, ref0)
-        accuracy = {k: st[k] for k in ("class_mismatch_frac", "rgb_mismatch_frac",
-                                       "phi_max_err", "hit_point_max_err")}
+        accuracy["frame0"] = {k: st[k] for k in ("class_mismatch_frac", "rgb_mismatch_frac",
+                                                 "phi_max_err", "hit_point_max_err")}
     path.close()
     r.close()
 
@@ -377,14 +414,15 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "disc_orbit: hole + thin disc 3..12 r_s, 1920x1080, camera at "
                                "20 r_s, inclination 80 deg, 120-frame orbit path as one HIP "
-                               "graph, FP32 march + FP64 set-up and photon-sphere band",
+                               "graph, FP32 march + FP64 set-up, FP64 pass over the "
+                               "photon-sphere band and texel-edge rays",
                    "width": W, "height": H, "frames": F, "rays_per_step": rays_per_step,
                    "gpu_event_ms_per_step": round(gpu_ms / args.steps, 4),
                    "device": dev["name"], "cu": dev["cu"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_side_by_side": side,
-        "accuracy_frame0": accuracy,
+        "accuracy": accuracy,
         "adjoint": adjoint,
         "batch": batch,
         "flat": flat,

@@ -43,6 +43,7 @@ struct bh_ctx {
   DevBuf scene, rgb, hit, phi, hitpt, steps, qitems, qcount;
   DevBuf dpart, dgrad, drgb, dhit, dgin;  // differentiable render
   DevBuf bscenes, bouts, boff, bctl, bband, bbandc, bend, btile;  // batches
+  DevBuf probe;                                                    // issue-rate probe
   int wall_khz = 0;
   void* noalloc = nullptr;  // this device's allocation-free scratch array
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -62,8 +63,12 @@ struct bh_path {
   bool fused = false;                // all frames FP32: one march + one band launch
   bool esc = false, sph = false;     // scene class of the fused launches
   std::vector<hipEvent_t> events;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // fused paths: the march over all frames and the FP64 pass over all frames
+  // are two graphs, launched back to back with timing events around each
+  // (HIP does not time events recorded inside a graph)
+  hipEvent_t k0 = nullptr, k1 = nullptr, k2 = nullptr;
+  hipGraph_t graph = nullptr, graph_band = nullptr;
+  hipGraphExec_t exec = nullptr, exec_band = nullptr;
 };
 
 namespace {
@@ -717,7 +722,7 @@ void bh_ctx_destroy(bh_ctx* ctx) {
   for (DevBuf* b : {&ctx->scene, &ctx->rgb, &ctx->hit, &ctx->phi, &ctx->hitpt, &ctx->steps,
                     &ctx->qitems, &ctx->qcount, &ctx->dpart, &ctx->dgrad, &ctx->drgb, &ctx->dhit,
                     &ctx->dgin, &ctx->bscenes, &ctx->bouts, &ctx->boff, &ctx->bctl, &ctx->bband,
-                    &ctx->bend, &ctx->btile})
+                    &ctx->bbandc, &ctx->bend, &ctx->btile, &ctx->probe})
     if (b->p) (void)hipFree(b->p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1014,8 +1019,8 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
     int rc = BH_OK;
     bool used_side = false;
     if (p->fused) {
+      // the march graph here; the FP64 pass is captured into its own graph below
       hipError_t le = path_march(p, st);
-      if (le == hipSuccess) le = path_band(p, st);
       if (le != hipSuccess) rc = hip_fail(ctx, le, "path capture launch");
     }
     for (int i = 0; i < n_frames && rc == BH_OK && !p->fused; ++i) {
@@ -1053,6 +1058,21 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
     if ((e = hipStreamEndCapture(st, &p->graph)) != hipSuccess) return cleanup(e, "end capture");
     if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess)
       return cleanup(e, "hipGraphInstantiate");
+    if (p->fused) {
+      if ((e = hipEventCreate(&p->k0)) != hipSuccess || (e = hipEventCreate(&p->k1)) != hipSuccess ||
+          (e = hipEventCreate(&p->k2)) != hipSuccess)
+        return cleanup(e, "hipEventCreate");
+      if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return cleanup(e, "capture");
+      const hipError_t le = path_band(p, st);
+      hipGraph_t g = nullptr;
+      e = hipStreamEndCapture(st, &g);
+      p->graph_band = g;
+      if (le != hipSuccess) return cleanup(le, "path capture launch");
+      if (e != hipSuccess) return cleanup(e, "end capture");
+      if ((e = hipGraphInstantiate(&p->exec_band, p->graph_band, nullptr, nullptr, 0)) != hipSuccess)
+        return cleanup(e, "hipGraphInstantiate");
+    }
     *out = p;
     return BH_OK;
   });
@@ -1064,11 +1084,52 @@ int bh_path_launch(bh_path* p, float* ms) {
     bh_ctx* ctx = p->ctx;
     (void)hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
+    if (p->exec_band) {
+      // the march graph, then the FP64 pass's, with timing events around each
+      HIP_TRY(ctx, hipEventRecord(p->k0, st));
+      HIP_TRY(ctx, hipGraphLaunch(p->exec, st));
+      HIP_TRY(ctx, hipEventRecord(p->k1, st));
+      HIP_TRY(ctx, hipGraphLaunch(p->exec_band, st));
+      HIP_TRY(ctx, hipEventRecord(p->k2, st));
+      HIP_TRY(ctx, hipEventSynchronize(p->k2));
+      if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, p->k0, p->k2));
+      return BH_OK;
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     HIP_TRY(ctx, hipGraphLaunch(p->exec, st));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
     if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return BH_OK;
+  });
+}
+
+int bh_path_last_ms(bh_path* p, double* march_ms, double* band_ms) {
+  return guarded(nullptr, [&]() -> int {
+    if (!p) return fail(nullptr, BH_ERR_BAD_HANDLE, "path is NULL");
+    bh_ctx* ctx = p->ctx;
+    if (!p->exec_band) return fail(ctx, BH_ERR_UNSUPPORTED, "path frames are not all in the FP32 march");
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipEventSynchronize(p->k2));
+    float a = 0.f, b = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&a, p->k0, p->k1));
+    HIP_TRY(ctx, hipEventElapsedTime(&b, p->k1, p->k2));
+    if (march_ms) *march_ms = a;
+    if (band_ms) *band_ms = b;
+    return BH_OK;
+  });
+}
+
+int bh_fma_issue_rate(bh_ctx* ctx, double* fp32_tflops, double* fp64_tflops) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!fp32_tflops || !fp64_tflops) return fail(ctx, BH_ERR_INVALID_ARG, "outputs are NULL");
+    (void)hipSetDevice(ctx->device);
+    int rc = ensure(ctx, ctx->probe, 256 * sizeof(double));
+    if (rc) return rc;
+    const int n_cu = ctx->n_cu > 0 ? ctx->n_cu : 1;
+    HIP_TRY(ctx, bh::fma_probe<float>(n_cu, ctx->stream, (float*)ctx->probe.p, fp32_tflops));
+    HIP_TRY(ctx, bh::fma_probe<double>(n_cu, ctx->stream, (double*)ctx->probe.p, fp64_tflops));
     return BH_OK;
   });
 }
@@ -1123,12 +1184,16 @@ void bh_path_destroy(bh_path* p) {
   }
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->exec_band) (void)hipGraphExecDestroy(p->exec_band);
+  if (p->graph_band) (void)hipGraphDestroy(p->graph_band);
   if (p->d_scenes) (void)hipFree(p->d_scenes);
   if (p->d_rgb) (void)hipFree(p->d_rgb);
   if (p->d_hit) (void)hipFree(p->d_hit);
   if (p->d_qitems) (void)hipFree(p->d_qitems);
   if (p->d_qcount) (void)hipFree(p->d_qcount);
   for (auto ev : p->events)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : {p->k0, p->k1, p->k2})
     if (ev) (void)hipEventDestroy(ev);
   delete p;
 }

@@ -145,6 +145,11 @@ NoallocLayout noalloc_layout(void* base);
 // Scene record into the array, queue counts cleared; on stream.
 hipError_t launch_noalloc_setup(const DevScene& s, const NoallocLayout& l, hipStream_t stream);
 
+// FMA issue-rate probe (bh_probe.hip): TFLOP/s of plain FMA chains in
+// precision T, 8 waves per SIMD on n_cu compute units; sink: 256 elements.
+template <typename T>
+hipError_t fma_probe(int n_cu, hipStream_t stream, T* sink, double* tflops);
+
 hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, int H,
                              hipStream_t stream);
 hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, int H,

@@ -185,6 +185,13 @@ class Path:
         self.r._check(self.r.lib.bh_path_time(self.handle, iters, C.byref(m), C.byref(b)))
         return m.value, b.value
 
+    def last_ms(self):
+        """(march_ms, band_ms) of the last graph launch, from timing events
+        the graph records around its two kernels (bh_path_last_ms)."""
+        m, b = C.c_double(), C.c_double()
+        self.r._check(self.r.lib.bh_path_last_ms(self.handle, C.byref(m), C.byref(b)))
+        return m.value, b.value
+
     def read(self, frame: int):
         rgb = np.zeros((self.h, self.w, 3), np.uint8)
         hit = np.zeros((self.h, self.w), np.uint8)

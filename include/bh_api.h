@@ -255,21 +255,29 @@ int bh_render_device_noalloc(bh_ctx* ctx, const bh_scene* scene, const bh_option
                              const bh_outputs* out_device);
 
 /* ======================================================================== */
-/* Camera paths — one frame per camera, replayed as one HIP graph            */
+/* Camera paths — one frame per camera, replayed as HIP graphs              */
 /* ======================================================================== */
 typedef struct bh_path bh_path;
-/* Capture n_frames renders of scene with cameras[i] into a graph writing
+/* Capture n_frames renders of scene with cameras[i] into HIP graphs writing
  * rgb/hit planes of frame i at offset i*H*W*{3,1} of device buffers owned by
  * the path.  All cameras must share width/height. */
 int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                    const bh_camera* cameras, int n_frames, bh_path** out);
-/* Launch the captured graph once; ms (optional) = GPU time from HIP events.
- * When every frame takes the FP32 march, the graph is one march launch over
- * all frames followed by one FP64 band launch over all frames' queues. */
+/* Launch the captured path once; ms (optional) = GPU time from HIP events.
+ * When every frame takes the FP32 march, the path is two graphs launched back
+ * to back on the context stream: one march launch over all frames, then one
+ * FP64 launch over all frames' queues (band and texel-edge rays); otherwise
+ * one graph of per-frame launches. */
 int bh_path_launch(bh_path* path, float* ms);
 /* Average ms of those two launches over iters eager runs (HIP events on the
  * context stream); BH_ERR_UNSUPPORTED for paths with FP64 frames. */
 int bh_path_time(bh_path* path, int iters, double* march_ms, double* band_ms);
+/* The durations (ms) of the fused march launch and of the FP64 pass in the
+ * path's last bh_path_launch, from HIP events on the context stream around
+ * each (a path whose frames all take the FP32 march is two graphs, the march
+ * over all frames and the FP64 pass over all frames, launched back to back).
+ * Other paths: BH_ERR_UNSUPPORTED. */
+int bh_path_last_ms(bh_path* path, double* march_ms, double* band_ms);
 /* Copy frame i back to host buffers (rgb H*W*3, hit H*W; either may be NULL). */
 int bh_path_read(bh_path* path, int frame, uint8_t* rgb, uint8_t* hit);
 void bh_path_destroy(bh_path* path);
@@ -283,6 +291,14 @@ void bh_path_destroy(bh_path* path);
 int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                    int iters, double* avg_ms_main, double* avg_ms_total,
                    int64_t* band_rays);
+
+/* Measurement helper with no counterpart in the reference: the fused
+ * multiply-add throughput (TFLOP/s, 2 flops per FMA) that plain, not packed,
+ * FP32 and FP64 VALU instruction streams reach on the context's device,
+ * eight independent chains per lane at eight waves per SIMD.  This is the
+ * ceiling the FP32 march is issued against (the library is built without
+ * packed FP32). */
+int bh_fma_issue_rate(bh_ctx* ctx, double* fp32_tflops, double* fp64_tflops);
 
 /* ======================================================================== */
 /* Batches: many independent scenes in one persistent launch (no reference  */

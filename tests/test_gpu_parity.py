@@ -269,6 +269,9 @@ def test_path_graph_matches_eager(renderer, precision):
                 assert np.array_equal(rgb, e["rgb"]), (replay, k)
                 assert np.array_equal(hit, e["hit"]), (replay, k)
         if precision == abi.PREC_FP32:
+            # the graph's own timing events around its two kernels
+            m_ms, b_ms = path.last_ms()
+            assert m_ms > 0 and b_ms > 0
             march_ms, band_ms = path.time(iters=2)
             assert march_ms > 0 and band_ms > 0
             rgb, hit = path.read(3)  # timing launches leave the same frames behind
@@ -277,6 +280,9 @@ def test_path_graph_matches_eager(renderer, precision):
         else:
             with pytest.raises(BHError) as err:
                 path.time(iters=1)
+            assert err.value.code == abi.BH_ERR_UNSUPPORTED
+            with pytest.raises(BHError) as err:
+                path.last_ms()
             assert err.value.code == abi.BH_ERR_UNSUPPORTED
     finally:
         path.close()
