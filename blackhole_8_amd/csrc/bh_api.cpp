@@ -305,6 +305,7 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
     d->edge_b = (float)(E * d->disc_tex.rows / std::sqrt(l2));
   }
   d->edge_bg = (float)g;
+  d->pos_guard = (float)g;
   for (int i = 0; i < d->n_spheres; ++i) {
     double c2 = 0;
     for (int k = 0; k < 3; ++k) c2 += (d->sc[i][k] - d->B[k]) * (d->sc[i][k] - d->B[k]);

@@ -342,15 +342,18 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
   const bool band = r.band && !r.degenerate;
   if (!push_band(b, band, id, 0u, false) && band) return false;
   MarchOut<float> mo;
+  bool sredo = false;
   if (r.degenerate) {
     degenerate_out(mo);
   } else {
     Lane<float> L;
     const double du = (r.l - s.u0) * s.inv_nstep;
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
+    if constexpr (SPH) setup_spheres32(s, r, L);
     march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
+    if constexpr (SPH) sredo = L.sredo;
   }
-  const bool redo = finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo);
+  const bool redo = finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo) || sredo;
   push_band(b, redo, id, queue_outcome(mo.hit.cls, mo.steps), true);
   return true;
 }

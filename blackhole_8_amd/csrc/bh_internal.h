@@ -71,7 +71,9 @@ struct DevScene {
   float edge_bg;                    // background direction (radians)
   float sph_edge[BH_MAX_SPHERES];   // sphere i's normal direction (radians)
   int32_t fast_ok;        // the FP32 march applies to this scene
-  int32_t pad1;
+  // relative FP32 position error bound: a chord passing a sphere's surface
+  // within it (grazing) is marched again in FP64 (0: never)
+  float pos_guard;
 };
 
 // Output planes (device pointers, any may be null).  Pixel (x, y) of the

@@ -35,6 +35,15 @@ namespace bh {
 // stays at distance >= r_min cos(Da/2) from B, so it can only reach the
 // horizon when Da > 2 acos(R_h / r_min) >= pi (1 - R_h / r_min); that step is
 // tested exactly too.  Every other step provably hits nothing.
+//
+// Spheres.  A sphere meets the orbit plane in a circle (or not at all), and a
+// chord spanning a swept angle Da < pi stays inside the wedge of polar angles
+// it spans, while a circle of radius rho whose centre lies at distance D > rho
+// from B stays inside the wedge theta_c +- asin(rho / D).  A step is tested
+// against a sphere's circle only when its wedge meets that window (plus a
+// margin far above the FP32 angle error), once per revolution of the ray.
+constexpr int kSphSlots = 4;
+
 template <typename T>
 struct Lane {
   // G at the steps as a cubic c0 + x (c1 + x (c2 + x c3)) in a uniform x,
@@ -51,10 +60,20 @@ struct Lane {
   float guard;        // disc crossing-angle guard
   double chi0;        // a disc crossing angle (crossings at chi0 + j pi)
   double a_ev;        // next disc event angle (crossing - guard)
-  // scenes with spheres only
-  float e2[3], nrm[3];  // in-plane basis vector e2 and the orbit-plane normal
-  float sgn;            // sign(Y0): e1 = sgn F/|F|
-  uint32_t smask;       // spheres that cut the orbit plane
+  // Scenes with spheres only (setup_spheres): the spheres that cut the orbit
+  // plane, at most kSphSlots, in index order.  Slot k holds sphere
+  // (sids >> 8k) & 0xff as the circle it cuts from the plane (centre
+  // (sx, sy) in the (e1, e2) frame, radius^2 srho2) and the window of swept
+  // angle [slo, slo + sw] of the ray's next pass by that circle; a_sph is the
+  // earliest window start (1e308 once the ray has ended).
+  T sx[kSphSlots], sy[kSphSlots], srho2[kSphSlots];
+  float slo[kSphSlots], sw[kSphSlots];
+  uint32_t sids;
+  int ns;
+  double a_sph;
+  // FP32: a chord passed a sphere's surface within the FP32 position error,
+  // or more than kSphSlots spheres cut the plane (the ray is settled in FP64)
+  bool sredo;
 };
 
 template <typename T>
@@ -134,79 +153,111 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
   return found;
 }
 
-// Spheres (bit i of L.smask: sphere i cuts the orbit plane).  A sphere of
-// radius R whose centre lies at distance z from the plane cuts it in a circle
-// of radius sqrt(R^2 - z^2), and |P - c|^2 = |P - c'|^2 + z^2 for a point P
-// of the plane and the centre's projection c', so the reference's
-// segment/sphere test (blackhole_solution.h:65-88, the same test as the
-// horizon's) of an in-plane chord is exactly this circle test in the plane.
-// The chord from P_a = (ra, 0) to P_b = (bx, by), in the frame rotated by aa,
-// is tested; a hit nearer to the chord start than best (found) replaces hit.
+// sin and cos of a swept angle in precision T (FP64 reduction for both).
+template <typename T> __device__ __forceinline__ void sincos_swept(double a, T* s, T* c);
+template <> __device__ __forceinline__ void sincos_swept<float>(double a, float* s, float* c) {
+  sincos_angle(a, s, c);
+}
+template <> __device__ __forceinline__ void sincos_swept<double>(double a, double* s, double* c) {
+  sincos_small64(a, s, c);
+}
+
+constexpr float kTwoPiF = 6.28318531f;
+
+// Spheres (the slots of L).  A sphere of radius R whose centre lies at
+// distance z from the plane cuts it in a circle of radius sqrt(R^2 - z^2), and
+// |P - c|^2 = |P - c'|^2 + z^2 for a point P of the plane and the centre's
+// projection c', so the reference's segment/sphere test
+// (blackhole_solution.h:65-88, the same test as the horizon's) of an in-plane
+// chord is exactly this circle test in the plane.  The chord from
+// P_a = (ra, 0) to P_b = (bx, by), in the frame rotated by the swept angle aa
+// (sa, ca its sine and cosine), is tested against the slots whose window its
+// wedge meets (every slot when all); a hit nearer to the chord start than
+// best (found) replaces hit.  ab is the swept angle at the chord's end: a
+// window that ends before it moves on to the ray's next revolution.
 template <typename T>
-__device__ inline bool sphere_chord_pts(const DevScene& s, const Lane<T>& L, T ra, double aa,
-                                        T bx, T by, bool found, T best, HitRec<T>& hit) {
+__device__ inline bool sphere_slots(const DevScene& s, Lane<T>& L, T ra, T sa, T ca, double aa,
+                                    T bx, T by, float ab, bool all, bool found, T best,
+                                    HitRec<T>& hit) {
   const T qx = bx - ra, qy = by;
   const T qq = qx * qx + qy * qy;
-  float sa, ca;
-  sincos_angle(aa, &sa, &ca);
-  const float e1x = L.sgn * s.e1f[0], e1y = L.sgn * s.e1f[1], e1z = L.sgn * s.e1f[2];
-  for (int i = 0; i < s.n_spheres; ++i) {
-    if (!((L.smask >> i) & 1u)) continue;
-    const float cx = s.sphf[i][0], cy = s.sphf[i][1], cz = s.sphf[i][2], R = s.sphf[i][3];
-    const float X = cx * e1x + cy * e1y + cz * e1z;
-    const float Y = cx * L.e2[0] + cy * L.e2[1] + cz * L.e2[2];
-    const float Z = cx * L.nrm[0] + cy * L.nrm[1] + cz * L.nrm[2];
-    const T rho2 = (T)(R * R - Z * Z);
-    // centre in the rotated frame, then P_a - c' and P_b - c'
-    const T xc = (T)(X * ca + Y * sa), yc = (T)(Y * ca - X * sa);
-    const T ax = ra - xc, ay = -yc;
-    const T q1 = ax * ax + ay * ay;
-    const T q2 = (bx - xc) * (bx - xc) + (by - yc) * (by - yc);
-    if (q1 < rho2 && q2 < rho2) continue;
-    const T qd = qx * ax + qy * ay;
-    const T cc = qd * qd - qq * (q1 - rho2);
-    if (!(cc > (T)0)) continue;
-    const T t = (-qd - sqrt_t<T>(cc)) * rcp<T>(qq);
-    if (!(t > (T)0 && t < (T)1)) continue;
-    const T d2 = t * t * qq;
-    if (!found || d2 < best) {
-      found = true;
-      best = d2;
-      hit.cls = BH_HIT_OBJECT;
-      hit.obj = i;
-      hit.ang = aa;
-      hit.hx = ra + t * qx;
-      hit.hy = t * qy;
+  const T iqq = rcp<T>(qq);
+  double nxt = 1e308;
+#pragma unroll
+  for (int k = 0; k < kSphSlots; ++k) {
+    if (k >= L.ns) continue;
+    if (all || ab >= L.slo[k]) {
+      const T X = L.sx[k], Y = L.sy[k], rho2 = L.srho2[k];
+      // centre in the rotated frame, then P_a - c' and P_b - c'
+      const T xc = X * ca + Y * sa, yc = Y * ca - X * sa;
+      const T ax = ra - xc, ay = -yc;
+      const T q1 = ax * ax + ay * ay;
+      const T q2 = (bx - xc) * (bx - xc) + (by - yc) * (by - yc);
+      const T qd = qx * ax + qy * ay;
+      const T cc = qd * qd - qq * (q1 - rho2);
+      if constexpr (sizeof(T) == 4) {
+        // grazing: the line's nearest approach to c' lies on the chord and
+        // cc = |Q|^2 (rho^2 - dist^2) is within the FP32 position error
+        // (relative pos_guard of the scale |c'| + rho) of zero
+        const float rho = __builtin_sqrtf(fmaxf(rho2, 0.f));
+        const float dc = __builtin_sqrtf(X * X + Y * Y);
+        const float t0 = -qd * iqq;
+        if (t0 >= 0.f && t0 <= 1.f &&
+            fabsf(cc) < qq * (4.f * s.pos_guard) * rho * (dc + rho))
+          L.sredo = true;
+      }
+      if (!(q1 < rho2 && q2 < rho2) && cc > (T)0) {
+        const T t = (-qd - sqrt_t<T>(cc)) * iqq;
+        if (t > (T)0 && t < (T)1) {
+          const T d2 = t * t * qq;
+          if (!found || d2 < best) {
+            found = true;
+            best = d2;
+            hit.cls = BH_HIT_OBJECT;
+            hit.obj = (int)((L.sids >> (8 * k)) & 0xffu);
+            hit.ang = aa;
+            hit.hx = ra + t * qx;
+            hit.hy = t * qy;
+          }
+        }
+      }
     }
+    const float hi = L.slo[k] + L.sw[k];
+    if (ab > hi) L.slo[k] += kTwoPiF * (__builtin_floorf((ab - hi) * (1.f / kTwoPiF)) + 1.f);
+    nxt = fmin(nxt, (double)L.slo[k]);
   }
+  L.a_sph = nxt;
   return found;
 }
 
-// The step chord from (ka, aa) to (kb, aa + da) against the spheres.
+// The step chord from (ka, aa) to (kb, aa + da) against the spheres; ab and
+// all as for sphere_slots.
 template <typename T>
-__device__ inline bool sphere_chord(const DevScene& s, const Lane<T>& L, T ka, double aa, T kb,
-                                    T da, bool found, T best, HitRec<T>& hit) {
+__device__ inline bool sphere_chord(const DevScene& s, Lane<T>& L, T ka, double aa, T kb, T da,
+                                    float ab, bool all, bool found, T best, HitRec<T>& hit) {
   const T ra = rcp<T>(fma_t<T>(ka, L.du, L.u0));
   const T rb = rcp<T>(fma_t<T>(kb, L.du, L.u0));
-  T sd, cd;
+  T sd, cd, sa, ca;
   sincos_t<T>(da, &sd, &cd);
-  return sphere_chord_pts<T>(s, L, ra, aa, rb * cd, rb * sd, found, best, hit);
+  sincos_swept<T>(aa, &sa, &ca);
+  return sphere_slots<T>(s, L, ra, sa, ca, aa, rb * cd, rb * sd, ab, all, found, best, hit);
 }
 
 // A captured ray's last chord, from its last point (kappa = nstep - 0.1, at
 // the swept angle a) straight to B (blackhole_solution_test.cc:264-272): it
 // enters the horizon at distance r - R_h, unless a sphere is met before.
 template <typename T>
-__device__ inline void captured_chord(const DevScene& s, const Lane<T>& L, double a,
-                                      HitRec<T>& hit) {
+__device__ inline void captured_chord(const DevScene& s, Lane<T>& L, double a, HitRec<T>& hit) {
   hit.cls = BH_HIT_HORIZON;
   hit.ang = a;
   hit.hx = (T)s.Rh;
   hit.hy = (T)0;
-  if (L.smask) {
+  if (L.ns > 0) {
     const T ra = rcp<T>(fma_t<T>((T)s.nstep - (T)0.1, L.du, L.u0));
     const T dh = ra - (T)s.Rh;
-    sphere_chord_pts<T>(s, L, ra, a, (T)0, (T)0, true, dh * dh, hit);
+    T sa, ca;
+    sincos_swept<T>(a, &sa, &ca);
+    sphere_slots<T>(s, L, ra, sa, ca, a, (T)0, (T)0, (float)a, true, true, dh * dh, hit);
   }
 }
 
@@ -297,8 +348,8 @@ template <typename T> struct MarchState {
 // the exact chord tests
 // if the step is an event (a crossing angle passed, the horizon guard
 // exceeded, or force).  Returns true when the chord hits (hit is filled).
-// With SPH the chord is also tested against the spheres of L.smask, at every
-// step.
+// With SPH the chord is also tested against the spheres whose window it
+// meets.
 template <typename T, typename S, bool SPH = false>
 __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>& st,
                                            double first_aa, T G, T ka, T kb, bool force,
@@ -321,7 +372,11 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
     if (!found && L.chi0 != 1e300) L.a_ev = next_event(st.a, L.chi0, L.guard);
   }
   if constexpr (SPH) {
-    if (L.smask) found = sphere_chord<T>(s, L, ka, aa, kb, da, found, best, hit);
+    // every slot on a chord of Da >= 1 (the wedge argument needs Da < pi;
+    // phi_safe < pi on an armed lane, +inf on an ended one)
+    const bool all = force || (float)st.d > fmaxf(1.f, L.phi_safe);
+    if (all || st.a >= L.a_sph)
+      found = sphere_chord<T>(s, L, ka, aa, kb, da, (float)st.a, all, found, best, hit);
   }
   return found;
 }
@@ -357,7 +412,7 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     o.d = st.d;
     L.phi_safe = kInf;
     L.a_ev = 1e308;
-    L.smask = 0u;  // no sphere tests either once ended
+    if constexpr (SPH) L.a_sph = 1e308;  // no sphere tests either once ended
   };
   auto armed = [&]() { return L.phi_safe < kInf; };
   auto step = [&](int n, T G, T ka, T kb, bool force) {
@@ -545,22 +600,76 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, doubl
     }
   }
   if (L.a_ev == 1e300) L.chi0 = 1e300;  // no crossings: parity tests stay even
-  // spheres: orbit-plane basis and the spheres that cut the plane
-  L.sgn = sgn;
-  L.e2[0] = e2x;
-  L.e2[1] = e2y;
-  L.e2[2] = e2z;
-  const float e1x = sgn * s.e1f[0], e1y = sgn * s.e1f[1], e1z = sgn * s.e1f[2];
-  L.nrm[0] = e1y * e2z - e1z * e2y;
-  L.nrm[1] = e1z * e2x - e1x * e2z;
-  L.nrm[2] = e1x * e2y - e1y * e2x;
-  L.smask = 0u;
+}
+
+// The spheres that cut the orbit plane spanned by the unit vectors e1, e2
+// (precision T) into the slots of L, with their first windows: the pass of
+// the circle's wedge that ends at or after swept angle 0.  A circle around B
+// (D <= rho) has a window of all angles.  FP32 keeps a sphere within 1e-5 R
+// of the plane (the FP32 normal's error) and widens each window by 1e-4 of
+// its half-width plus 1e-4 rad (FP32 angles are good to ~1e-6 rad here).
+// Returns false when more than kSphSlots spheres cut the plane: the first
+// kSphSlots are kept and L.sredo is set.
+template <typename T>
+__device__ inline bool setup_spheres(const DevScene& s, const T e1[3], const T e2[3],
+                                     Lane<T>& L) {
+  const T n0 = e1[1] * e2[2] - e1[2] * e2[1];
+  const T n1 = e1[2] * e2[0] - e1[0] * e2[2];
+  const T n2 = e1[0] * e2[1] - e1[1] * e2[0];
+  const T margin = sizeof(T) == 4 ? (T)1.00001f : (T)(1.0 + 1e-9);
+  L.ns = 0;
+  L.sids = 0u;
+  L.a_sph = 1e308;
+  L.sredo = false;
+  bool ok = true;
   for (int i = 0; i < s.n_spheres; ++i) {
-    const float Z = s.sphf[i][0] * L.nrm[0] + s.sphf[i][1] * L.nrm[1] + s.sphf[i][2] * L.nrm[2];
-    // margin for the FP32 normal: a sphere within 1e-5 R of the plane counts
-    const float R = s.sphf[i][3] * 1.00001f;
-    if (Z * Z < R * R) L.smask |= 1u << i;
+    T c0, c1, c2, R;
+    if constexpr (sizeof(T) == 4) {
+      c0 = s.sphf[i][0];
+      c1 = s.sphf[i][1];
+      c2 = s.sphf[i][2];
+      R = s.sphf[i][3];
+    } else {
+      c0 = s.sc[i][0] - s.B[0];
+      c1 = s.sc[i][1] - s.B[1];
+      c2 = s.sc[i][2] - s.B[2];
+      R = s.sr[i];
+    }
+    const T Z = c0 * n0 + c1 * n1 + c2 * n2;
+    const T Rm = R * margin;
+    if (!(Z * Z < Rm * Rm)) continue;
+    if (L.ns == kSphSlots) {
+      ok = false;
+      continue;
+    }
+    const T X = c0 * e1[0] + c1 * e1[1] + c2 * e1[2];
+    const T Y = c0 * e2[0] + c1 * e2[1] + c2 * e2[2];
+    const T rho2 = R * R - Z * Z;
+    const float dc = __builtin_sqrtf((float)(X * X + Y * Y));
+    const float rho = __builtin_sqrtf(fmaxf((float)rho2, 0.f));
+    float lo = -__builtin_inff(), w2 = __builtin_inff();
+    if (dc > rho * 1.0001f + 1e-6f) {
+      const float w = asinf(rho / dc) * 1.0001f + 1e-4f;
+      lo = atan2_fast((float)Y, (float)X) - w;
+      w2 = 2.f * w;
+      if (lo + w2 < 0.f) lo += kTwoPiF;
+    }
+#pragma unroll
+    for (int k = 0; k < kSphSlots; ++k) {
+      if (k == L.ns) {
+        L.sx[k] = X;
+        L.sy[k] = Y;
+        L.srho2[k] = rho2;
+        L.slo[k] = lo;
+        L.sw[k] = w2;
+      }
+    }
+    L.sids |= (uint32_t)i << (8 * L.ns);
+    L.a_sph = fmin(L.a_sph, (double)lo);
+    ++L.ns;
   }
+  if (!ok) L.sredo = true;
+  return ok;
 }
 
 // Per-ray set-up shared by the FP32 march and the FP64 band kernel.
@@ -637,6 +746,13 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
       bisect = true;
   }
   if (bisect) r.l = solve_g64(sqrt(cc / dd), s.M, s.l0, s.root);
+}
+
+// The sphere slots of an FP32 lane, from its FP32 basis (e1 = sgn F/|F|).
+__device__ inline void setup_spheres32(const DevScene& s, const RaySetup& r, Lane<float>& L) {
+  const float e1[3] = {r.sgn * s.e1f[0], r.sgn * s.e1f[1], r.sgn * s.e1f[2]};
+  const float e2[3] = {r.e2x, r.e2y, r.e2z};
+  setup_spheres<float>(s, e1, e2, L);
 }
 
 // Hit point, colour and output planes of one ray.  Returns true when the
@@ -898,20 +1014,25 @@ template <bool ESCAPE, bool SPH>
 __device__ inline void fp64_ray(const DevScene& s, const DevOut& out, size_t idx, int x, int y,
                                 uint32_t outcome) {
   Pixel64 px;
-  if (!SPH) {
-    RaySetup r;
-    setup_ray(s, x, y, false, r);
-    if (!r.degenerate) {
-      double e1[3], e2[3];
-      basis64(s, x, y, r, e1, e2);
-      Lane<double> L;
-      const double du = (r.l - s.u0) * s.inv_nstep;
-      setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
-      disc_events64(s, e1, e2, L);
+  RaySetup r;
+  setup_ray(s, x, y, false, r);
+  if (!r.degenerate) {
+    double e1[3], e2[3];
+    basis64(s, x, y, r, e1, e2);
+    Lane<double> L;
+    const double du = (r.l - s.u0) * s.inv_nstep;
+    setup_lane<double>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
+    disc_events64(s, e1, e2, L);
+    // more spheres in the orbit plane than the slots hold: reference march
+    bool ok = true;
+    if constexpr (SPH) ok = setup_spheres<double>(s, e1, e2, L);
+    if (ok) {
       MarchOut<double> mo;
-      if (!(outcome != 0u && refine64<ESCAPE>(s, L, r, du, (int)(outcome >> 28),
-                                              (int)(outcome & 0x0fffffffu), mo)))
-        march<double, ESCAPE, false>(s, L, r.captured, r.first_aa, du, mo);
+      // spheres: the FP32 outcome is not refined (a sphere may have been
+      // grazed before the step it names)
+      if (SPH || !(outcome != 0u && refine64<ESCAPE>(s, L, r, du, (int)(outcome >> 28),
+                                                     (int)(outcome & 0x0fffffffu), mo)))
+        march<double, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
       if (!finish_ray64<ESCAPE>(s, out, idx, e1, e2, r, mo)) return;
       if (replay64(s, x, y, mo.steps, mo.hit.cls, px)) {
         store_pixel64(out, idx, px);

@@ -60,17 +60,20 @@ __device__ __forceinline__ void march_pixel(const DevScene& s, const DevOut& out
   // queued for the FP64 march (a full queue would march the ray here)
   if (r.band && !r.degenerate && queue_push_band(q, k, (uint32_t)idx)) return;
   MarchOut<float> mo;
+  bool sredo = false;
   if (r.degenerate) {
     degenerate_out(mo);
   } else {
     Lane<float> L;
     const double du = (r.l - s.u0) * s.inv_nstep;
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
+    if constexpr (SPH) setup_spheres32(s, r, L);
     march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
+    if constexpr (SPH) sredo = L.sredo;
   }
-  // the texel lies within the FP32 error of a texel edge: the FP64 pass
-  // refines the ray and overwrites what was just stored
-  if (finish_ray<ESCAPE>(s, out, idx, r, mo))
+  // the texel lies within the FP32 error of a texel edge (or a sphere was
+  // grazed): the FP64 pass settles the ray and overwrites what was just stored
+  if (finish_ray<ESCAPE>(s, out, idx, r, mo) || sredo)
     queue_push_edge(q, k, (uint32_t)idx, queue_outcome(mo.hit.cls, mo.steps));
 }
 

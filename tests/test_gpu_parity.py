@@ -241,6 +241,74 @@ def test_sphere_on_the_captured_chord(renderer, precision):
     assert st["rgb_mismatch"] <= _allowed(st["pixels"])
 
 
+def _sphere_scene(cam, spheres, precision=abi.PREC_FP32):
+    """The spheres preset's hole, background and options with this camera and
+    these (centre, radius) spheres (at most 16, textures of the preset)."""
+    p = scenes.spheres(cam.width, cam.height, precision=precision)
+    s = abi.Scene.from_buffer_copy(p.scene)
+    s.camera = cam
+    s.n_spheres = len(spheres)
+    for i, (c, rad) in enumerate(spheres):
+        s.spheres[i].center[:] = c
+        s.spheres[i].radius = rad
+    p.scene = s
+    return p
+
+
+def _check_fp32_spheres(renderer, p, name, min_objects=100):
+    g = renderer.render_preset(p, ALL)
+    r = oracle.render_preset(p, THREADS, ALL)
+    st = compare(g, r)
+    _log(name, st)
+    assert (r["hit"] == abi.HIT_OBJECT).sum() > min_objects
+    assert st["class_mismatch"] <= _allowed(st["pixels"])
+    assert st["rgb_mismatch"] <= _allowed(st["pixels"])
+    assert st["hit_point_max_err"] <= HITPT_ERR
+    return g, r
+
+
+def test_fp32_spheres_ring_in_the_orbit_planes(renderer):
+    # 12 spheres on a ring around the hole, in the plane of the camera: the
+    # orbit planes of the middle rows cut more spheres than a lane keeps
+    # (those rays are settled by the FP64 pass), and rays bent around the
+    # hole meet the spheres behind it on a later revolution.
+    cam = scenes.look_at(320, 180, math.pi / 3, (-16.0, 0.0, 0.3))
+    ring = [((9.0 * math.cos(math.pi * (2 * k + 1) / 12),
+              9.0 * math.sin(math.pi * (2 * k + 1) / 12), 0.0), 1.0) for k in range(12)]
+    p = _sphere_scene(cam, ring)
+    print("[case] fp32 ring of 12 spheres 320x180", flush=True)
+    g, r = _check_fp32_spheres(renderer, p, "fp32 ring of 12 spheres")
+    # spheres on both sides of the hole are seen
+    assert len(np.unique(r["steps"][r["hit"] == abi.HIT_OBJECT])) > 5
+
+
+def test_fp32_sphere_around_the_hole(renderer):
+    # a sphere that contains the hole (its circle in every orbit plane
+    # surrounds B: every swept angle is in its window), and two outside it
+    cam = scenes.look_at(320, 180, math.pi / 3, (-15.0, 0.0, 2.0))
+    p = _sphere_scene(cam, [((0.6, 0.2, 0.0), 3.5), ((4.0, -6.0, 1.0), 1.2),
+                            ((-2.0, 7.0, -1.0), 0.8)])
+    print("[case] fp32 sphere around the hole 320x180", flush=True)
+    _check_fp32_spheres(renderer, p, "fp32 sphere around the hole", min_objects=5000)
+
+
+def test_fp32_spheres_camera_turned_away(renderer):
+    # rays with Y0 < 0, whose first chord runs from the camera (angle pi in
+    # their frame, forced tests) past the hole, meeting spheres near the hole
+    # on it; spheres in front of the camera for the other rays
+    cam = scenes.look_at(320, 180, math.pi / 3, (-8.0, 0.0, 1.5),
+                         target=(-8.0 + 20 * math.cos(math.radians(100)),
+                                 20 * math.sin(math.radians(100)), 1.5))
+    near = [((2.2 * math.cos(math.radians(a)), 2.2 * math.sin(math.radians(a)), z), 0.45)
+            for a, z in ((0, 0.1), (60, 0.3), (120, 0.5), (180, 0.6), (240, 0.0), (300, -0.3))]
+    ahead = [((-8.0 + 6 * math.cos(math.radians(a)), 6 * math.sin(math.radians(a)), z), 0.9)
+             for a, z in ((80, 1.0), (100, 2.5), (120, 0.0), (95, -1.0))]
+    p = _sphere_scene(cam, near + ahead)
+    print("[case] fp32 spheres, camera turned away 320x180", flush=True)
+    g, r = _check_fp32_spheres(renderer, p, "fp32 spheres camera turned away")
+    assert ((r["hit"] == abi.HIT_OBJECT) & (r["steps"] == 1)).sum() > 1000
+
+
 def test_fp64_spheres_matches_oracle(renderer):
     print("[case] fp64 spheres 320x180", flush=True)
     st = _check_spheres(renderer, scenes.spheres(320, 180, precision=abi.PREC_FP64),
