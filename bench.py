@@ -17,7 +17,9 @@ FP32 vector peak; beside it the FMA throughput a plain (not packed) FP32
 instruction stream reaches on this device, measured here (bh_fma_issue_rate),
 and the march's VALU busy fraction from rocprof counters (profiles/).
 `accuracy` compares frames 0, 59 and 119 of the timed graph's own output
-with the FP64 oracle.  `cpu_baseline` is the FP64 oracle (a C restatement
+with the FP64 oracle.  `spheres` gives the same roofline for the 720p
+16-sphere scene's march kernel (march_kernel<true,true>) and its whole-frame
+time.  `cpu_baseline` is the FP64 oracle (a C restatement
 of the reference march) on this host's cores.
 """
 from __future__ import annotations
@@ -154,6 +156,41 @@ def cpu_side_by_side(iters: int = 20) -> dict:
     return out
 
 
+def spheres_timing(iters: int = 20) -> dict:
+    """The 16-sphere scene (1280x720, FP32, escape to r = 100 with a
+    background): the FP32 march kernel alone (march_kernel<true,true>, iters
+    back-to-back launches between HIP events on the context stream) against
+    the scene's algorithmic work, and the whole frame with its FP64 pass."""
+    from blackhole_8_amd import Renderer, flops, scenes
+
+    p = scenes.spheres()
+    r = Renderer(0)
+    try:
+        r.set_textures(p.textures)
+        out = r.render(p.scene, p.options, ("hit", "steps"))
+        main_ms, total_ms, queued = r.time_render(p.scene, p.options, iters)
+    finally:
+        r.close()
+    s = p.scene
+    in_plane = flops.spheres_in_plane(s)
+    work = flops.frame_flops(out["steps"], out["hit"], bool(s.has_disc), in_plane,
+                             s.n_spheres, background=s.background >= 0)
+    achieved = work / (main_ms * 1e-3) / 1e12
+    rays = s.camera.width * s.camera.height
+    return {"workload": "spheres: hole + 16 textured spheres (radius 0.3..1.5, 5..30 r_s), "
+                        "1280x720, FP32, escape at 100 r_s to a background",
+            "rays": rays, "steps_per_ray": round(float(out["steps"].mean()), 2),
+            "spheres_in_plane_per_ray": round(float(in_plane.mean()), 3),
+            "algorithmic_gflop": round(work / 1e9, 4),
+            "roofline": {"bound": "valu",
+                         "kernel": "march_kernel<true,true> (FP32 event march, spheres)",
+                         "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                         "launch_ms": round(main_ms, 5)},
+            "frame_ms": round(total_ms, 5), "fp64_pass_rays": int(queued),
+            "mrays_per_s": round(rays / total_ms / 1e3, 1), "iters": iters}
+
+
 def adjoint_timing(iters: int = 50) -> dict:
     """Forward+adjoint against the forward on the 512x512 inverse-fit scene
     (FP32): the differentiable forward, the fused forward+adjoint launch, and
@@ -222,6 +259,9 @@ def main() -> int:
     ap.add_argument("--batch-only", action="store_true",
                     help="only time the 512-scene batch and exit")
     ap.add_argument("--no-flat", action="store_true")
+    ap.add_argument("--no-spheres", action="store_true")
+    ap.add_argument("--spheres-only", action="store_true",
+                    help="only time the 16-sphere 720p scene and exit")
     ap.add_argument("--flat-only", action="store_true",
                     help="only time the flat-space (ray_tracer_test) scene and exit")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -249,6 +289,9 @@ def main() -> int:
         return 0
     if args.flat_only:
         print(json.dumps({"flat": flat_timing()}))
+        return 0
+    if args.spheres_only:
+        print(json.dumps({"spheres": spheres_timing()}))
         return 0
     if args.side_only:
         print(json.dumps({"cpu_side_by_side": cpu_side_by_side()}))
@@ -398,6 +441,7 @@ def main() -> int:
     adjoint = None if args.no_adjoint else adjoint_timing()
     batch = None if args.no_batch else batch_timing()
     flat = None if args.no_flat else flat_timing()
+    spheres = None if args.no_spheres else spheres_timing()
 
     line = {
         "metric": "Mrays/s at 1080p BH+disc; fwd+adjoint ms/frame vs plain fwd; % FP32 peak",
@@ -426,6 +470,7 @@ def main() -> int:
         "adjoint": adjoint,
         "batch": batch,
         "flat": flat,
+        "spheres": spheres,
     }
     print(json.dumps(line))
     return 0
