@@ -139,8 +139,9 @@ struct RayRun {
 __device__ inline bool finish(const BatchDev& b, const RayRun& R) {
   const DevScene& s = b.scenes[R.sc];
   const DevOut out = b.outs[R.sc];
-  if (s.escape) return finish_ray<true>(s, out, R.idx, R.r, R.o);
-  return finish_ray<false>(s, out, R.idx, R.r, R.o);
+  const float phi0 = phi0_of(out, R.r);
+  if (s.escape) return finish_ray<true>(s, out, R.idx, R.r, R.o, phi0);
+  return finish_ray<false>(s, out, R.idx, R.r, R.o, phi0);
 }
 
 // Advance one ray by one step of the reference rule (the per-frame march's
@@ -341,6 +342,7 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
   setup_ray(s, x, y, false, r);
   const bool band = r.band && !r.degenerate;
   if (!push_band(b, band, id, 0u, false) && band) return false;
+  const float phi0 = phi0_of(b.outs[sc], r);
   MarchOut<float> mo;
   bool sredo = false;
   if (r.degenerate) {
@@ -353,7 +355,7 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
     march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
     if constexpr (SPH) sredo = L.sredo;
   }
-  const bool redo = finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo) || sredo;
+  const bool redo = finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo, phi0) || sredo;
   push_band(b, redo, id, queue_outcome(mo.hit.cls, mo.steps), true);
   return true;
 }

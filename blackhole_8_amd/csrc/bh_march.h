@@ -5,6 +5,8 @@
 // inputs, so a ray's result does not depend on which kernel marched it.
 #pragma once
 
+#include <type_traits>
+
 #include "bh_math.h"
 
 namespace bh {
@@ -33,8 +35,10 @@ namespace bh {
 // chi_0 + j pi; the step is tested exactly when a passes the next one (minus
 // a guard for the FP32 angle).  A chord between radii >= r_min subtending Da
 // stays at distance >= r_min cos(Da/2) from B, so it can only reach the
-// horizon when Da > 2 acos(R_h / r_min) >= pi (1 - R_h / r_min); that step is
-// tested exactly too.  Every other step provably hits nothing.
+// horizon when Da > 2 acos(R_h / r_min) >= 2 sqrt(2t) (1 + t/12) with
+// t = 1 - R_h / r_min (acos(1 - t) = 2 asin(sqrt(t/2)), whose series in t has
+// positive terms); that step is tested exactly too.  Every other step
+// provably hits nothing.
 //
 // Spheres.  A sphere meets the orbit plane in a circle (or not at all), and a
 // chord spanning a swept angle Da < pi stays inside the wedge of polar angles
@@ -58,7 +62,10 @@ struct Lane {
   T u0;               // 1/|C - B|
   float phi_safe;     // horizon guard angle
   float guard;        // disc crossing-angle guard
-  double chi0;        // a disc crossing angle (crossings at chi0 + j pi)
+  // a disc crossing angle, chi0 (crossings at chi0 + j pi): FP64 lanes hold
+  // it, FP32 lanes the FP32 atan2 it is formed from (chi0 = that + pi/2 in
+  // FP64, chi0_of), one VGPR fewer live through the march
+  std::conditional_t<sizeof(T) == 4, float, double> chi0s;
   double a_ev;        // next disc event angle (crossing - guard)
   // Scenes with spheres only (setup_spheres): the spheres that cut the orbit
   // plane, at most kSphSlots, in index order.  Slot k holds sphere
@@ -75,6 +82,13 @@ struct Lane {
   // or more than kSphSlots spheres cut the plane (the ray is settled in FP64)
   bool sredo;
 };
+
+template <typename T> __device__ __forceinline__ double chi0_of(const Lane<T>& L) {
+  if constexpr (sizeof(T) == 4)
+    return (double)L.chi0s + kPiD / 2.0;
+  else
+    return L.chi0s;
+}
 
 template <typename T>
 struct HitRec {
@@ -123,10 +137,11 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
     // Annulus::Collide (vector_object.h:328-347): n.(P - B) ~ r sin(a - chi)
     // changes sign an odd number of times; crossing by linear interpolation
     // on the chord with weights |t_b|, |t_a|
-    const double fa = floor((aa - L.chi0) * kInvPiD);
-    const double fb = floor((ab - L.chi0) * kInvPiD);
+    const double chi0 = chi0_of(L);
+    const double fa = floor((aa - chi0) * kInvPiD);
+    const double fb = floor((ab - chi0) * kInvPiD);
     if (((int)fb - (int)fa) & 1) {
-      const double chik = L.chi0 + fmax(fa, fb) * kPiD;
+      const double chik = chi0 + fmax(fa, fb) * kPiD;
       T sa, ca, sb, cb;
       sincos_t<T>((T)(aa - chik), &sa, &ca);
       sincos_t<T>((T)(ab - chik), &sb, &cb);
@@ -344,16 +359,9 @@ template <typename T> struct MarchState {
   T d;       // last increment
 };
 
-// One step (G = G(u) at the step): trapezoid update of the swept angle, then
-// the exact chord tests
-// if the step is an event (a crossing angle passed, the horizon guard
-// exceeded, or force).  Returns true when the chord hits (hit is filled).
-// With SPH the chord is also tested against the spheres whose window it
-// meets.
-template <typename T, typename S, bool SPH = false>
-__device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>& st,
-                                           double first_aa, T G, T ka, T kb, bool force,
-                                           HitRec<T>& hit) {
+// The trapezoid update of the swept angle by one step (G = G(u) at the step).
+template <typename T>
+__device__ __forceinline__ void advance(const Lane<T>& L, MarchState<T>& st, T G) {
   const T wn = rsq<T>(G);  // FP32: w h (setup_lane scales the cubic)
   if constexpr (sizeof(T) == 4)
     st.d = st.w + wn;
@@ -361,6 +369,16 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
     st.d = (st.w + wn) * L.h;
   st.w = wn;
   st.a += (double)st.d;
+}
+
+// The exact chord tests of the step just advanced (st.a, st.d), if it is an
+// event (a crossing angle passed, the horizon guard exceeded, or force).
+// Returns true when the chord hits (hit is filled).  With SPH the chord is
+// also tested against the spheres whose window it meets.
+template <typename T, typename S, bool SPH = false>
+__device__ __forceinline__ bool step_tests(const S& s, Lane<T>& L, const MarchState<T>& st,
+                                           double first_aa, T ka, T kb, bool force,
+                                           HitRec<T>& hit) {
   const bool eh = (float)st.d > L.phi_safe;
   const double aa = force ? first_aa : st.a - (double)st.d;
   const T da = force ? (T)(st.a - first_aa) : st.d;
@@ -369,7 +387,7 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
   if (eh || st.a >= L.a_ev || force) {
     found = event_chord<T, S>(s, L, ka, aa, kb, st.a, da, eh || force, s.has_disc != 0, hit,
                               &best);
-    if (!found && L.chi0 != 1e300) L.a_ev = next_event(st.a, L.chi0, L.guard);
+    if (!found && st.a >= L.a_ev) L.a_ev = next_event(st.a, chi0_of(L), L.guard);
   }
   if constexpr (SPH) {
     // every slot on a chord of Da >= 1 (the wedge argument needs Da < pi;
@@ -381,6 +399,15 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
   return found;
 }
 
+// One step: advance, then its tests.
+template <typename T, typename S, bool SPH = false>
+__device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>& st,
+                                           double first_aa, T G, T ka, T kb, bool force,
+                                           HitRec<T>& hit) {
+  advance<T>(L, st, G);
+  return step_tests<T, S, SPH>(s, L, st, first_aa, ka, kb, force, hit);
+}
+
 // The march of one ray in precision T.  first_aa is the angle of the camera
 // in the (e1, e2) frame: 0 when Y0 > 0, pi when Y0 < 0 (the reference's
 // phi0 = atan(b/Y0) then points away from the camera, and only the first
@@ -388,10 +415,11 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
 //
 // Every lane of a wave is at the same step, so the loops run wave-uniform
 // trip counts and keep no per-lane flag: a ray that has ended is disarmed by
-// setting its event thresholds to +inf and clearing its sphere mask (its
-// results were recorded when it ended; it keeps stepping harmlessly), and
-// every four steps the wave leaves once no lane is armed.  This keeps the exec-mask bookkeeping of the hot
-// loop to a handful of scalar instructions.
+// setting its event thresholds (and sphere windows) to +inf (its results
+// were recorded when it ended; it keeps stepping harmlessly), and every four
+// steps the wave leaves once no lane is armed.  The four steps take one
+// event check between them (block), which keeps the exec-mask bookkeeping of
+// the hot loop to a handful of scalar instructions per four steps.
 template <typename T, bool ESCAPE, bool SPH = false>
 __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, double first_aa,
                              double du64, MarchOut<T>& o) {
@@ -428,12 +456,55 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     coeffs_out<T>(n, nstep, L, &G, &ka, &kb);
     step(n, G, ka, kb, false);
   };
+  // Four steps n0..n0+3 (coeffs: coeffs_in or coeffs_out): the increments
+  // first, then one check of the block.  The swept angle only grows, so a
+  // crossing threshold or sphere window is passed in the block iff the
+  // block's last angle passes it, and the horizon guard iff the largest
+  // increment exceeds it; only a lane whose block holds such an event
+  // replays the four steps' tests in order, on the same angles (the same
+  // FP64 additions), so every step is tested exactly as one at a time.
+  // The increments grow along the inbound steps and shrink along the
+  // outbound ones (G falls as u rises below 1/(3M)), so the largest of a
+  // block is its last inbound or its first outbound one (the 1e-4 margin of
+  // phi_safe covers rounding).
+  auto block = [&](int n0, auto coeffs, bool inbound) {
+    const double a0 = st.a;
+    T dk[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      T G, ka, kb;
+      coeffs(n0 + k, &G, &ka, &kb);
+      advance<T>(L, st, G);
+      dk[k] = st.d;
+    }
+    const float dmax = (float)(inbound ? dk[3] : dk[0]);
+    bool ev = dmax > L.phi_safe || st.a >= L.a_ev;
+    if constexpr (SPH) ev = ev || dmax > fmaxf(1.f, L.phi_safe) || st.a >= L.a_sph;
+    if (ev) {
+      // the replay recomputes the four sums from a0 rather than keeping them
+      // (8 more live VGPRs in the fast path would cost occupancy): an empty
+      // asm statement hides from the compiler that the copy equals a0
+      double ar = a0;
+      __asm__ volatile("" : "+v"(ar));
+      st.a = ar;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        T G, ka, kb;
+        coeffs(n0 + k, &G, &ka, &kb);
+        st.d = dk[k];
+        st.a += (double)dk[k];
+        if (step_tests<T, DevScene, SPH>(s, L, st, first_aa, ka, kb, false, o.hit))
+          end_ray(n0 + k);
+      }
+    }
+  };
+  auto in_coeffs = [&](int n, T* G, T* ka, T* kb) { coeffs_in<T>(n, nstep, L, G, ka, kb); };
+  auto out_coeffs = [&](int n, T* G, T* ka, T* kb) { coeffs_out<T>(n, nstep, L, G, ka, kb); };
   // inbound (blackhole_solution_test.cc:217-239)
   step_in(1, first_aa != 0.0);
   int n = 2;
   for (; n + 3 < nstep; n += 4) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) step_in(n + k, false);
+    block(n, in_coeffs, true);
     if (!__any(armed())) goto done;
   }
   for (; n < nstep; ++n) step_in(n, false);
@@ -462,8 +533,7 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     // outbound: u -= du (:274-296)
     n = nstep + 1;
     for (; n + 3 < 2 * nstep; n += 4) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) step_out(n + k);
+      block(n, out_coeffs, false);
       if (!__any(armed())) goto done;
     }
     for (; n < 2 * nstep; ++n) step_out(n);
@@ -577,11 +647,15 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, doubl
     L.u0 = s.u0f;  // (float)u0, set with the scene
   else
     L.u0 = (T)s.u0;
-  // smallest radius on the march: u = u0 + (nstep - 0.1) du (FP32: the guard
-  // keeps a 1e-4 relative margin)
+  // smallest radius on the march: u = u0 + (nstep - 0.1) du; the guard is
+  // the lower bound of 2 acos(x) above, x = R_h / r_min (FP32: a 1e-4
+  // relative margin; at most 2 sqrt(2) 13/12 < pi)
   const float x = s.Rhf * __builtin_fmaf(s.ktf, (float)du, s.u0f);
-  L.phi_safe = x < 1.f ? 3.14159265f * (1.f - x) * 0.9999f : 0.f;
-  L.chi0 = 0.0;
+  const float t = 1.f - x;
+  L.phi_safe = x < 1.f ? 2.f * __builtin_amdgcn_sqrtf(2.f * t) *
+                             __builtin_fmaf(t, 1.f / 12.f, 1.f) * 0.9999f
+                       : 0.f;
+  L.chi0s = 0;
   L.a_ev = 1e300;
   L.guard = 0.f;
   if (s.has_disc) {
@@ -592,14 +666,18 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, doubl
       // al cos a + be sin a vanishes at a = atan2(be, al) + pi/2 (mod pi)
       const float at = atan2_fast(be, al);
       const float chi = at + 1.57079633f;
-      L.chi0 = (double)at + kPiD / 2.0;
+      if constexpr (sizeof(T) == 4)
+        L.chi0s = at;
+      else
+        L.chi0s = (double)at + kPiD / 2.0;
       L.guard = 4e-6f + 4e-7f * __builtin_amdgcn_rsqf(amp2);
       // first crossing threshold at or after 0 (FP32: far inside the guard)
       L.a_ev = (double)(chi + (__builtin_floorf((-L.guard - chi) * 0.318309886f) + 1.f) *
                                   3.14159265f - L.guard);
     }
   }
-  if (L.a_ev == 1e300) L.chi0 = 1e300;  // no crossings: parity tests stay even
+  // no crossings: parity tests stay even
+  if (L.a_ev == 1e300) L.chi0s = sizeof(T) == 4 ? (T)1e30f : (T)1e300;
 }
 
 // The spheres that cut the orbit plane spanned by the unit vectors e1, e2
@@ -755,12 +833,19 @@ __device__ inline void setup_spheres32(const DevScene& s, const RaySetup& r, Lan
   setup_spheres<float>(s, e1, e2, L);
 }
 
+// The reference's phi0 = atan(b / Y0) of the ray in FP32, for the optional
+// phi plane (the FP32 march's angles are good to ~1e-6 rad), formed before
+// the march so that cc and dF need not stay live through it.
+__device__ __forceinline__ float phi0_of(const DevOut& out, const RaySetup& r) {
+  return (out.phi && !r.degenerate) ? (float)atan(sqrt(r.cc) / r.dF) : 0.f;
+}
+
 // Hit point, colour and output planes of one ray.  Returns true when the
 // ray's texel coordinate lies within the scene's edge guard of a texel edge:
 // its texel is then settled by the FP64 pass.
 template <bool ESCAPE>
 __device__ inline bool finish_ray(const DevScene& s, const DevOut& out, size_t idx,
-                                  const RaySetup& r, const MarchOut<float>& mo) {
+                                  const RaySetup& r, const MarchOut<float>& mo, float phi0) {
   const float e1x = r.sgn * s.e1f[0], e1y = r.sgn * s.e1f[1], e1z = r.sgn * s.e1f[2];
   uint32_t rgb = 0;
   bool redo = false;
@@ -811,7 +896,7 @@ __device__ inline bool finish_ray(const DevScene& s, const DevOut& out, size_t i
     out.hitpt[3 * idx + 1] = (double)qy;
     out.hitpt[3 * idx + 2] = (double)qz;
   }
-  if (out.phi) out.phi[idx] = r.degenerate ? mo.a : atan(sqrt(r.cc) / r.dF) + mo.a;
+  if (out.phi) out.phi[idx] = r.degenerate ? mo.a : (double)phi0 + mo.a;
   return redo;
 }
 
@@ -877,14 +962,14 @@ __device__ inline void basis64(const DevScene& s, int x, int y, const RaySetup& 
 // a = atan2(be, al) + pi/2 (mod pi).
 __device__ inline void disc_events64(const DevScene& s, const double e1[3], const double e2[3],
                                      Lane<double>& L) {
-  L.chi0 = 1e300;
+  L.chi0s = 1e300;
   L.a_ev = 1e300;
   if (!s.has_disc) return;
   const double al = s.dn[0] * e1[0] + s.dn[1] * e1[1] + s.dn[2] * e1[2];
   const double be = s.dn[0] * e2[0] + s.dn[1] * e2[1] + s.dn[2] * e2[2];
   if (!(al * al + be * be > 0.0)) return;
-  L.chi0 = atan2(be, al) + kPiD / 2.0;
-  L.a_ev = next_event(0.0, L.chi0, L.guard);
+  L.chi0s = atan2(be, al) + kPiD / 2.0;
+  L.a_ev = next_event(0.0, L.chi0s, L.guard);
 }
 
 // Hit point, colour and output planes of pixel idx marched in FP64, with the

@@ -59,6 +59,7 @@ __device__ __forceinline__ void march_pixel(const DevScene& s, const DevOut& out
   setup_ray(s, x, y, false, r);
   // queued for the FP64 march (a full queue would march the ray here)
   if (r.band && !r.degenerate && queue_push_band(q, k, (uint32_t)idx)) return;
+  const float phi0 = phi0_of(out, r);
   MarchOut<float> mo;
   bool sredo = false;
   if (r.degenerate) {
@@ -73,13 +74,14 @@ __device__ __forceinline__ void march_pixel(const DevScene& s, const DevOut& out
   }
   // the texel lies within the FP32 error of a texel edge (or a sphere was
   // grazed): the FP64 pass settles the ray and overwrites what was just stored
-  if (finish_ray<ESCAPE>(s, out, idx, r, mo) || sredo)
+  if (finish_ray<ESCAPE>(s, out, idx, r, mo, phi0) || sredo)
     queue_push_edge(q, k, (uint32_t)idx, queue_outcome(mo.hit.cls, mo.steps));
 }
 
+// Without spheres the march fits 64 VGPRs: 8 waves per SIMD.
 template <bool ESCAPE, bool SPH>
-__global__ __launch_bounds__(256) void march_kernel(const DevScene* __restrict__ sp, DevOut out,
-                                                    DevQueue q) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 1 : 8)))
+void march_kernel(const DevScene* __restrict__ sp, DevOut out, DevQueue q) {
   const DevScene& s = *sp;
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -104,9 +106,9 @@ __device__ __forceinline__ void frame_slice(const DevOut& out, const DevQueue& q
 // Every frame of a camera path in one launch (blockIdx.z = frame - frame0):
 // consecutive frames share no ramp-up or drain of the machine.
 template <bool ESCAPE, bool SPH>
-__global__ __launch_bounds__(256) void march_frames_kernel(const DevScene* __restrict__ sp,
-                                                           DevOut out, DevQueue q, size_t npix,
-                                                           int frame0) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 1 : 8)))
+void march_frames_kernel(const DevScene* __restrict__ sp, DevOut out, DevQueue q, size_t npix,
+                         int frame0) {
   const int f = frame0 + (int)blockIdx.z;
   const DevScene& s = sp[f];
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
