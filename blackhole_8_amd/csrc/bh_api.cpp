@@ -43,7 +43,7 @@ struct bh_ctx {
   DevBuf scene, rgb, hit, phi, hitpt, steps, qitems, qcount;
   DevBuf dpart, dgrad, drgb, dhit, dgin;  // differentiable render
   DevBuf bscenes, bouts, boff, bctl, bband, bbandc, bend, btile;  // batches
-  DevBuf probe;                                                    // issue-rate probe
+  DevBuf probe;   // issue-rate probe; turning points of bh_turning_points
   int wall_khz = 0;
   void* noalloc = nullptr;  // this device's allocation-free scratch array
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -1131,6 +1131,29 @@ int bh_fma_issue_rate(bh_ctx* ctx, double* fp32_tflops, double* fp64_tflops) {
     const int n_cu = ctx->n_cu > 0 ? ctx->n_cu : 1;
     HIP_TRY(ctx, bh::fma_probe<float>(n_cu, ctx->stream, (float*)ctx->probe.p, fp32_tflops));
     HIP_TRY(ctx, bh::fma_probe<double>(n_cu, ctx->stream, (double*)ctx->probe.p, fp64_tflops));
+    return BH_OK;
+  });
+}
+
+int bh_turning_points(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                      double* l_host) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!l_host) return fail(ctx, BH_ERR_INVALID_ARG, "l_host is NULL");
+    DevScene d;
+    int rc = build_scene(ctx, scene, opt, &d);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)d.W * d.H;
+    if ((rc = ensure(ctx, ctx->scene, sizeof(DevScene)))) return rc;
+    if ((rc = ensure(ctx, ctx->probe, npix * sizeof(double)))) return rc;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->scene.p, &d, sizeof d, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, bh::launch_turning((const DevScene*)ctx->scene.p, (double*)ctx->probe.p, d.W,
+                                    d.H, st));
+    HIP_TRY(ctx, hipMemcpyAsync(l_host, ctx->probe.p, npix * sizeof(double),
+                                hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
     return BH_OK;
   });
 }

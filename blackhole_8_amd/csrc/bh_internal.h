@@ -167,6 +167,7 @@ hipError_t launch_band_frames(const DevScene* scenes, const DevOut& out, int W, 
                               int frame0, int nf, const DevQueue& q, hipStream_t stream,
                               bool escape, bool spheres);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
+hipError_t launch_turning(const DevScene* scene, double* l, int W, int H, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // Batch of independent FP32 scenes marched by one persistent launch.  Ray ids

@@ -795,17 +795,30 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
   r.e2y = (float)(__builtin_fma(dF, Fy, -s.r02 * dvy) * k);
   r.e2z = (float)(__builtin_fma(dF, Fz, -s.r02 * dvz) * k);
   if (r.captured) return;
-  // Turning point: StaticBlackhole::SolveG's lower bisection end.  The root
-  // u* = (1/(3M)) (1/2 + sin(asin(2q - 1)/3)), q = (b_c/b)^2, is refined by
-  // two Newton steps in FP64; the bisection cell containing it is then exact
-  // unless u* sits within 1e-5 cells of a cell edge, where (as in the
-  // photon-sphere band) the reference's bisection is rerun.
+  // Turning point: StaticBlackhole::SolveG's lower bisection end.  With
+  // y = 3 M u and q = (b_c/b)^2, G(u) = 0 reads 3y^2 - 2y^3 = q, a cubic
+  // symmetric under y -> 1 - y, q -> 1 - q.  Its root y* in [0, 1) starts
+  // from s P(s), s = sqrt(q), for q <= 1/2 and from 1 - s P(s),
+  // s = sqrt(1 - q), above, where P is a least-squares degree-5 fit to y/s on
+  // [0, sqrt(1/2)] (relative error 3e-5): the start is good to 3e-5 of
+  // min(y*, 1 - y*), which cancels the Newton constant's growth as y* -> 1,
+  // and two Newton steps in FP64 bring u* within 1e-15 of the root outside
+  // the photon-sphere band.  The bisection cell containing it is then exact
+  // unless u* sits within 1e-5 cells of a cell edge, where (as in the band)
+  // the reference's bisection is rerun.
   bool bisect = force_bisect || r.band || s.root == BH_ROOT_EXACT;
   if (!bisect) {
-    const float q = fminf(1.f, (float)(s.bc2 * r.ib2));
-    float sn, cs;
-    sincos_core(asinf(2.f * q - 1.f) * (1.f / 3.f), &sn, &cs);
-    double u = s.Uc * (0.5 + (double)sn);
+    const double q = s.bc2 * r.ib2;
+    const float qf = (float)q, qc = (float)(1.0 - q);
+    const float sq = __builtin_amdgcn_sqrtf(fmaxf(fminf(qf, qc), 0.f));
+    float p = 0.18513675f;
+    p = __builtin_fmaf(p, sq, -0.17374028f);
+    p = __builtin_fmaf(p, sq, 0.12028659f);
+    p = __builtin_fmaf(p, sq, 0.037219696f);
+    p = __builtin_fmaf(p, sq, 0.11216883f);
+    p = __builtin_fmaf(p, sq, 0.57733941f);
+    const float z = sq * p;
+    double u = s.Uc * (double)(qf <= qc ? z : 1.f - z);
     const double M2 = s.M2, M6 = s.M6;
     // (the FP32 1/G' leaves the quadratic convergence a 1e-7 linear term)
 #pragma unroll

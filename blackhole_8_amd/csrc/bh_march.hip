@@ -243,6 +243,25 @@ hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, in
   return hipGetLastError();
 }
 
+// The turning point of every pixel as the per-ray set-up forms it (0 for a
+// captured or degenerate ray).
+__global__ __launch_bounds__(256) void turning_kernel(const DevScene* __restrict__ sp,
+                                                      double* __restrict__ l) {
+  const DevScene& s = *sp;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= s.W || y >= s.H) return;
+  RaySetup r;
+  setup_ray(s, x, y, false, r);
+  l[(size_t)y * s.W + x] = (r.captured || r.degenerate) ? 0.0 : r.l;
+}
+
+hipError_t launch_turning(const DevScene* scene, double* l, int W, int H, hipStream_t stream) {
+  hipLaunchKernelGGL(turning_kernel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, stream,
+                     scene, l);
+  return hipGetLastError();
+}
+
 // FP64 pass of one frame; the last workgroup clears the queue, so it is clear
 // when the launch completes.
 hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,

@@ -300,6 +300,16 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
  * packed FP32). */
 int bh_fma_issue_rate(bh_ctx* ctx, double* fp32_tflops, double* fp64_tflops);
 
+/* Diagnostic with the same role as the reference's StaticBlackhole::SolveG
+ * (blackhole_solution.h:38-55), per pixel: the turning point l (the lower
+ * end of the 20-step bisection cell of the root of G) of every pixel of the
+ * frame, as the device's per-ray set-up forms it for the FP32 march (closed
+ * form, two Newton steps and the cell, or the bisection itself near a cell
+ * edge and in the photon-sphere band); 0 for a captured or degenerate ray.
+ * l_host receives width x height doubles, row-major. */
+int bh_turning_points(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
+                      double* l_host);
+
 /* ======================================================================== */
 /* Batches: many independent scenes in one persistent launch (no reference  */
 /* counterpart: the reference renders one scene at a time,                  */

@@ -460,3 +460,55 @@ def test_error_paths(renderer):
     s.camera.width = 0
     assert renderer.check_scene(s, p.options) == abi.BH_ERR_INVALID_ARG
     assert renderer.check_scene(p.scene, p.options) == abi.BH_OK
+
+
+@pytest.mark.parametrize("case", ["disc_orbit", "plain_hole", "turned_away", "near_band"])
+def test_turning_points_match_the_reference_bisection(renderer, case):
+    # the device's per-ray turning point (closed-form start, two FP64 Newton
+    # steps, the 20-step bisection cell; bisection near a cell edge) lies in
+    # the bisection cell of the reference's SolveG (blackhole_solution.h:
+    # 38-55, the oracle's or_solve_g) for the pixel's own impact parameter:
+    # the device forms the cell's lower end as l0 + k cell, the reference by
+    # 20 midpoint halvings, so the two agree to a few ulp (a cell is ~3e-7 of
+    # the range: a wrong cell would differ by far more than 1e-12)
+    if case == "disc_orbit":
+        p = scenes.disc_orbit(320, 180, frames=1)
+    elif case == "plain_hole":
+        p = scenes.plain_hole()
+        p.options.precision = abi.PREC_FP32
+    elif case == "turned_away":
+        cam = scenes.look_at(320, 180, math.pi / 3, (-8.0, 0.0, 1.5),
+                             target=(-8.0 + 20 * math.cos(math.radians(100)),
+                                     20 * math.sin(math.radians(100)), 1.5))
+        p = scenes.disc_orbit(320, 180, frames=1).with_camera(cam)
+    else:
+        # a narrow frame around the shadow's edge: every ray near b_c
+        # (q = (b_c/b)^2 up to the band), with no band (every ray takes the
+        # closed form)
+        cam = scenes.look_at(320, 180, math.pi / 900, (-20.0, 0.0, 0.0),
+                             target=(0.0, 2.62, 0.0))
+        p = scenes.disc_orbit(320, 180, frames=1).with_camera(cam)
+        p.options.band = 0.0
+    s, o = p.scene, p.options
+    W, H = s.camera.width, s.camera.height
+    renderer.set_textures(p.textures)
+    print(f"[case] turning points {case} {W}x{H}", flush=True)
+    got = np.empty((H, W), dtype=np.float64)
+    assert renderer.lib.bh_turning_points(renderer.ctx, C.byref(s), C.byref(o),
+                                          got.ctypes.data_as(C.POINTER(C.c_double))) == abi.BH_OK
+    lib = oracle.load()
+    M = s.hole.mass
+    want = np.zeros_like(got)
+    for y in range(H):
+        for x in range(W):
+            ray = oracle.ray_setup(s, o, x, y)
+            if not (ray.captured or ray.degenerate):
+                want[y, x] = lib.or_solve_g(ray.b, M, o.root)
+    live = want > 0
+    assert np.array_equal(got > 0, live)  # the same rays captured
+    rel = np.abs(got[live] - want[live]) / want[live]
+    print(f"[turning] {case}: {int(live.sum())} of {W * H} not captured, "
+          f"max relative difference {rel.max():.3g}, "
+          f"{int((rel > 0).sum())} not bit-equal", flush=True)
+    assert live.sum() > W * H // 10
+    assert rel.max() < 1e-12
