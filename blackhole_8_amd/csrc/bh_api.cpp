@@ -249,6 +249,8 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
   // per-frame constants of the FP32 march
   for (int i = 0; i < 3; ++i) {
     d->fvB[i] = d->fv[i] - d->B[i];
+    d->hW = d->W / 2.0;
+    d->hH = d->H / 2.0;
     d->e1f[i] = (float)(d->F[i] / r0);
     d->nf[i] = (float)d->dn[i];
     d->Bf[i] = (float)d->B[i];

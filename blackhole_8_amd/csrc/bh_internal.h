@@ -49,6 +49,7 @@ struct DevScene {
 
   // ---- per-frame constants of the FP32 march (host-computed) ----
   double fvB[3];          // focus vector relative to B
+  double hW, hH;          // W/2 and H/2
   double bc2, lo2, hi2;   // b_c^2 and the band's bounds on b^2
   double Uc;              // 1/(3M)
   double r02;             // |C - B|^2

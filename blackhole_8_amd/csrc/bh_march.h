@@ -759,7 +759,7 @@ struct RaySetup {
 
 __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bisect,
                                  RaySetup& r) {
-  const double sx = s.W / 2.0 - x, sy = s.H / 2.0 - y;
+  const double sx = s.hW - (double)x, sy = s.hH - (double)y;
   const double pvx = __builtin_fma(-s.vz[0], sy, __builtin_fma(-s.vy[0], sx, s.fvB[0]));
   const double pvy = __builtin_fma(-s.vz[1], sy, __builtin_fma(-s.vy[1], sx, s.fvB[1]));
   const double pvz = __builtin_fma(-s.vz[2], sy, __builtin_fma(-s.vy[2], sx, s.fvB[2]));
@@ -789,11 +789,12 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
   rc = rc * __builtin_fma(-cc, rc, 2.0);
   r.ib2 = dd * rc;
   // e2 = sign(Y0) ((dv.F) F - r0^2 dv) / (r0 |cv|), kept in FP32: the
-  // FP32 1/|cv| scales it by 1 + O(1e-7)
-  const double k = (double)(r.sgn * __builtin_amdgcn_rsqf((float)cc)) * s.u0;
-  r.e2x = (float)(__builtin_fma(dF, Fx, -s.r02 * dvx) * k);
-  r.e2y = (float)(__builtin_fma(dF, Fy, -s.r02 * dvy) * k);
-  r.e2z = (float)(__builtin_fma(dF, Fz, -s.r02 * dvz) * k);
+  // numerator in FP64 (it cancels), its scale in FP32, which sets the length
+  // of e2 to 1 + O(2e-7)
+  const float k = r.sgn * __builtin_amdgcn_rsqf((float)cc) * s.u0f;
+  r.e2x = (float)__builtin_fma(dF, Fx, -s.r02 * dvx) * k;
+  r.e2y = (float)__builtin_fma(dF, Fy, -s.r02 * dvy) * k;
+  r.e2z = (float)__builtin_fma(dF, Fz, -s.r02 * dvz) * k;
   if (r.captured) return;
   // Turning point: StaticBlackhole::SolveG's lower bisection end.  With
   // y = 3 M u and q = (b_c/b)^2, G(u) = 0 reads 3y^2 - 2y^3 = q, a cubic
