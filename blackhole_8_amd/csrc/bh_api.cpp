@@ -42,7 +42,7 @@ struct bh_ctx {
   DeviceTexture tex[BH_MAX_TEXTURES];
   DevBuf scene, rgb, hit, phi, hitpt, steps, qitems, qcount;
   DevBuf dpart, dgrad, drgb, dhit, dgin;  // differentiable render
-  DevBuf bscenes, bouts, boff, bctl, bband, bbandc, bend, btile;  // batches
+  DevBuf bscenes, bouts, boff, bctl, bband, bbandc, bend, btile, btscene;  // batches
   DevBuf probe;   // issue-rate probe; turning points of bh_turning_points
   int wall_khz = 0;
   void* noalloc = nullptr;  // this device's allocation-free scratch array
@@ -456,7 +456,8 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   if ((rc = ensure(ctx, ctx->bctl, 64))) return rc;
   if ((rc = ensure(ctx, ctx->bend, (n + 1) * sizeof(unsigned long long)))) return rc;
   if (t.mode == BH_BATCH_GRID &&
-      (rc = ensure(ctx, ctx->btile, ((size_t)(p.n_rays >> 6) + 1) * sizeof(unsigned long long))))
+      ((rc = ensure(ctx, ctx->btile, ((size_t)(p.n_rays >> 6) + 1) * sizeof(unsigned long long))) ||
+       (rc = ensure(ctx, ctx->btscene, ((size_t)(p.n_rays >> 6) + 1) * sizeof(uint32_t)))))
     return rc;
   // FP64 queue: every ray id is queued at most once (band or texel edge), in
   // the sub-queue of its 64-ray tile
@@ -499,6 +500,7 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   b.refill_idle = t.refill_idle;
   b.mode = t.mode;
   b.tile_end = (unsigned long long*)ctx->btile.p;
+  b.tile_scene = t.mode == BH_BATCH_GRID ? (uint32_t*)ctx->btscene.p : nullptr;
   b.chunk = t.chunk;
   b.scene_end = (unsigned long long*)ctx->bend.p;
   HIP_TRY(ctx, bh::launch_batch_clear(b, st));
@@ -725,7 +727,7 @@ void bh_ctx_destroy(bh_ctx* ctx) {
   for (DevBuf* b : {&ctx->scene, &ctx->rgb, &ctx->hit, &ctx->phi, &ctx->hitpt, &ctx->steps,
                     &ctx->qitems, &ctx->qcount, &ctx->dpart, &ctx->dgrad, &ctx->drgb, &ctx->dhit,
                     &ctx->dgin, &ctx->bscenes, &ctx->bouts, &ctx->boff, &ctx->bctl, &ctx->bband,
-                    &ctx->bbandc, &ctx->bend, &ctx->btile, &ctx->probe})
+                    &ctx->bbandc, &ctx->bend, &ctx->btile, &ctx->btscene, &ctx->probe})
     if (b->p) (void)hipFree(b->p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

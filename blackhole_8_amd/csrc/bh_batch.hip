@@ -389,15 +389,15 @@ __device__ inline void run_tiles(const BatchDev& b, const DevScene* __restrict__
 // Grid mode: one wave per tile of [b.tile_lo, b.tile_hi), no queue; the
 // wave's finish time goes to tile_end[tile] (reduced per scene afterwards).
 template <bool ESCAPE, bool SPH>
-__global__ __launch_bounds__(256) void batch_grid_kernel(BatchDev b,
-                                                         const DevScene* __restrict__ scenes) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 1 : 8)))
+void batch_grid_kernel(BatchDev b, const DevScene* __restrict__ scenes) {
   // the first workgroup is dispatched first: its start is the batch's
   if (blockIdx.x == 0 && threadIdx.x == 0)
     atomicMin(b.t_start, (unsigned long long)wall_clock64());
   const uint32_t tile = b.tile_lo + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (tile >= b.tile_hi) return;
   const uint32_t id0 = tile << 6;
-  const uint32_t sc = (uint32_t)__builtin_amdgcn_readfirstlane((int)scene_of(b, id0));
+  const uint32_t sc = b.tile_scene[__builtin_amdgcn_readfirstlane((int)tile)];
   march_tile<ESCAPE, SPH>(b, scenes[sc], sc, id0 + (uint32_t)lane_id());
   if (lane_id() == 0) b.tile_end[tile] = (unsigned long long)wall_clock64();
 }
@@ -477,11 +477,22 @@ __global__ void batch_clear_kernel(BatchDev b) {
   if (i == 0) b.t_start[0] = ~0ull;
 }
 
+// Grid mode: the scene of every tile (workgroup s fills scene s's tiles), so
+// that a wave finds its scene with one load rather than a search.
+__global__ __launch_bounds__(256) void batch_tile_scene_kernel(BatchDev b) {
+  const uint32_t s = blockIdx.x;
+  const uint32_t t1 = b.ray_off[s + 1] >> 6;
+  for (uint32_t t = (b.ray_off[s] >> 6) + threadIdx.x; t < t1; t += blockDim.x)
+    b.tile_scene[t] = s;
+}
+
 }  // namespace
 
 hipError_t launch_batch_clear(const BatchDev& b, hipStream_t stream) {
   const uint32_t n = b.n_scenes > kQueueSubs ? b.n_scenes : kQueueSubs;
   hipLaunchKernelGGL(batch_clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, b);
+  if (b.tile_scene && b.n_scenes > 0)
+    hipLaunchKernelGGL(batch_tile_scene_kernel, dim3(b.n_scenes), dim3(256), 0, stream, b);
   return hipGetLastError();
 }
 
