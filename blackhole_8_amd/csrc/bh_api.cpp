@@ -457,7 +457,7 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   if ((rc = ensure(ctx, ctx->bend, (n + 1) * sizeof(unsigned long long)))) return rc;
   if (t.mode == BH_BATCH_GRID &&
       ((rc = ensure(ctx, ctx->btile, ((size_t)(p.n_rays >> 6) + 1) * sizeof(unsigned long long))) ||
-       (rc = ensure(ctx, ctx->btscene, ((size_t)(p.n_rays >> 6) + 1) * sizeof(uint32_t)))))
+       (rc = ensure(ctx, ctx->btscene, ((size_t)(p.n_rays >> 6) + 1) * 2 * sizeof(uint32_t)))))
     return rc;
   // FP64 queue: every ray id is queued at most once (band or texel edge), in
   // the sub-queue of its 64-ray tile
@@ -500,7 +500,7 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   b.refill_idle = t.refill_idle;
   b.mode = t.mode;
   b.tile_end = (unsigned long long*)ctx->btile.p;
-  b.tile_scene = t.mode == BH_BATCH_GRID ? (uint32_t*)ctx->btscene.p : nullptr;
+  b.tile_info = t.mode == BH_BATCH_GRID ? (uint2*)ctx->btscene.p : nullptr;
   b.chunk = t.chunk;
   b.scene_end = (unsigned long long*)ctx->bend.p;
   HIP_TRY(ctx, bh::launch_batch_clear(b, st));

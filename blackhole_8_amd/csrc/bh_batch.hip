@@ -330,10 +330,9 @@ __device__ inline void run_lanes(const BatchDev& b, uint32_t waves) {
 // registers).  Returns whether this lane's ray was finished here (not queued
 // for the band pass).
 template <bool ESCAPE, bool SPH>
-__device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t sc,
-                                  uint32_t id) {
-  int x, y;
-  if (!pixel_of(s, id - b.ray_off[sc], &x, &y)) {
+__device__ inline bool march_tile_xy(const BatchDev& b, const DevScene& s, uint32_t sc,
+                                     uint32_t id, int x, int y) {
+  if (!(x < s.W && y < s.H)) {
     push_band(b, false, id, 0u, false);
     return false;
   }
@@ -358,6 +357,14 @@ __device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t
   const bool redo = finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo, phi0) || sredo;
   push_band(b, redo, id, queue_outcome(mo.hit.cls, mo.steps), true);
   return true;
+}
+
+template <bool ESCAPE, bool SPH>
+__device__ inline bool march_tile(const BatchDev& b, const DevScene& s, uint32_t sc,
+                                  uint32_t id) {
+  int x, y;
+  pixel_of(s, id - b.ray_off[sc], &x, &y);
+  return march_tile_xy<ESCAPE, SPH>(b, s, sc, id, x, y);
 }
 
 // Tiles [b.tile_lo, b.tile_hi) hold scenes of one class (escape, spheres),
@@ -397,8 +404,11 @@ void batch_grid_kernel(BatchDev b, const DevScene* __restrict__ scenes) {
   const uint32_t tile = b.tile_lo + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (tile >= b.tile_hi) return;
   const uint32_t id0 = tile << 6;
-  const uint32_t sc = b.tile_scene[__builtin_amdgcn_readfirstlane((int)tile)];
-  march_tile<ESCAPE, SPH>(b, scenes[sc], sc, id0 + (uint32_t)lane_id());
+  // the tile's scene and its position in the scene's grid of 8x8 tiles
+  const uint2 ti = b.tile_info[__builtin_amdgcn_readfirstlane((int)tile)];
+  const int lane = lane_id();
+  const int x = (int)(ti.y & 0xffffu) * 8 + (lane & 7), y = (int)(ti.y >> 16) * 8 + (lane >> 3);
+  march_tile_xy<ESCAPE, SPH>(b, scenes[ti.x], ti.x, id0 + (uint32_t)lane, x, y);
   if (lane_id() == 0) b.tile_end[tile] = (unsigned long long)wall_clock64();
 }
 
@@ -477,13 +487,15 @@ __global__ void batch_clear_kernel(BatchDev b) {
   if (i == 0) b.t_start[0] = ~0ull;
 }
 
-// Grid mode: the scene of every tile (workgroup s fills scene s's tiles), so
-// that a wave finds its scene with one load rather than a search.
-__global__ __launch_bounds__(256) void batch_tile_scene_kernel(BatchDev b) {
+// Grid mode: the scene of every tile and the tile's column and row in the
+// scene (workgroup s fills scene s's tiles), so that a wave finds its scene
+// and pixels with one load rather than a search and a division.
+__global__ __launch_bounds__(256) void batch_tile_info_kernel(BatchDev b) {
   const uint32_t s = blockIdx.x;
-  const uint32_t t1 = b.ray_off[s + 1] >> 6;
-  for (uint32_t t = (b.ray_off[s] >> 6) + threadIdx.x; t < t1; t += blockDim.x)
-    b.tile_scene[t] = s;
+  const uint32_t t0 = b.ray_off[s] >> 6, t1 = b.ray_off[s + 1] >> 6;
+  const uint32_t ntx = ((uint32_t)b.scenes[s].W + 7) >> 3;
+  for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x)
+    b.tile_info[t] = make_uint2(s, ((t - t0) % ntx) | (((t - t0) / ntx) << 16));
 }
 
 }  // namespace
@@ -491,8 +503,8 @@ __global__ __launch_bounds__(256) void batch_tile_scene_kernel(BatchDev b) {
 hipError_t launch_batch_clear(const BatchDev& b, hipStream_t stream) {
   const uint32_t n = b.n_scenes > kQueueSubs ? b.n_scenes : kQueueSubs;
   hipLaunchKernelGGL(batch_clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, b);
-  if (b.tile_scene && b.n_scenes > 0)
-    hipLaunchKernelGGL(batch_tile_scene_kernel, dim3(b.n_scenes), dim3(256), 0, stream, b);
+  if (b.tile_info && b.n_scenes > 0)
+    hipLaunchKernelGGL(batch_tile_info_kernel, dim3(b.n_scenes), dim3(256), 0, stream, b);
   return hipGetLastError();
 }
 

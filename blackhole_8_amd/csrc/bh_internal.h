@@ -191,7 +191,8 @@ struct BatchDev {
   int32_t refill_idle;       // refill once at least this many lanes of 64 idle
   int32_t chunk;             // least tiles per queue pop
   unsigned long long* tile_end;   // grid mode: finish time of each tile
-  uint32_t* tile_scene;           // grid mode: scene of each tile (set by the clear launch)
+  uint2* tile_info;               // grid mode: per tile its scene and (column | row << 16)
+                                  // in the scene's tiles (set by the clear launch)
   unsigned long long* scene_end;  // per scene: latest finish time (wall clock)
   unsigned long long* t_start;    // earliest wave start (wall clock)
 };
