@@ -250,7 +250,11 @@ int bh_render_device(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
  * queue of more than 3 x 2^20 items (larger than about 2560 x 1080) return
  * BH_ERR_INVALID_ARG.  All calls on one GPU share its array: calls that
  * overlap on one GPU (other streams, other contexts) race; calls on one
- * context's stream are ordered. */
+ * context's stream are ordered.  The kernels of scenes without escape or
+ * spheres use no scratch memory; those of escape and sphere scenes spill a
+ * few registers, and the HIP runtime reserves scratch memory for a kernel at
+ * its first launch on a device, so a caller that cannot allocate renders such
+ * a scene once beforehand (with this or any other entry point). */
 int bh_render_device_noalloc(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                              const bh_outputs* out_device);
 
