@@ -243,7 +243,7 @@ def flat_timing(iters: int = 50) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=8,
                     help="untimed launches first: the clock ramps up over the first few")
     ap.add_argument("--frames", type=int, default=120)
