@@ -46,7 +46,9 @@ namespace bh {
 // from B stays inside the wedge theta_c +- asin(rho / D).  A step is tested
 // against a sphere's circle only when its wedge meets that window (plus a
 // margin far above the FP32 angle error), once per revolution of the ray.
-constexpr int kSphSlots = 4;
+// An FP32 lane keeps two spheres (96 VGPRs: 5 waves per SIMD; a ray with more
+// in its plane goes to the FP64 pass), an FP64 lane four.
+template <typename T> constexpr int kSphSlots = sizeof(T) == 4 ? 2 : 4;
 
 template <typename T>
 struct Lane {
@@ -68,18 +70,18 @@ struct Lane {
   std::conditional_t<sizeof(T) == 4, float, double> chi0s;
   double a_ev;        // next disc event angle (crossing - guard)
   // Scenes with spheres only (setup_spheres): the spheres that cut the orbit
-  // plane, at most kSphSlots, in index order.  Slot k holds sphere
+  // plane, at most kSphSlots<T>, in index order.  Slot k holds sphere
   // (sids >> 8k) & 0xff as the circle it cuts from the plane (centre
   // (sx, sy) in the (e1, e2) frame, radius^2 srho2) and the window of swept
   // angle [slo, slo + sw] of the ray's next pass by that circle; a_sph is the
   // earliest window start (1e308 once the ray has ended).
-  T sx[kSphSlots], sy[kSphSlots], srho2[kSphSlots];
-  float slo[kSphSlots], sw[kSphSlots];
+  T sx[kSphSlots<T>], sy[kSphSlots<T>], srho2[kSphSlots<T>];
+  float slo[kSphSlots<T>], sw[kSphSlots<T>];
   uint32_t sids;
   int ns;
   double a_sph;
   // FP32: a chord passed a sphere's surface within the FP32 position error,
-  // or more than kSphSlots spheres cut the plane (the ray is settled in FP64)
+  // or more than kSphSlots<T> spheres cut the plane (the ray is settled in FP64)
   bool sredo;
 };
 
@@ -199,7 +201,7 @@ __device__ inline bool sphere_slots(const DevScene& s, Lane<T>& L, T ra, T sa, T
   const T iqq = rcp<T>(qq);
   double nxt = 1e308;
 #pragma unroll
-  for (int k = 0; k < kSphSlots; ++k) {
+  for (int k = 0; k < kSphSlots<T>; ++k) {
     if (k >= L.ns) continue;
     if (all || ab >= L.slo[k]) {
       const T X = L.sx[k], Y = L.sy[k], rho2 = L.srho2[k];
@@ -686,8 +688,8 @@ __device__ inline void setup_lane(const DevScene& s, double l, double ib2, doubl
 // (D <= rho) has a window of all angles.  FP32 keeps a sphere within 1e-5 R
 // of the plane (the FP32 normal's error) and widens each window by 1e-4 of
 // its half-width plus 1e-4 rad (FP32 angles are good to ~1e-6 rad here).
-// Returns false when more than kSphSlots spheres cut the plane: the first
-// kSphSlots are kept and L.sredo is set.
+// Returns false when more than kSphSlots<T> spheres cut the plane: the first
+// kSphSlots<T> are kept and L.sredo is set.
 template <typename T>
 __device__ inline bool setup_spheres(const DevScene& s, const T e1[3], const T e2[3],
                                      Lane<T>& L) {
@@ -716,7 +718,7 @@ __device__ inline bool setup_spheres(const DevScene& s, const T e1[3], const T e
     const T Z = c0 * n0 + c1 * n1 + c2 * n2;
     const T Rm = R * margin;
     if (!(Z * Z < Rm * Rm)) continue;
-    if (L.ns == kSphSlots) {
+    if (L.ns == kSphSlots<T>) {
       ok = false;
       continue;
     }
@@ -733,7 +735,7 @@ __device__ inline bool setup_spheres(const DevScene& s, const T e1[3], const T e
       if (lo + w2 < 0.f) lo += kTwoPiF;
     }
 #pragma unroll
-    for (int k = 0; k < kSphSlots; ++k) {
+    for (int k = 0; k < kSphSlots<T>; ++k) {
       if (k == L.ns) {
         L.sx[k] = X;
         L.sy[k] = Y;

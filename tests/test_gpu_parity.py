@@ -512,3 +512,29 @@ def test_turning_points_match_the_reference_bisection(renderer, case):
           f"{int((rel > 0).sum())} not bit-equal", flush=True)
     assert live.sum() > W * H // 10
     assert rel.max() < 1e-12
+
+
+def test_widest_edge_guard_settles_every_disc_hit_in_fp64(renderer):
+    # edge_guard 0.01 of r_out spans several texels, so every disc hit of the
+    # FP32 march lies "near a texel edge" and goes through the FP64 pass: the
+    # FP64 crossing point for every disc hit
+    p = scenes.disc_orbit(320, 180, frames=1)
+    o = abi.Options.from_buffer_copy(p.options)
+    o.edge_guard = 0.01
+    renderer.set_textures(p.textures)
+    print("[case] edge guard 0.01 disc_orbit 320x180", flush=True)
+    g = renderer.render(p.scene, o, ALL)
+    _, _, queued = renderer.time_render(p.scene, o, iters=2)
+    r = oracle.render_preset(p, THREADS, ALL)
+    disc = int((r["hit"] == abi.HIT_DISC).sum())
+    print(f"[edge guard] queued {queued}, disc hits {disc}", flush=True)
+    assert disc > 5000 and queued >= disc
+    st = compare(g, r)
+    _log("edge guard 0.01", st)
+    assert st["class_mismatch"] <= _allowed(st["pixels"])
+    assert st["rgb_mismatch"] <= _allowed(st["pixels"])
+    # the disc hits' points are the FP64 pass's (the FP32 ones are ~1e-5 off)
+    both = (g["hit"] == abi.HIT_DISC) & (r["hit"] == abi.HIT_DISC)
+    err = np.abs(g["hitpt"][both] - r["hitpt"][both]).max()
+    print(f"[edge guard] disc hit point max error {err:.3g}", flush=True)
+    assert err <= 1e-9
