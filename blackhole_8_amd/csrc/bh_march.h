@@ -373,16 +373,17 @@ __device__ __forceinline__ void advance(const Lane<T>& L, MarchState<T>& st, T G
   st.a += (double)st.d;
 }
 
-// The exact chord tests of the step just advanced (st.a, st.d), if it is an
-// event (a crossing angle passed, the horizon guard exceeded, or force).
-// Returns true when the chord hits (hit is filled).  With SPH the chord is
-// also tested against the spheres whose window it meets.
+// The exact chord tests of the step just advanced (from angle a_prev to st.a,
+// increment st.d), if it is an event (a crossing angle passed, the horizon
+// guard exceeded, or force).  Returns true when the chord hits (hit is
+// filled).  With SPH the chord is also tested against the spheres whose
+// window it meets.
 template <typename T, typename S, bool SPH = false>
 __device__ __forceinline__ bool step_tests(const S& s, Lane<T>& L, const MarchState<T>& st,
-                                           double first_aa, T ka, T kb, bool force,
-                                           HitRec<T>& hit) {
+                                           double a_prev, double first_aa, T ka, T kb,
+                                           bool force, HitRec<T>& hit) {
   const bool eh = (float)st.d > L.phi_safe;
-  const double aa = force ? first_aa : st.a - (double)st.d;
+  const double aa = force ? first_aa : a_prev;
   const T da = force ? (T)(st.a - first_aa) : st.d;
   bool found = false;
   T best = (T)0;
@@ -407,7 +408,7 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
                                            double first_aa, T G, T ka, T kb, bool force,
                                            HitRec<T>& hit) {
   advance<T>(L, st, G);
-  return step_tests<T, S, SPH>(s, L, st, first_aa, ka, kb, force, hit);
+  return step_tests<T, S, SPH>(s, L, st, st.a - (double)st.d, first_aa, ka, kb, force, hit);
 }
 
 // The march of one ray in precision T.  first_aa is the angle of the camera
@@ -463,40 +464,93 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
   // crossing threshold or sphere window is passed in the block iff the
   // block's last angle passes it, and the horizon guard iff the largest
   // increment exceeds it; only a lane whose block holds such an event
-  // replays the four steps' tests in order, on the same angles (the same
-  // FP64 additions), so every step is tested exactly as one at a time.
+  // replays the four steps' tests in order, on angles that rise monotonely
+  // to the block's last one, so every step is tested as one at a time.
   // The increments grow along the inbound steps and shrink along the
   // outbound ones (G falls as u rises below 1/(3M)), so the largest of a
   // block is its last inbound or its first outbound one (the 1e-4 margin of
   // phi_safe covers rounding).
+  //
+  // FP32 lanes add the block's four trapezoid increments in FP32 before the
+  // FP64 sum: with w_k the half-steps w h, the increments are w_{k-1} + w_k
+  // and their sum telescopes to (w_0 + w_4) + 2 (w_1 + w_2 + w_3), four
+  // FP32 operations and one FP64 addition in place of four increments and
+  // four FP64 additions.  The replay forms its intermediate angles as a0
+  // plus the FP32 partial sums of the increments (monotone, below the
+  // block's sum by at least the last increment) and ends on the block's own
+  // angle; each chord starts where the previous one ended.  FP64 lanes sum
+  // every increment in FP64.
   auto block = [&](int n0, auto coeffs, bool inbound) {
     const double a0 = st.a;
-    T dk[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      T G, ka, kb;
-      coeffs(n0 + k, &G, &ka, &kb);
-      advance<T>(L, st, G);
-      dk[k] = st.d;
-    }
-    const float dmax = (float)(inbound ? dk[3] : dk[0]);
-    bool ev = dmax > L.phi_safe || st.a >= L.a_ev;
-    if constexpr (SPH) ev = ev || dmax > fmaxf(1.f, L.phi_safe) || st.a >= L.a_sph;
-    if (ev) {
-      // the replay recomputes the four sums from a0 rather than keeping them
-      // (8 more live VGPRs in the fast path would cost occupancy): an empty
-      // asm statement hides from the compiler that the copy equals a0
-      double ar = a0;
-      __asm__ volatile("" : "+v"(ar));
-      st.a = ar;
+    if constexpr (sizeof(T) == 4) {
+      const T w0 = st.w;
+      T wk[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         T G, ka, kb;
         coeffs(n0 + k, &G, &ka, &kb);
-        st.d = dk[k];
-        st.a += (double)dk[k];
-        if (step_tests<T, DevScene, SPH>(s, L, st, first_aa, ka, kb, false, o.hit))
-          end_ray(n0 + k);
+        wk[k] = rsq<T>(G);
+      }
+      const T mid = (wk[0] + wk[1]) + wk[2];
+      const T sum = __builtin_fmaf(2.f, mid, w0 + wk[3]);
+      st.w = wk[3];
+      st.a = a0 + (double)sum;
+      const T dlast = wk[2] + wk[3];
+      // the last increment is read after the block only by end_ray, which
+      // an outbound block's successor may call before stepping
+      if (inbound || n0 + 8 > 2 * nstep) st.d = dlast;
+      const float dmax = inbound ? dlast : w0 + wk[0];
+      bool ev = dmax > L.phi_safe || st.a >= L.a_ev;
+      if constexpr (SPH) ev = ev || dmax > fmaxf(1.f, L.phi_safe) || st.a >= L.a_sph;
+      if (ev) {
+        // the block's angles are recomputed from a0 rather than kept (more
+        // live VGPRs in the fast path would cost occupancy): an empty asm
+        // statement hides from the compiler that the copy equals a0
+        double ar = a0;
+        __asm__ volatile("" : "+v"(ar));
+        const double a_end = st.a;
+        double ap = ar;
+        T part = (T)0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          T G, ka, kb;
+          coeffs(n0 + k, &G, &ka, &kb);
+          st.d = (k == 0 ? w0 : wk[k - 1]) + wk[k];
+          part = k == 0 ? st.d : part + st.d;
+          st.a = k == 3 ? a_end : ar + (double)part;
+          if (step_tests<T, DevScene, SPH>(s, L, st, ap, first_aa, ka, kb, false, o.hit))
+            end_ray(n0 + k);
+          ap = st.a;
+        }
+      }
+    } else {
+      T dk[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        T G, ka, kb;
+        coeffs(n0 + k, &G, &ka, &kb);
+        advance<T>(L, st, G);
+        dk[k] = st.d;
+      }
+      const float dmax = (float)(inbound ? dk[3] : dk[0]);
+      bool ev = dmax > L.phi_safe || st.a >= L.a_ev;
+      if constexpr (SPH) ev = ev || dmax > fmaxf(1.f, L.phi_safe) || st.a >= L.a_sph;
+      if (ev) {
+        // the replay recomputes the four sums from a0 (the same FP64
+        // additions) rather than keeping them live
+        double ar = a0;
+        __asm__ volatile("" : "+v"(ar));
+        st.a = ar;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          T G, ka, kb;
+          coeffs(n0 + k, &G, &ka, &kb);
+          st.d = dk[k];
+          st.a += (double)dk[k];
+          if (step_tests<T, DevScene, SPH>(s, L, st, st.a - (double)st.d, first_aa, ka, kb,
+                                           false, o.hit))
+            end_ray(n0 + k);
+        }
       }
     }
   };
