@@ -97,6 +97,7 @@ struct HitRec {
   int cls;       // BH_HIT_*
   int obj;       // sphere index of an object hit
   double ang;    // rotation of the frame the hit point is expressed in
+                 // (FP32 disc hits: the frame of chi0, ang not set)
   T hx, hy;      // hit point relative to B in that frame
 };
 
@@ -142,7 +143,8 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
     const double chi0 = chi0_of(L);
     const double fa = floor((aa - chi0) * kInvPiD);
     const double fb = floor((ab - chi0) * kInvPiD);
-    if (((int)fb - (int)fa) & 1) {
+    const int ja = (int)fa, jb = (int)fb;
+    if ((jb - ja) & 1) {
       const double chik = chi0 + fmax(fa, fb) * kPiD;
       T sa, ca, sb, cb;
       sincos_t<T>((T)(aa - chik), &sa, &ca);
@@ -159,9 +161,17 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
           found = true;
           best = d2;
           hit.cls = BH_HIT_DISC;
-          hit.ang = chik;
-          hit.hx = px;
-          hit.hy = py;
+          if constexpr (sizeof(T) == 4) {
+            // FP32 lanes keep the point in the frame of chi0 (finish_ray
+            // forms its direction without trig): (-1)^j folded into it
+            const bool odd = max(ja, jb) & 1;
+            hit.hx = odd ? -px : px;
+            hit.hy = odd ? -py : py;
+          } else {
+            hit.ang = chik;
+            hit.hx = px;
+            hit.hy = py;
+          }
         }
       }
     }
@@ -921,9 +931,21 @@ __device__ inline bool finish_ray(const DevScene& s, const DevOut& out, size_t i
   bool redo = false;
   float qx = __builtin_nanf(""), qy = qx, qz = qx;
   if (mo.hit.cls != BH_HIT_NONE) {
-    if (!r.degenerate) {
+    // a horizon hit is black: its point is formed only for the hit-point plane
+    if (!r.degenerate && (mo.hit.cls != BH_HIT_HORIZON || out.hitpt)) {
       float sa, ca;
-      sincos_angle(mo.hit.ang, &sa, &ca);
+      if (mo.hit.cls == BH_HIT_DISC) {
+        // a disc hit's point is given in the frame of chi0, where the orbit
+        // plane meets the disc plane: al cos a + be sin a = 0 (setup_lane's
+        // al, be), so (cos, sin)(chi0) = (-be, al) / |(al, be)|, without trig
+        const float al = r.sgn * s.alpha_f;
+        const float be = s.nf[0] * r.e2x + s.nf[1] * r.e2y + s.nf[2] * r.e2z;
+        const float ia = __builtin_amdgcn_rsqf(al * al + be * be);
+        ca = -be * ia;
+        sa = al * ia;
+      } else {
+        sincos_angle(mo.hit.ang, &sa, &ca);
+      }
       const float X = mo.hit.hx * ca - mo.hit.hy * sa;
       const float Y = mo.hit.hx * sa + mo.hit.hy * ca;
       qx = __builtin_fmaf(Y, r.e2x, __builtin_fmaf(X, e1x, s.Bf[0]));

@@ -190,7 +190,7 @@ __device__ __forceinline__ void band_rays(const DevScene& s, const DevOut& out, 
 // Workgroups of the FP64 pass of one frame: edge workgroups per sub-queue and
 // band workgroups.
 constexpr uint32_t kEdgeParts1 = 16, kBandWaves1 = 16;     // one frame alone
-constexpr uint32_t kEdgePartsN = 2, kBandWavesN = 8;       // each frame of a path
+constexpr uint32_t kEdgePartsN = 2, kBandWavesN = 24;      // each frame of a path
 
 template <bool ESCAPE, bool SPH>
 __global__ __launch_bounds__(64) void band_kernel(const DevScene* __restrict__ sp, DevOut out,
