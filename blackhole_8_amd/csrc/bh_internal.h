@@ -50,6 +50,10 @@ struct DevScene {
   // ---- per-frame constants of the FP32 march (host-computed) ----
   double fvB[3];          // focus vector relative to B
   double hW, hH;          // W/2 and H/2
+  // the pixel point pv = fvB - sx vy - sy vz (sx = W/2 - x, sy = H/2 - y)
+  // enters the set-up as dv = F - pv = pxD + sx vy + sy vz,
+  // pv x F = pxC - sx pxCy - sy pxCz and dv . F = pxF[0] + sx pxF[1] + sy pxF[2]
+  double pxD[3], pxC[3], pxCy[3], pxCz[3], pxF[3];
   double bc2, lo2, hi2;   // b_c^2 and the band's bounds on b^2
   double Uc;              // 1/(3M)
   double r02;             // |C - B|^2

@@ -117,7 +117,7 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
     // StaticBlackhole::Collide (blackhole_solution.h:65-88) in the plane
     const T R = (T)s.Rh;
     T sd, cd;
-    sincos_t<T>(da, &sd, &cd);
+    sincos_chord<T>(da, &sd, &cd);
     const T qx = rb * cd - ra, qy = rb * sd;
     const T qd = qx * ra;
     const T qq = qx * qx + qy * qy;
@@ -147,8 +147,9 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
     if ((jb - ja) & 1) {
       const double chik = chi0 + fmax(fa, fb) * kPiD;
       T sa, ca, sb, cb;
-      sincos_t<T>((T)(aa - chik), &sa, &ca);
-      sincos_t<T>((T)(ab - chik), &sb, &cb);
+      // aa <= chik <= ab: both angles lie within the chord's span
+      sincos_chord<T>((T)(aa - chik), &sa, &ca);
+      sincos_chord<T>((T)(ab - chik), &sb, &cb);
       const T wa = fabs(ra * sa), wb = fabs(rb * sb);
       const T inv = rcp<T>(wa + wb);
       const T px = (wb * ra * ca + wa * rb * cb) * inv;
@@ -623,9 +624,7 @@ done:;
 // tex_index for an FP32 coordinate, without the FP64 round trip: the same
 // index for every float v (NaN -> 0; n - 1 < 2^24 is exact in FP32).
 __device__ __forceinline__ int tex_index32(float v, int n) {
-  if (!(v >= 0.f)) return 0;
-  if (v >= (float)(n - 1)) return n - 1;
-  return (int)v;
+  return (int)fminf(fmaxf(v, 0.f), (float)(n - 1));  // max(NaN, 0) = 0
 }
 
 // True when v lies within tol of an integer, a texel edge: the nearest-texel
@@ -825,18 +824,20 @@ struct RaySetup {
 
 __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bisect,
                                  RaySetup& r) {
+  // the pixel point pv = fvB - sx vy - sy vz (relative to B) enters only
+  // through dv = F - pv (along yv), cv = pv x F (|cv| = b |dv|) and dv.F,
+  // each linear in (sx, sy) with coefficients formed once per frame
   const double sx = s.hW - (double)x, sy = s.hH - (double)y;
-  const double pvx = __builtin_fma(-s.vz[0], sy, __builtin_fma(-s.vy[0], sx, s.fvB[0]));
-  const double pvy = __builtin_fma(-s.vz[1], sy, __builtin_fma(-s.vy[1], sx, s.fvB[1]));
-  const double pvz = __builtin_fma(-s.vz[2], sy, __builtin_fma(-s.vy[2], sx, s.fvB[2]));
   const double Fx = s.F[0], Fy = s.F[1], Fz = s.F[2];
-  const double dvx = Fx - pvx, dvy = Fy - pvy, dvz = Fz - pvz;  // along yv
-  const double cvx = pvy * Fz - pvz * Fy;                        // pv x F, |cv| = b |dv|
-  const double cvy = pvz * Fx - pvx * Fz;
-  const double cvz = pvx * Fy - pvy * Fx;
+  const double dvx = __builtin_fma(s.vz[0], sy, __builtin_fma(s.vy[0], sx, s.pxD[0]));
+  const double dvy = __builtin_fma(s.vz[1], sy, __builtin_fma(s.vy[1], sx, s.pxD[1]));
+  const double dvz = __builtin_fma(s.vz[2], sy, __builtin_fma(s.vy[2], sx, s.pxD[2]));
+  const double cvx = __builtin_fma(-s.pxCz[0], sy, __builtin_fma(-s.pxCy[0], sx, s.pxC[0]));
+  const double cvy = __builtin_fma(-s.pxCz[1], sy, __builtin_fma(-s.pxCy[1], sx, s.pxC[1]));
+  const double cvz = __builtin_fma(-s.pxCz[2], sy, __builtin_fma(-s.pxCy[2], sx, s.pxC[2]));
   const double cc = __builtin_fma(cvz, cvz, __builtin_fma(cvy, cvy, cvx * cvx));
   const double dd = __builtin_fma(dvz, dvz, __builtin_fma(dvy, dvy, dvx * dvx));
-  const double dF = __builtin_fma(dvz, Fz, __builtin_fma(dvy, Fy, dvx * Fx));
+  const double dF = __builtin_fma(s.pxF[2], sy, __builtin_fma(s.pxF[1], sx, s.pxF[0]));
   r.cc = cc;
   r.dd = dd;
   r.dF = dF;
