@@ -153,7 +153,10 @@ __device__ inline bool event_chord(const S& s, const Lane<T>& L, T ka, double aa
       const T wa = fabs(ra * sa), wb = fabs(rb * sb);
       const T inv = rcp<T>(wa + wb);
       const T px = (wb * ra * ca + wa * rb * cb) * inv;
-      const T py = (wb * ra * sa + wa * rb * sb) * inv;
+      // sa <= 0 <= sb, so the weighted heights cancel exactly: the point
+      // lies on the crossing line (FP32 lanes take py = 0, FP64 lanes keep
+      // the reference's interpolation and its rounding)
+      const T py = sizeof(T) == 4 ? (T)0 : (wb * ra * sa + wa * rb * sb) * inv;
       const T rho2 = px * px + py * py;
       if (!(rho2 > (T)s.r_out2) && !(rho2 < (T)s.r_in2)) {
         const T dx = ra * ca - px, dy = ra * sa - py;
