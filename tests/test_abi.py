@@ -103,3 +103,56 @@ def test_oracle_library_is_separate_from_the_product():
     assert "or_render" not in out and "or_trace_pixel" not in out
     deps = subprocess.run(["ldd", str(abi.LIB_PATH)], capture_output=True, text=True).stdout
     assert "oracle" not in deps
+
+
+# ---------------------------------------------------------------------------
+# Register budget of the FP32 march kernels, read from the library's own gfx950
+# code objects (the .hip_fatbin section holds one offload bundle per source
+# file).  The march is issue-bound at 8 waves per SIMD, which needs <= 64
+# VGPRs, and a kernel with a private segment (scratch) makes the runtime
+# allocate scratch memory at its first launch.
+# ---------------------------------------------------------------------------
+_LLVM = Path("/opt/rocm/llvm/bin")
+_BUNDLE_MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
+
+
+def _kernel_metadata():
+    """{kernel symbol: (vgpr_count, private_segment_fixed_size)} over every
+    gfx950 code object of the built library."""
+    bundler, readelf = _LLVM / "clang-offload-bundler", _LLVM / "llvm-readelf"
+    if not (bundler.exists() and readelf.exists()):
+        pytest.skip("ROCm LLVM tools not present")
+    meta = {}
+    with tempfile.TemporaryDirectory() as d:
+        fat = Path(d) / "fat.bin"
+        subprocess.run(["objcopy", "--dump-section", f".hip_fatbin={fat}", str(abi.LIB_PATH)],
+                       check=True, capture_output=True)
+        blob = fat.read_bytes()
+        starts = [m.start() for m in re.finditer(re.escape(_BUNDLE_MAGIC), blob)]
+        assert starts, "no offload bundle in the library"
+        for i, s in enumerate(starts):
+            part = Path(d) / f"b{i}.bin"
+            part.write_bytes(blob[s:starts[i + 1] if i + 1 < len(starts) else len(blob)])
+            co = Path(d) / f"b{i}.co"
+            subprocess.run([str(bundler), "--unbundle", "--type=o",
+                            "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--input={part}",
+                            f"--output={co}"], check=True, capture_output=True)
+            notes = subprocess.run([str(readelf), "--notes", str(co)], check=True,
+                                   capture_output=True, text=True).stdout
+            for block in notes.split("  - .")[1:]:
+                name = re.search(r"\.?name:\s+(\S+)", block)
+                vg = re.search(r"\.vgpr_count:\s+(\d+)", block)
+                ps = re.search(r"\.private_segment_fixed_size:\s+(\d+)", block)
+                if name and vg and ps:
+                    meta[name.group(1)] = (int(vg.group(1)), int(ps.group(1)))
+    return meta
+
+
+def test_fp32_march_kernels_fit_eight_waves_without_scratch():
+    meta = _kernel_metadata()
+    hot = {k: v for k, v in meta.items()
+           if re.search(r"(march_frames_kernel|march_kernel|batch_grid_kernel)ILb0ELb0E", k)}
+    assert len(hot) == 3, sorted(meta)
+    for name, (vgpr, scratch) in hot.items():
+        assert vgpr <= 64, (name, vgpr)
+        assert scratch == 0, (name, scratch)
