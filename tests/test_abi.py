@@ -125,8 +125,9 @@ def _kernel_metadata():
     meta = {}
     with tempfile.TemporaryDirectory() as d:
         fat = Path(d) / "fat.bin"
-        subprocess.run(["objcopy", "--dump-section", f".hip_fatbin={fat}", str(abi.LIB_PATH)],
-                       check=True, capture_output=True)
+        # (objcopy given one file rewrites it in place: always name an output)
+        subprocess.run(["objcopy", "-O", "binary", "--only-section=.hip_fatbin",
+                        str(abi.LIB_PATH), str(fat)], check=True, capture_output=True)
         blob = fat.read_bytes()
         starts = [m.start() for m in re.finditer(re.escape(_BUNDLE_MAGIC), blob)]
         assert starts, "no offload bundle in the library"
