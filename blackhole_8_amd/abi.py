@@ -131,6 +131,7 @@ SIGNATURES = {
     "bh_render_device_noalloc": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options),
                                            C.POINTER(Outputs)]),
     "bh_fma_issue_rate": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bh_chord_sincos": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64]),
     "bh_turning_points": (C.c_int, [P, C.POINTER(Scene), C.POINTER(Options),
                                     C.POINTER(C.c_double)]),
     "bh_path_last_ms": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

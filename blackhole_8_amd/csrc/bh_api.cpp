@@ -1156,6 +1156,26 @@ int bh_fma_issue_rate(bh_ctx* ctx, double* fp32_tflops, double* fp64_tflops) {
   });
 }
 
+int bh_chord_sincos(bh_ctx* ctx, const float* x_host, float* sc_host, int64_t n) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return fail(nullptr, BH_ERR_BAD_HANDLE, "ctx is NULL");
+    if (!x_host || !sc_host) return fail(ctx, BH_ERR_INVALID_ARG, "buffers are NULL");
+    if (n <= 0 || n > (int64_t)1 << 24) return fail(ctx, BH_ERR_INVALID_ARG, "n must be in 1..2^24");
+    (void)hipSetDevice(ctx->device);
+    int rc = ensure(ctx, ctx->probe, (size_t)n * 3 * sizeof(float));
+    if (rc) return rc;
+    float* x = (float*)ctx->probe.p;
+    float* out = x + n;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(x, x_host, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, bh::launch_chord_sincos(x, out, (int)n, st));
+    HIP_TRY(ctx, hipMemcpyAsync(sc_host, out, (size_t)n * 2 * sizeof(float),
+                                hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    return BH_OK;
+  });
+}
+
 int bh_turning_points(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
                       double* l_host) {
   return guarded(ctx, [&]() -> int {

@@ -156,6 +156,9 @@ hipError_t launch_noalloc_setup(const DevScene& s, const NoallocLayout& l, hipSt
 // precision T, 8 waves per SIMD on n_cu compute units; sink: 256 elements.
 template <typename T>
 hipError_t fma_probe(int n_cu, hipStream_t stream, T* sink, double* tflops);
+// The FP32 chord tests' sin and cos of n angles x (bh_probe.hip): out holds
+// n (sin, cos) pairs.
+hipError_t launch_chord_sincos(const float* x, float* out, int n, hipStream_t stream);
 
 hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, int H,
                              hipStream_t stream);

@@ -65,32 +65,6 @@ __device__ __forceinline__ void sincos_small(float x, float* s, float* c) {
   quadrant((int)k & 3, sr, cr, s, c);
 }
 
-// FP32 sin/cos for |x| <= pi (a chord's angular span and the angles within
-// it): the Taylor series of h = x/2 (sin to h^13, cos to h^12; truncation
-// below 7e-10 and 7e-9 on |h| <= pi/2), then sin x = 2 sin h cos h and
-// cos x = 1 - 2 sin^2 h, with no quadrant reduction.  Errors against FP64:
-// below 9e-8 for |x| <= 1, below 2e-7 (sin) and 5e-7 (cos) up to pi; the
-// larger ones come only with chords of a radian or more, which lie near the
-// hole (r of a few r_s), where they move a point by ~1e-6 r_s.
-__device__ __forceinline__ void sincos_pi(float x, float* s, float* c) {
-  const float h = 0.5f * x, h2 = h * h;
-  float ps = __builtin_fmaf(h2, 1.6059044e-10f, -2.5052108e-8f);  // 1/13!, -1/11!
-  ps = __builtin_fmaf(ps, h2, 2.7557319e-6f);                      // 1/9!
-  ps = __builtin_fmaf(ps, h2, -1.9841270e-4f);                     // -1/7!
-  ps = __builtin_fmaf(ps, h2, 8.3333333e-3f);                      // 1/5!
-  ps = __builtin_fmaf(ps, h2, -1.6666667e-1f);                     // -1/3!
-  const float sh = __builtin_fmaf(ps * h2, h, h);
-  float pc = __builtin_fmaf(h2, 2.0876757e-9f, -2.7557319e-7f);   // 1/12!, -1/10!
-  pc = __builtin_fmaf(pc, h2, 2.4801587e-5f);                      // 1/8!
-  pc = __builtin_fmaf(pc, h2, -1.3888889e-3f);                     // -1/6!
-  pc = __builtin_fmaf(pc, h2, 4.1666667e-2f);                      // 1/4!
-  pc = __builtin_fmaf(pc, h2, -0.5f);
-  const float ch = __builtin_fmaf(pc, h2, 1.0f);
-  const float s2 = sh + sh;
-  *s = s2 * ch;
-  *c = __builtin_fmaf(-s2, sh, 1.0f);
-}
-
 // sin/cos of an FP64 angle of any size: reduce in FP64, evaluate in FP32.
 __device__ __forceinline__ void sincos_angle(double a, float* s, float* c) {
   const double k = rint(a * (2.0 * kInvPiD));
@@ -141,10 +115,17 @@ __device__ inline void sincos_small64(double x, double* s, double* c) {
 template <typename T> __device__ __forceinline__ void sincos_t(T x, T* s, T* c);
 template <> __device__ __forceinline__ void sincos_t<float>(float x, float* s, float* c) { sincos_small(x, s, c); }
 template <> __device__ __forceinline__ void sincos_t<double>(double x, double* s, double* c) { sincos_small64(x, s, c); }
-// sin/cos of a chord's span and the angles within it (|x| < pi): FP32 lanes
-// use the half-angle series.
+// sin/cos of a chord's span and the angles within it (|x| < pi).  FP32
+// lanes use the hardware sine and cosine (argument in revolutions): within
+// 1.4e-7 of FP64 for |x| <= 1 and 2.7e-7 up to pi, the sine within 1e-6 of
+// itself at small angles (measured with bh_chord_sincos; the GPU test holds
+// these bounds).  Both run on the transcendental unit, beside the VALU work.
 template <typename T> __device__ __forceinline__ void sincos_chord(T x, T* s, T* c);
-template <> __device__ __forceinline__ void sincos_chord<float>(float x, float* s, float* c) { sincos_pi(x, s, c); }
+template <> __device__ __forceinline__ void sincos_chord<float>(float x, float* s, float* c) {
+  const float r = x * 0.15915494309189535f;  // 1 / (2 pi)
+  *s = __builtin_amdgcn_sinf(r);
+  *c = __builtin_amdgcn_cosf(r);
+}
 template <> __device__ __forceinline__ void sincos_chord<double>(double x, double* s, double* c) { sincos_small64(x, s, c); }
 
 // atan2 in FP32 to ~1e-7 rad: octant reduction and an odd polynomial for

@@ -4,6 +4,7 @@
 // is issued against.  Every lane runs eight independent FMA chains, eight
 // waves per SIMD, so dependency latency is hidden and the time is issue time.
 #include "bh_internal.h"
+#include "bh_math.h"
 
 namespace bh {
 namespace {
@@ -68,5 +69,22 @@ hipError_t fma_probe(int n_cu, hipStream_t stream, T* sink, double* tflops) {
 
 template hipError_t fma_probe<float>(int, hipStream_t, float*, double*);
 template hipError_t fma_probe<double>(int, hipStream_t, double*, double*);
+
+// sin and cos of n FP32 angles as the FP32 march's chord tests form them
+// (out: n (sin, cos) pairs).
+__global__ __launch_bounds__(256) void chord_sincos_kernel(const float* __restrict__ x,
+                                                           float* __restrict__ out, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float s, c;
+  sincos_chord<float>(x[i], &s, &c);
+  out[2 * i] = s;
+  out[2 * i + 1] = c;
+}
+
+hipError_t launch_chord_sincos(const float* x, float* out, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(chord_sincos_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, out, n);
+  return hipGetLastError();
+}
 
 }  // namespace bh

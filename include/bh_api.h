@@ -304,6 +304,12 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
  * packed FP32). */
 int bh_fma_issue_rate(bh_ctx* ctx, double* fp32_tflops, double* fp64_tflops);
 
+/* Measurement helper with no counterpart in the reference: the sine and
+ * cosine of n FP32 angles (|x| < pi, the span of a chord and the angles within
+ * it) exactly as the FP32 march's chord tests form them on the context's
+ * device; sc_host receives n (sin, cos) pairs.  n in 1..2^24. */
+int bh_chord_sincos(bh_ctx* ctx, const float* x_host, float* sc_host, int64_t n);
+
 /* Diagnostic with the same role as the reference's StaticBlackhole::SolveG
  * (blackhole_solution.h:38-55), per pixel: the turning point l (the lower
  * end of the 20-step bisection cell of the root of G) of every pixel of the

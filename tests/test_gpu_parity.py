@@ -538,3 +538,29 @@ def test_widest_edge_guard_settles_every_disc_hit_in_fp64(renderer):
     err = np.abs(g["hitpt"][both] - r["hitpt"][both]).max()
     print(f"[edge guard] disc hit point max error {err:.3g}", flush=True)
     assert err <= 1e-9
+
+
+def test_chord_sincos_within_its_stated_bounds(renderer):
+    # the FP32 chord tests' sin/cos (hardware sine and cosine) against FP64 over
+    # the angles a chord can span: [-pi, pi] uniformly and, log-spaced, the
+    # small spans of chords far from the hole down to 1e-8 rad
+    x = np.concatenate([np.linspace(-np.pi, np.pi, 1 << 20),
+                        np.logspace(-8, 0, 4096), -np.logspace(-8, 0, 4096)]).astype(np.float32)
+    sc = np.empty((x.size, 2), dtype=np.float32)
+    print(f"[case] chord sincos, {x.size} angles", flush=True)
+    assert renderer.lib.bh_chord_sincos(renderer.ctx, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                        sc.ctypes.data_as(C.POINTER(C.c_float)),
+                                        x.size) == abi.BH_OK
+    xd = x.astype(np.float64)
+    es = np.abs(sc[:, 0] - np.sin(xd))
+    ec = np.abs(sc[:, 1] - np.cos(xd))
+    small = np.abs(xd) <= 1.0
+    tiny = (np.abs(xd) >= 1e-8) & small
+    rel = es[tiny] / np.abs(np.sin(xd[tiny]))
+    print(f"[chord sincos] max |sin err| {es.max():.3g} ({es[small].max():.3g} for |x| <= 1), "
+          f"max |cos err| {ec.max():.3g} ({ec[small].max():.3g}), "
+          f"max relative sin err {rel.max():.3g}", flush=True)
+    assert np.isfinite(sc).all()
+    assert es.max() <= 4e-7 and ec.max() <= 4e-7
+    assert es[small].max() <= 2e-7 and ec[small].max() <= 2e-7
+    assert rel.max() <= 2e-6
