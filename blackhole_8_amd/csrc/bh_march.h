@@ -832,15 +832,17 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
   // each linear in (sx, sy) with coefficients formed once per frame
   const double sx = s.hW - (double)x, sy = s.hH - (double)y;
   const double Fx = s.F[0], Fy = s.F[1], Fz = s.F[2];
-  const double dvx = __builtin_fma(s.vz[0], sy, __builtin_fma(s.vy[0], sx, s.pxD[0]));
-  const double dvy = __builtin_fma(s.vz[1], sy, __builtin_fma(s.vy[1], sx, s.pxD[1]));
-  const double dvz = __builtin_fma(s.vz[2], sy, __builtin_fma(s.vy[2], sx, s.pxD[2]));
-  const double cvx = __builtin_fma(-s.pxCz[0], sy, __builtin_fma(-s.pxCy[0], sx, s.pxC[0]));
-  const double cvy = __builtin_fma(-s.pxCz[1], sy, __builtin_fma(-s.pxCy[1], sx, s.pxC[1]));
-  const double cvz = __builtin_fma(-s.pxCz[2], sy, __builtin_fma(-s.pxCy[2], sx, s.pxC[2]));
+  // (a vector instruction reads one scalar operand: the pixel terms first,
+  // the frame's constant term last, so no constant is copied to a register)
+  const double dvx = __builtin_fma(s.vz[0], sy, s.vy[0] * sx) + s.pxD[0];
+  const double dvy = __builtin_fma(s.vz[1], sy, s.vy[1] * sx) + s.pxD[1];
+  const double dvz = __builtin_fma(s.vz[2], sy, s.vy[2] * sx) + s.pxD[2];
+  const double cvx = s.pxC[0] - __builtin_fma(s.pxCz[0], sy, s.pxCy[0] * sx);
+  const double cvy = s.pxC[1] - __builtin_fma(s.pxCz[1], sy, s.pxCy[1] * sx);
+  const double cvz = s.pxC[2] - __builtin_fma(s.pxCz[2], sy, s.pxCy[2] * sx);
   const double cc = __builtin_fma(cvz, cvz, __builtin_fma(cvy, cvy, cvx * cvx));
   const double dd = __builtin_fma(dvz, dvz, __builtin_fma(dvy, dvy, dvx * dvx));
-  const double dF = __builtin_fma(s.pxF[2], sy, __builtin_fma(s.pxF[1], sx, s.pxF[0]));
+  const double dF = __builtin_fma(s.pxF[2], sy, s.pxF[1] * sx) + s.pxF[0];
   r.cc = cc;
   r.dd = dd;
   r.dF = dF;
