@@ -62,6 +62,7 @@ struct bh_path {
   uint32_t sub_cap = 0;              // FP64 queue of each frame: kQueueSubs x sub_cap items
   bool fused = false;                // all frames FP32: one march + one band launch
   bool esc = false, sph = false;     // scene class of the fused launches
+  int nstep = 0;                     // the frames' step count (one options record)
   std::vector<hipEvent_t> events;
   // fused paths: the march over all frames and the FP64 pass over all frames
   // are two graphs, launched back to back with timing events around each
@@ -354,7 +355,7 @@ int launch_frame(bh_ctx* ctx, const DevScene& host_scene, DevScene* d_scene, con
       return fail(ctx, BH_ERR_INVALID_ARG, "FP32 march launched without a band queue");
     const bool esc = host_scene.escape != 0;
     const bool sph = host_scene.n_spheres > 0;
-    HIP_TRY(ctx, bh::launch_march_fp32(d_scene, out, W, H, q, st, esc, sph));
+    HIP_TRY(ctx, bh::launch_march_fp32(d_scene, out, W, H, q, st, esc, sph, host_scene.nstep));
     HIP_TRY(ctx, bh::launch_band(d_scene, out, q, st, esc, sph));
   } else {
     HIP_TRY(ctx, bh::launch_march_fp64(d_scene, out, W, H, st));
@@ -670,7 +671,7 @@ hipError_t path_march(const bh_path* p, hipStream_t st) {
   for (int f0 = 0; f0 < p->n_frames; f0 += kFramesPerLaunch) {
     const int nf = std::min(kFramesPerLaunch, p->n_frames - f0);
     const hipError_t e = bh::launch_march_frames(p->d_scenes, o, p->W, p->H, f0, nf, q, st,
-                                                 p->esc, p->sph);
+                                                 p->esc, p->sph, p->nstep);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -955,7 +956,8 @@ int bh_time_render(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt, in
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     for (int it = 0; it < iters; ++it) {
       if (fast) {
-        HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0, d.n_spheres > 0));
+        HIP_TRY(ctx, bh::launch_march_fp32(ds, o, d.W, d.H, q, st, d.escape != 0, d.n_spheres > 0,
+                                           d.nstep));
       } else {
         HIP_TRY(ctx, bh::launch_march_fp64(ds, o, d.W, d.H, st));
       }
@@ -1028,6 +1030,7 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
     for (const DevScene& f : frames) p->fused = p->fused && fp32_eligible(f);
     p->esc = frames[0].escape != 0;
     p->sph = frames[0].n_spheres > 0;
+    p->nstep = frames[0].nstep;
     if ((e = hipMalloc(&p->d_qitems, qitems * n_frames * sizeof(uint64_t))) != hipSuccess) return cleanup(e, "hipMalloc queue");
     if ((e = hipMalloc(&p->d_qcount, qwords * 4 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMalloc queue count");
     if ((e = hipMemset(p->d_qcount, 0, qwords * 4 * (size_t)n_frames)) != hipSuccess) return cleanup(e, "hipMemset queue count");
@@ -1052,7 +1055,8 @@ int bh_path_create(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,
       if (fp32_eligible(f)) {
         const bool esc = f.escape != 0;
         const bool sph = f.n_spheres > 0;
-        hipError_t le = bh::launch_march_fp32(p->d_scenes + i, o, p->W, p->H, q, st, esc, sph);
+        hipError_t le =
+            bh::launch_march_fp32(p->d_scenes + i, o, p->W, p->H, q, st, esc, sph, f.nstep);
         if (le == hipSuccess) le = hipEventRecord(p->events[(size_t)i], st);
         if (le == hipSuccess) le = hipStreamWaitEvent(side, p->events[(size_t)i], 0);
         if (le == hipSuccess) le = bh::launch_band(p->d_scenes + i, o, q, side, esc, sph);

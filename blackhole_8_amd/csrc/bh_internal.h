@@ -162,15 +162,18 @@ hipError_t launch_chord_sincos(const float* x, float* out, int n, hipStream_t st
 
 hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, int H,
                              hipStream_t stream);
+// nstep: the scene's step count (the reference's 20 takes the instance
+// compiled for it).
 hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, int H,
-                             const DevQueue& q, hipStream_t stream, bool escape, bool spheres);
+                             const DevQueue& q, hipStream_t stream, bool escape, bool spheres,
+                             int nstep);
 hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue& q,
                        hipStream_t stream, bool escape, bool spheres);
 // Frames [frame0, frame0 + nf) of a path: frame f's queue is q shifted by f
 // queues of queue_items(W, H) items and kQueueCountWords count words.
 hipError_t launch_march_frames(const DevScene* scenes, const DevOut& out, int W, int H,
                                int frame0, int nf, const DevQueue& q, hipStream_t stream,
-                               bool escape, bool spheres);
+                               bool escape, bool spheres, int nstep);
 hipError_t launch_band_frames(const DevScene* scenes, const DevOut& out, int W, int H,
                               int frame0, int nf, const DevQueue& q, hipStream_t stream,
                               bool escape, bool spheres);

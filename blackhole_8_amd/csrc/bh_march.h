@@ -437,10 +437,17 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
 // steps the wave leaves once no lane is armed.  The four steps take one
 // event check between them (block), which keeps the exec-mask bookkeeping of
 // the hot loop to a handful of scalar instructions per four steps.
-template <typename T, bool ESCAPE, bool SPH = false>
+// The step count of the reference's step rule (blackhole_solution_test.cc
+// and the shipped scene): the FP32 march has an instance compiled for it.
+constexpr int kRefSteps = 20;
+
+template <typename T, bool ESCAPE, bool SPH = false, int NS = 0>
 __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, double first_aa,
                              double du64, MarchOut<T>& o) {
-  const int nstep = s.nstep;
+  // NS > 0: the scene's nstep, known at compile time: the blocks unroll, and
+  // their step values and chord ends become literal operands (no per-step
+  // integer-to-float conversion, no loop bookkeeping)
+  const int nstep = NS > 0 ? NS : s.nstep;
   MarchState<T> st{(T)0, 0.0, (T)0};
   o.hit.cls = BH_HIT_NONE;
   o.hit.obj = 0;
@@ -573,9 +580,18 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
   // inbound (blackhole_solution_test.cc:217-239)
   step_in(1, first_aa != 0.0);
   int n = 2;
-  for (; n + 3 < nstep; n += 4) {
-    block(n, in_coeffs, true);
-    if (!__any(armed())) goto done;
+  if constexpr (NS > 0) {
+#pragma unroll
+    for (int b = 0; b < (NS - 2) / 4; ++b) {
+      block(2 + 4 * b, in_coeffs, true);
+      if (!__any(armed())) goto done;
+    }
+    n = 2 + 4 * ((NS - 2) / 4);
+  } else {
+    for (; n + 3 < nstep; n += 4) {
+      block(n, in_coeffs, true);
+      if (!__any(armed())) goto done;
+    }
   }
   for (; n < nstep; ++n) step_in(n, false);
   {
@@ -602,9 +618,18 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     if (!__any(armed())) goto done;
     // outbound: u -= du (:274-296)
     n = nstep + 1;
-    for (; n + 3 < 2 * nstep; n += 4) {
-      block(n, out_coeffs, false);
-      if (!__any(armed())) goto done;
+    if constexpr (NS > 0) {
+#pragma unroll
+      for (int b = 0; b < (NS - 1) / 4; ++b) {
+        block(NS + 1 + 4 * b, out_coeffs, false);
+        if (!__any(armed())) goto done;
+      }
+      n = NS + 1 + 4 * ((NS - 1) / 4);
+    } else {
+      for (; n + 3 < 2 * nstep; n += 4) {
+        block(n, out_coeffs, false);
+        if (!__any(armed())) goto done;
+      }
     }
     for (; n < 2 * nstep; ++n) step_out(n);
     if (ESCAPE) {

@@ -51,7 +51,7 @@ __device__ __forceinline__ void queue_push_edge(const DevQueue& q, uint32_t k, u
 
 // One ray of the FP32 march: pixel (x, y) of frame s, marched by a workgroup
 // of the frame whose queue is sub-queue k.
-template <bool ESCAPE, bool SPH>
+template <bool ESCAPE, bool SPH, int NS>
 __device__ __forceinline__ void march_pixel(const DevScene& s, const DevOut& out,
                                             const DevQueue& q, uint32_t k, int x, int y) {
   const size_t idx = (size_t)y * s.W + x;
@@ -69,7 +69,7 @@ __device__ __forceinline__ void march_pixel(const DevScene& s, const DevOut& out
     const double du = (r.l - s.u0) * s.inv_nstep;
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
     if constexpr (SPH) setup_spheres32(s, r, L);
-    march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
+    march<float, ESCAPE, SPH, NS>(s, L, r.captured, r.first_aa, du, mo);
     if constexpr (SPH) sredo = L.sredo;
   }
   // the texel lies within the FP32 error of a texel edge (or a sphere was
@@ -79,14 +79,14 @@ __device__ __forceinline__ void march_pixel(const DevScene& s, const DevOut& out
 }
 
 // Without spheres the march fits 64 VGPRs: 8 waves per SIMD.
-template <bool ESCAPE, bool SPH>
+template <bool ESCAPE, bool SPH, int NS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 1 : 8)))
 void march_kernel(const DevScene* __restrict__ sp, DevOut out, DevQueue q) {
   const DevScene& s = *sp;
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= s.W || y >= s.H) return;
-  march_pixel<ESCAPE, SPH>(s, out, q, queue_sub(blockIdx.x, blockIdx.y), x, y);
+  march_pixel<ESCAPE, SPH, NS>(s, out, q, queue_sub(blockIdx.x, blockIdx.y), x, y);
 }
 
 // Per-frame planes and queue of frame f of a camera path: planes at offset
@@ -105,7 +105,7 @@ __device__ __forceinline__ void frame_slice(const DevOut& out, const DevQueue& q
 
 // Every frame of a camera path in one launch (blockIdx.z = frame - frame0):
 // consecutive frames share no ramp-up or drain of the machine.
-template <bool ESCAPE, bool SPH>
+template <bool ESCAPE, bool SPH, int NS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 1 : 8)))
 void march_frames_kernel(const DevScene* __restrict__ sp, DevOut out, DevQueue q, size_t npix,
                          int frame0) {
@@ -117,7 +117,7 @@ void march_frames_kernel(const DevScene* __restrict__ sp, DevOut out, DevQueue q
   DevOut o;
   DevQueue qf;
   frame_slice(out, q, f, npix, &o, &qf);
-  march_pixel<ESCAPE, SPH>(s, o, qf, queue_sub(blockIdx.x, blockIdx.y), x, y);
+  march_pixel<ESCAPE, SPH, NS>(s, o, qf, queue_sub(blockIdx.x, blockIdx.y), x, y);
 }
 
 // FP64 pass over one frame's queue, by one-wave workgroups of two kinds, so
@@ -227,18 +227,30 @@ hipError_t launch_march_fp64(const DevScene* scene, const DevOut& out, int W, in
 }
 
 hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, int H,
-                             const DevQueue& q, hipStream_t stream, bool escape, bool spheres) {
+                             const DevQueue& q, hipStream_t stream, bool escape, bool spheres,
+                             int nstep) {
   dim3 grid((W + 15) / 16, (H + 15) / 16);
+  const bool ref = nstep == kRefSteps;  // the instance compiled for the reference's nstep
   if (spheres) {
     if (escape)
-      hipLaunchKernelGGL((march_kernel<true, true>), grid, dim3(256), 0, stream, scene, out, q);
+      hipLaunchKernelGGL((march_kernel<true, true, 0>), grid, dim3(256), 0, stream, scene, out, q);
     else
-      hipLaunchKernelGGL((march_kernel<false, true>), grid, dim3(256), 0, stream, scene, out, q);
+      hipLaunchKernelGGL((march_kernel<false, true, 0>), grid, dim3(256), 0, stream, scene, out,
+                         q);
+  } else if (escape) {
+    if (ref)
+      hipLaunchKernelGGL((march_kernel<true, false, kRefSteps>), grid, dim3(256), 0, stream,
+                         scene, out, q);
+    else
+      hipLaunchKernelGGL((march_kernel<true, false, 0>), grid, dim3(256), 0, stream, scene, out,
+                         q);
   } else {
-    if (escape)
-      hipLaunchKernelGGL((march_kernel<true, false>), grid, dim3(256), 0, stream, scene, out, q);
+    if (ref)
+      hipLaunchKernelGGL((march_kernel<false, false, kRefSteps>), grid, dim3(256), 0, stream,
+                         scene, out, q);
     else
-      hipLaunchKernelGGL((march_kernel<false, false>), grid, dim3(256), 0, stream, scene, out, q);
+      hipLaunchKernelGGL((march_kernel<false, false, 0>), grid, dim3(256), 0, stream, scene, out,
+                         q);
   }
   return hipGetLastError();
 }
@@ -281,26 +293,35 @@ hipError_t launch_band(const DevScene* scene, const DevOut& out, const DevQueue&
   return hipGetLastError();
 }
 
-// Frames [frame0, frame0 + nf) of a camera path, nf <= 65535.
+// Frames [frame0, frame0 + nf) of a camera path, nf <= 65535, all with the
+// step count nstep.
 hipError_t launch_march_frames(const DevScene* scenes, const DevOut& out, int W, int H,
                                int frame0, int nf, const DevQueue& q, hipStream_t stream,
-                               bool escape, bool spheres) {
+                               bool escape, bool spheres, int nstep) {
   const dim3 grid((W + 15) / 16, (H + 15) / 16, nf);
   const size_t npix = (size_t)W * H;
+  const bool ref = nstep == kRefSteps;
   if (spheres) {
     if (escape)
-      hipLaunchKernelGGL((march_frames_kernel<true, true>), grid, dim3(256), 0, stream, scenes,
-                         out, q, npix, frame0);
+      hipLaunchKernelGGL((march_frames_kernel<true, true, 0>), grid, dim3(256), 0, stream,
+                         scenes, out, q, npix, frame0);
     else
-      hipLaunchKernelGGL((march_frames_kernel<false, true>), grid, dim3(256), 0, stream, scenes,
-                         out, q, npix, frame0);
+      hipLaunchKernelGGL((march_frames_kernel<false, true, 0>), grid, dim3(256), 0, stream,
+                         scenes, out, q, npix, frame0);
+  } else if (escape) {
+    if (ref)
+      hipLaunchKernelGGL((march_frames_kernel<true, false, kRefSteps>), grid, dim3(256), 0,
+                         stream, scenes, out, q, npix, frame0);
+    else
+      hipLaunchKernelGGL((march_frames_kernel<true, false, 0>), grid, dim3(256), 0, stream,
+                         scenes, out, q, npix, frame0);
   } else {
-    if (escape)
-      hipLaunchKernelGGL((march_frames_kernel<true, false>), grid, dim3(256), 0, stream, scenes,
-                         out, q, npix, frame0);
+    if (ref)
+      hipLaunchKernelGGL((march_frames_kernel<false, false, kRefSteps>), grid, dim3(256), 0,
+                         stream, scenes, out, q, npix, frame0);
     else
-      hipLaunchKernelGGL((march_frames_kernel<false, false>), grid, dim3(256), 0, stream, scenes,
-                         out, q, npix, frame0);
+      hipLaunchKernelGGL((march_frames_kernel<false, false, 0>), grid, dim3(256), 0, stream,
+                         scenes, out, q, npix, frame0);
   }
   return hipGetLastError();
 }

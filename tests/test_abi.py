@@ -150,10 +150,15 @@ def _kernel_metadata():
 
 
 def test_fp32_march_kernels_fit_eight_waves_without_scratch():
+    # every instance (runtime step count and the one compiled for the
+    # reference's 20 steps) of the path, single-frame and batch-grid march of
+    # scenes without escape or spheres
     meta = _kernel_metadata()
     hot = {k: v for k, v in meta.items()
            if re.search(r"(march_frames_kernel|march_kernel|batch_grid_kernel)ILb0ELb0E", k)}
-    assert len(hot) == 3, sorted(meta)
+    for family in ("march_frames_kernel", "12march_kernel", "batch_grid_kernel"):
+        assert any(family in k for k in hot), (family, sorted(meta))
+    assert any("Li20E" in k for k in hot), "no instance for the reference's step count"
     for name, (vgpr, scratch) in hot.items():
         assert vgpr <= 64, (name, vgpr)
         assert scratch == 0, (name, scratch)

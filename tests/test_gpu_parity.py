@@ -121,6 +121,18 @@ def test_fp32_disc_orbit_small_matches_oracle(renderer):
     _check_fp32(renderer, scenes.disc_orbit(480, 270, frames=1), "fp32 disc_orbit 480x270")
 
 
+@pytest.mark.parametrize("nstep", [16, 23])
+def test_fp32_other_step_counts_match_oracle(renderer, nstep):
+    # the FP32 march's instance for a step count known only at run time (the
+    # reference's 20 steps take an instance compiled for them): blocks of four
+    # steps with remainders of 2 and 1 inbound, 3 and 2 outbound (odd sizes:
+    # no pixel centre on the camera's mirror plane, so no texel ties)
+    p = scenes.disc_orbit(317, 179, frames=1)
+    p.options.nstep = nstep
+    print(f"[case] fp32 disc_orbit 317x179 nstep {nstep}", flush=True)
+    _check_fp32(renderer, p, f"fp32 disc_orbit nstep {nstep}")
+
+
 def test_fp32_disc_orbit_1080p_matches_oracle(renderer):
     print("[case] fp32 disc_orbit 1920x1080 frame 0", flush=True)
     st = _check_fp32(renderer, scenes.disc_orbit(frames=1), "fp32 disc_orbit 1080p")
