@@ -387,6 +387,7 @@ struct BatchPlan {
   std::vector<int> fast;       // indices of the scenes in the persistent march, by class
   std::vector<int> solo;       // indices rendered one by one
   uint32_t cls_tile[5] = {0, 0, 0, 0, 0};  // tiles of class c: [cls_tile[c], cls_tile[c+1])
+  int cls_nstep[4] = {0, 0, 0, 0};         // step count shared by class c's scenes (else 0)
   uint32_t n_rays = 0;
 };
 
@@ -419,7 +420,9 @@ int plan_batch(bh_ctx* ctx, int n, const bh_scene* scenes, const bh_options* opt
   p->off.push_back(0);
   for (int c = 0; c < 4; ++c) {
     p->cls_tile[c] = (uint32_t)(rays >> 6);
+    p->cls_nstep[c] = by_cls[c].empty() ? 0 : p->scenes[by_cls[c][0]].nstep;
     for (int i : by_cls[c]) {
+      if (p->scenes[i].nstep != p->cls_nstep[c]) p->cls_nstep[c] = 0;
       const DevScene& d = p->scenes[i];
       rays += (uint64_t)((d.W + 7) / 8) * (uint64_t)((d.H + 7) / 8) * 64u;
       if (rays >= ((uint64_t)1 << 31))
@@ -513,6 +516,7 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
   b.t_start = (unsigned long long*)((char*)ctx->bctl.p + 24);
   b.tile_lo = 0;
   b.tile_hi = p.n_rays >> 6;
+  b.nstep = 0;
   b.band = DevQueue{(uint64_t*)ctx->bband.p, (uint32_t*)ctx->bbandc.p, sub_cap};
   b.refill_steps = t.refill_steps;
   b.refill_idle = t.refill_idle;
@@ -535,6 +539,7 @@ int run_batch(bh_ctx* ctx, const BatchPlan& p, const bh_batch_tuning& t, bh::Bat
         if (p.cls_tile[c + 1] == p.cls_tile[c]) continue;
         bc.tile_lo = p.cls_tile[c];
         bc.tile_hi = p.cls_tile[c + 1];
+        bc.nstep = p.cls_nstep[c];
         bc.head = b.head + c;
         HIP_TRY(ctx, bh::launch_batch(bc, c, t.block, grid, st));
       }

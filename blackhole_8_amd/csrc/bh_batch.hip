@@ -329,7 +329,7 @@ __device__ inline void run_lanes(const BatchDev& b, uint32_t waves) {
 // the per-frame kernel's wave-uniform loop (scene constants in scalar
 // registers).  Returns whether this lane's ray was finished here (not queued
 // for the band pass).
-template <bool ESCAPE, bool SPH>
+template <bool ESCAPE, bool SPH, int NS = 0>
 __device__ inline bool march_tile_xy(const BatchDev& b, const DevScene& s, uint32_t sc,
                                      uint32_t id, int x, int y) {
   if (!(x < s.W && y < s.H)) {
@@ -351,7 +351,7 @@ __device__ inline bool march_tile_xy(const BatchDev& b, const DevScene& s, uint3
     const double du = (r.l - s.u0) * s.inv_nstep;
     setup_lane<float>(s, r.l, r.ib2, du, r.e2x, r.e2y, r.e2z, r.sgn, L);
     if constexpr (SPH) setup_spheres32(s, r, L);
-    march<float, ESCAPE, SPH>(s, L, r.captured, r.first_aa, du, mo);
+    march<float, ESCAPE, SPH, NS>(s, L, r.captured, r.first_aa, du, mo);
     if constexpr (SPH) sredo = L.sredo;
   }
   const bool redo = finish_ray<ESCAPE>(s, b.outs[sc], idx, r, mo, phi0) || sredo;
@@ -395,7 +395,7 @@ __device__ inline void run_tiles(const BatchDev& b, const DevScene* __restrict__
 
 // Grid mode: one wave per tile of [b.tile_lo, b.tile_hi), no queue; the
 // wave's finish time goes to tile_end[tile] (reduced per scene afterwards).
-template <bool ESCAPE, bool SPH>
+template <bool ESCAPE, bool SPH, int NS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 1 : 8)))
 void batch_grid_kernel(BatchDev b, const DevScene* __restrict__ scenes) {
   // the first workgroup is dispatched first: its start is the batch's
@@ -408,7 +408,7 @@ void batch_grid_kernel(BatchDev b, const DevScene* __restrict__ scenes) {
   const uint2 ti = b.tile_info[__builtin_amdgcn_readfirstlane((int)tile)];
   const int lane = lane_id();
   const int x = (int)(ti.y & 0xffffu) * 8 + (lane & 7), y = (int)(ti.y >> 16) * 8 + (lane >> 3);
-  march_tile_xy<ESCAPE, SPH>(b, scenes[ti.x], ti.x, id0 + (uint32_t)lane, x, y);
+  march_tile_xy<ESCAPE, SPH, NS>(b, scenes[ti.x], ti.x, id0 + (uint32_t)lane, x, y);
   if (lane_id() == 0) b.tile_end[tile] = (unsigned long long)wall_clock64();
 }
 
@@ -513,18 +513,33 @@ hipError_t launch_batch(const BatchDev& b, int cls, int block, int grid, hipStre
     const uint32_t tiles = b.tile_hi - b.tile_lo;
     const uint32_t per = (uint32_t)block / 64;
     const dim3 g((tiles + per - 1) / per);
+    // scenes without spheres that all take the reference's step count: the
+    // instance compiled for it
+    const bool ref = b.nstep == kRefSteps;
     switch (cls) {
       case 0:
-        hipLaunchKernelGGL((batch_grid_kernel<false, false>), g, dim3(block), 0, stream, b, b.scenes);
+        if (ref)
+          hipLaunchKernelGGL((batch_grid_kernel<false, false, kRefSteps>), g, dim3(block), 0,
+                             stream, b, b.scenes);
+        else
+          hipLaunchKernelGGL((batch_grid_kernel<false, false, 0>), g, dim3(block), 0, stream, b,
+                             b.scenes);
         break;
       case 1:
-        hipLaunchKernelGGL((batch_grid_kernel<true, false>), g, dim3(block), 0, stream, b, b.scenes);
+        if (ref)
+          hipLaunchKernelGGL((batch_grid_kernel<true, false, kRefSteps>), g, dim3(block), 0,
+                             stream, b, b.scenes);
+        else
+          hipLaunchKernelGGL((batch_grid_kernel<true, false, 0>), g, dim3(block), 0, stream, b,
+                             b.scenes);
         break;
       case 2:
-        hipLaunchKernelGGL((batch_grid_kernel<false, true>), g, dim3(block), 0, stream, b, b.scenes);
+        hipLaunchKernelGGL((batch_grid_kernel<false, true, 0>), g, dim3(block), 0, stream, b,
+                           b.scenes);
         break;
       default:
-        hipLaunchKernelGGL((batch_grid_kernel<true, true>), g, dim3(block), 0, stream, b, b.scenes);
+        hipLaunchKernelGGL((batch_grid_kernel<true, true, 0>), g, dim3(block), 0, stream, b,
+                           b.scenes);
         break;
     }
     return hipGetLastError();

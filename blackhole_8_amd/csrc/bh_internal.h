@@ -192,6 +192,7 @@ struct BatchDev {
   const uint32_t* ray_off;   // n_scenes + 1 entries
   uint32_t n_scenes, n_rays;
   uint32_t tile_lo, tile_hi; // tile-mode launch: the tiles of one scene class
+  int32_t nstep;             // step count shared by the launch's scenes (0: they differ)
   uint32_t* head;            // queue head of this launch (four slots, cleared together)
   DevQueue band;             // FP64 pass: items hold ray ids (low 32 bits) and
                              // outcomes; sub-queue k takes the tiles k, k +
