@@ -513,8 +513,8 @@ hipError_t launch_batch(const BatchDev& b, int cls, int block, int grid, hipStre
     const uint32_t tiles = b.tile_hi - b.tile_lo;
     const uint32_t per = (uint32_t)block / 64;
     const dim3 g((tiles + per - 1) / per);
-    // scenes without spheres that all take the reference's step count: the
-    // instance compiled for it
+    // scenes that all take the reference's step count: the instance compiled
+    // for it
     const bool ref = b.nstep == kRefSteps;
     switch (cls) {
       case 0:
@@ -534,12 +534,20 @@ hipError_t launch_batch(const BatchDev& b, int cls, int block, int grid, hipStre
                              b.scenes);
         break;
       case 2:
-        hipLaunchKernelGGL((batch_grid_kernel<false, true, 0>), g, dim3(block), 0, stream, b,
-                           b.scenes);
+        if (ref)
+          hipLaunchKernelGGL((batch_grid_kernel<false, true, kRefSteps>), g, dim3(block), 0,
+                             stream, b, b.scenes);
+        else
+          hipLaunchKernelGGL((batch_grid_kernel<false, true, 0>), g, dim3(block), 0, stream, b,
+                             b.scenes);
         break;
       default:
-        hipLaunchKernelGGL((batch_grid_kernel<true, true, 0>), g, dim3(block), 0, stream, b,
-                           b.scenes);
+        if (ref)
+          hipLaunchKernelGGL((batch_grid_kernel<true, true, kRefSteps>), g, dim3(block), 0,
+                             stream, b, b.scenes);
+        else
+          hipLaunchKernelGGL((batch_grid_kernel<true, true, 0>), g, dim3(block), 0, stream, b,
+                             b.scenes);
         break;
     }
     return hipGetLastError();

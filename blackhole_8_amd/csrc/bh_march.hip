@@ -232,8 +232,14 @@ hipError_t launch_march_fp32(const DevScene* scene, const DevOut& out, int W, in
   dim3 grid((W + 15) / 16, (H + 15) / 16);
   const bool ref = nstep == kRefSteps;  // the instance compiled for the reference's nstep
   if (spheres) {
-    if (escape)
+    if (escape && ref)
+      hipLaunchKernelGGL((march_kernel<true, true, kRefSteps>), grid, dim3(256), 0, stream,
+                         scene, out, q);
+    else if (escape)
       hipLaunchKernelGGL((march_kernel<true, true, 0>), grid, dim3(256), 0, stream, scene, out, q);
+    else if (ref)
+      hipLaunchKernelGGL((march_kernel<false, true, kRefSteps>), grid, dim3(256), 0, stream,
+                         scene, out, q);
     else
       hipLaunchKernelGGL((march_kernel<false, true, 0>), grid, dim3(256), 0, stream, scene, out,
                          q);
@@ -302,9 +308,15 @@ hipError_t launch_march_frames(const DevScene* scenes, const DevOut& out, int W,
   const size_t npix = (size_t)W * H;
   const bool ref = nstep == kRefSteps;
   if (spheres) {
-    if (escape)
+    if (escape && ref)
+      hipLaunchKernelGGL((march_frames_kernel<true, true, kRefSteps>), grid, dim3(256), 0,
+                         stream, scenes, out, q, npix, frame0);
+    else if (escape)
       hipLaunchKernelGGL((march_frames_kernel<true, true, 0>), grid, dim3(256), 0, stream,
                          scenes, out, q, npix, frame0);
+    else if (ref)
+      hipLaunchKernelGGL((march_frames_kernel<false, true, kRefSteps>), grid, dim3(256), 0,
+                         stream, scenes, out, q, npix, frame0);
     else
       hipLaunchKernelGGL((march_frames_kernel<false, true, 0>), grid, dim3(256), 0, stream,
                          scenes, out, q, npix, frame0);
