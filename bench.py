@@ -18,7 +18,7 @@ instruction stream reaches on this device, measured here (bh_fma_issue_rate),
 and the march's VALU busy fraction from rocprof counters (profiles/).
 `accuracy` compares frames 0, 59 and 119 of the timed graph's own output
 with the FP64 oracle.  `spheres` gives the same roofline for the 720p
-16-sphere scene's march kernel (march_kernel<true,true>) and its whole-frame
+16-sphere scene's march kernel (march_kernel<true,true,20>) and its whole-frame
 time.  `cpu_baseline` is the FP64 oracle (a C restatement
 of the reference march) on this host's cores.
 """
@@ -158,7 +158,7 @@ def cpu_side_by_side(iters: int = 20) -> dict:
 
 def spheres_timing(iters: int = 20) -> dict:
     """The 16-sphere scene (1280x720, FP32, escape to r = 100 with a
-    background): the FP32 march kernel alone (march_kernel<true,true>, iters
+    background): the FP32 march kernel alone (march_kernel<true,true,20>, iters
     back-to-back launches between HIP events on the context stream) against
     the scene's algorithmic work, and the whole frame with its FP64 pass."""
     from blackhole_8_amd import Renderer, flops, scenes
@@ -183,7 +183,7 @@ def spheres_timing(iters: int = 20) -> dict:
             "spheres_in_plane_per_ray": round(float(in_plane.mean()), 3),
             "algorithmic_gflop": round(work / 1e9, 4),
             "roofline": {"bound": "valu",
-                         "kernel": "march_kernel<true,true> (FP32 event march, spheres)",
+                         "kernel": "march_kernel<true,true,20> (FP32 event march, spheres)",
                          "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                          "launch_ms": round(main_ms, 5)},
@@ -371,7 +371,7 @@ def main() -> int:
             valu = None
     roofline = {
         "bound": "valu",
-        "kernel": "march_frames_kernel<false,false> (FP32 event march, all frames in one launch)",
+        "kernel": "march_frames_kernel<false,false,20> (FP32 event march compiled for the reference's 20 steps, all frames in one launch)",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
