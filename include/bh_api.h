@@ -129,7 +129,8 @@ typedef struct bh_scene {
 /* ---- options ----------------------------------------------------------- */
 typedef struct bh_options {
   int32_t precision; /* BH_PREC_*                                           */
-  int32_t nstep;     /* reference: 20 (blackhole_solution_test.cc:202)      */
+  int32_t nstep;     /* reference: 20 (blackhole_solution_test.cc:202); the  */
+                     /* FP32 march has an instance compiled for 20 (fastest) */
   int32_t escape;    /* 0: march ends near the camera radius (reference);
                         1: continue outbound until r >= r_escape or step_cap */
   int32_t step_cap;  /* total steps per ray in escape mode (10000)          */
