@@ -433,10 +433,11 @@ __device__ __forceinline__ bool march_step(const S& s, Lane<T>& L, MarchState<T>
 // Every lane of a wave is at the same step, so the loops run wave-uniform
 // trip counts and keep no per-lane flag: a ray that has ended is disarmed by
 // setting its event thresholds (and sphere windows) to +inf (its results
-// were recorded when it ended; it keeps stepping harmlessly), and every four
-// steps the wave leaves once no lane is armed.  The four steps take one
-// event check between them (block), which keeps the exec-mask bookkeeping of
-// the hot loop to a handful of scalar instructions per four steps.
+// were recorded when it ended; it keeps stepping harmlessly), and after each
+// block of four (or six) steps the wave leaves once no lane is armed.  The
+// steps of a block take one event check between them, which keeps the
+// exec-mask bookkeeping of the hot loop to a handful of scalar instructions
+// per block.
 // The step count of the reference's step rule (blackhole_solution_test.cc
 // and the shipped scene): the FP32 march has an instance compiled for it.
 constexpr int kRefSteps = 20;
@@ -448,6 +449,11 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
   // their step values and chord ends become literal operands (no per-step
   // integer-to-float conversion, no loop bookkeeping)
   const int nstep = NS > 0 ? NS : s.nstep;
+  // block length: six steps in the compiled instance of an FP32 lane
+  // without sphere slots (fewer checks and single tail steps: 20 steps leave
+  // none inbound and one outbound; nine-step blocks measured no faster and
+  // spill), four otherwise (six would cost the sphere lanes occupancy)
+  constexpr int kB = (NS > 0 && sizeof(T) == 4 && !SPH) ? 6 : 4;
   MarchState<T> st{(T)0, 0.0, (T)0};
   o.hit.cls = BH_HIT_NONE;
   o.hit.obj = 0;
@@ -480,46 +486,49 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     coeffs_out<T>(n, nstep, L, &G, &ka, &kb);
     step(n, G, ka, kb, false);
   };
-  // Four steps n0..n0+3 (coeffs: coeffs_in or coeffs_out): the increments
-  // first, then one check of the block.  The swept angle only grows, so a
+  // K steps n0..n0+K-1 (coeffs: coeffs_in or coeffs_out; len carries K,
+  // four or six): the increments first, then one check of the block.  The swept angle only grows, so a
   // crossing threshold or sphere window is passed in the block iff the
   // block's last angle passes it, and the horizon guard iff the largest
   // increment exceeds it; only a lane whose block holds such an event
-  // replays the four steps' tests in order, on angles that rise monotonely
+  // replays the block's step tests in order, on angles that rise monotonely
   // to the block's last one, so every step is tested as one at a time.
   // The increments grow along the inbound steps and shrink along the
   // outbound ones (G falls as u rises below 1/(3M)), so the largest of a
   // block is its last inbound or its first outbound one (the 1e-4 margin of
   // phi_safe covers rounding).
   //
-  // FP32 lanes add the block's four trapezoid increments in FP32 before the
+  // FP32 lanes add the block's trapezoid increments in FP32 before the
   // FP64 sum: with w_k the half-steps w h, the increments are w_{k-1} + w_k
-  // and their sum telescopes to (w_0 + w_4) + 2 (w_1 + w_2 + w_3), four
-  // FP32 operations and one FP64 addition in place of four increments and
-  // four FP64 additions.  The replay forms its intermediate angles as a0
+  // and their sum telescopes to (w_0 + w_K) + 2 (w_1 + ... + w_{K-1}), K
+  // FP32 operations and one FP64 addition in place of K increments and K
+  // FP64 additions.  The replay forms its intermediate angles as a0
   // plus the FP32 partial sums of the increments (monotone, below the
   // block's sum by at least the last increment) and ends on the block's own
   // angle; each chord starts where the previous one ended.  FP64 lanes sum
   // every increment in FP64.
-  auto block = [&](int n0, auto coeffs, bool inbound) {
+  auto block = [&](int n0, auto coeffs, bool inbound, auto len) {
+    constexpr int K = decltype(len)::value;
     const double a0 = st.a;
     if constexpr (sizeof(T) == 4) {
       const T w0 = st.w;
-      T wk[4];
+      T wk[K];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < K; ++k) {
         T G, ka, kb;
         coeffs(n0 + k, &G, &ka, &kb);
         wk[k] = rsq<T>(G);
       }
-      const T mid = (wk[0] + wk[1]) + wk[2];
-      const T sum = __builtin_fmaf(2.f, mid, w0 + wk[3]);
-      st.w = wk[3];
+      T mid = wk[0];
+#pragma unroll
+      for (int k = 1; k < K - 1; ++k) mid += wk[k];
+      const T sum = __builtin_fmaf(2.f, mid, w0 + wk[K - 1]);
+      st.w = wk[K - 1];
       st.a = a0 + (double)sum;
-      const T dlast = wk[2] + wk[3];
+      const T dlast = wk[K - 2] + wk[K - 1];
       // the last increment is read after the block only by end_ray, which
       // an outbound block's successor may call before stepping
-      if (inbound || n0 + 8 > 2 * nstep) st.d = dlast;
+      if (inbound || n0 + 2 * K > 2 * nstep) st.d = dlast;
       const float dmax = inbound ? dlast : w0 + wk[0];
       bool ev = dmax > L.phi_safe || st.a >= L.a_ev;
       if constexpr (SPH) ev = ev || dmax > fmaxf(1.f, L.phi_safe) || st.a >= L.a_sph;
@@ -533,18 +542,19 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
         double ap = ar;
         T part = (T)0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < K; ++k) {
           T G, ka, kb;
           coeffs(n0 + k, &G, &ka, &kb);
           st.d = (k == 0 ? w0 : wk[k - 1]) + wk[k];
           part = k == 0 ? st.d : part + st.d;
-          st.a = k == 3 ? a_end : ar + (double)part;
+          st.a = k == K - 1 ? a_end : ar + (double)part;
           if (step_tests<T, DevScene, SPH>(s, L, st, ap, first_aa, ka, kb, false, o.hit))
             end_ray(n0 + k);
           ap = st.a;
         }
       }
     } else {
+      static_assert(K == 4, "FP64 lanes march four-step blocks");
       T dk[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -582,14 +592,14 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
   int n = 2;
   if constexpr (NS > 0) {
 #pragma unroll
-    for (int b = 0; b < (NS - 2) / 4; ++b) {
-      block(2 + 4 * b, in_coeffs, true);
+    for (int b = 0; b < (NS - 2) / kB; ++b) {
+      block(2 + kB * b, in_coeffs, true, std::integral_constant<int, kB>{});
       if (!__any(armed())) goto done;
     }
-    n = 2 + 4 * ((NS - 2) / 4);
+    n = 2 + kB * ((NS - 2) / kB);
   } else {
     for (; n + 3 < nstep; n += 4) {
-      block(n, in_coeffs, true);
+      block(n, in_coeffs, true, std::integral_constant<int, 4>{});
       if (!__any(armed())) goto done;
     }
   }
@@ -620,14 +630,14 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     n = nstep + 1;
     if constexpr (NS > 0) {
 #pragma unroll
-      for (int b = 0; b < (NS - 1) / 4; ++b) {
-        block(NS + 1 + 4 * b, out_coeffs, false);
+      for (int b = 0; b < (NS - 1) / kB; ++b) {
+        block(NS + 1 + kB * b, out_coeffs, false, std::integral_constant<int, kB>{});
         if (!__any(armed())) goto done;
       }
-      n = NS + 1 + 4 * ((NS - 1) / 4);
+      n = NS + 1 + kB * ((NS - 1) / kB);
     } else {
       for (; n + 3 < 2 * nstep; n += 4) {
-        block(n, out_coeffs, false);
+        block(n, out_coeffs, false, std::integral_constant<int, 4>{});
         if (!__any(armed())) goto done;
       }
     }
