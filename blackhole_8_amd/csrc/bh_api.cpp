@@ -309,7 +309,7 @@ int build_scene(bh_ctx* ctx, const bh_scene* sc, const bh_options* opt_in, DevSc
     d->sphf[i][3] = (float)d->sr[i];
   }
   // Texel-edge guards from the relative error bound g of an FP32 hit point
-  // (the largest measured: 1.43e-6 of the disc's outer radius, 1080p frames
+  // (the largest measured: 1.51e-6 of the disc's outer radius, 1080p frames
   // of the disc orbit) and of an FP32 escape direction: the disc's in texel
   // units, the spheres' and the background's as angles of the direction that
   // picks the texel.
