@@ -202,7 +202,9 @@ struct Hit64 {
   d3 q;
 };
 
-__device__ inline bool find_collision64(const DevScene& s, d3 p1, d3 p2, Hit64* h) {
+// (sphere i is tested when bit i of smask is set; the hole and the disc always)
+__device__ inline bool find_collision64(const DevScene& s, d3 p1, d3 p2, Hit64* h,
+                                        uint32_t smask = ~0u) {
   bool found = false;
   double best = 0;
   d3 q;
@@ -226,6 +228,7 @@ __device__ inline bool find_collision64(const DevScene& s, d3 p1, d3 p2, Hit64* 
     }
   }
   for (int i = 0; i < s.n_spheres; ++i) {
+    if (!((smask >> i) & 1u)) continue;
     if (seg_sphere64(p1, p2, ld3(s.sc[i]), s.sr[i], &q)) {
       d3 d = p1 - q;
       double k = dot(d, d);
@@ -335,6 +338,36 @@ struct Pixel64 {
   int steps;
 };
 
+// The spheres (a bit each) that a chord of the march in frame f can meet.
+// Every point of the march is B + yv r cos(phi) + xv r sin(phi) (matrow with
+// the first component 0) or the camera or B, all in the plane through B
+// spanned by yv and xv (R's second and third columns) up to rounding of
+// ~1e-15 (r + |B|), with r <= reach when du > 0 (u only grows from u0, or
+// the escape march stops short of r_esc).  A sphere whose centre lies farther
+// from that plane than R + tol misses every chord in exact arithmetic by
+// D^2 - R^2 >= 1e-12 L^2, L bounding |q - c| over the march's points q, far
+// above the rounding of seg_sphere64's discriminant (~1e-15 |Q|^2 L^2 against
+// its exact |Q|^2 (R^2 - D^2)), so its test would return false on every
+// chord: skipping it changes no outcome.  With du <= 0 (or not finite) the
+// points are not bounded and every sphere is kept.
+__device__ inline uint32_t plane_spheres64(const DevScene& s, const Frame64& f) {
+  if (!(f.du > 0.0) || !(f.r0 < 1e300)) return ~0u;
+  const d3 yv = mk3(f.R[1], f.R[4], f.R[7]), xv = mk3(f.R[2], f.R[5], f.R[8]);
+  const d3 n = normalize(cross(yv, xv));
+  if (!(dot(n, n) > 0.5)) return ~0u;
+  const d3 B = ld3(s.B);
+  const double reach = s.escape ? fmax(f.r0, s.r_esc) : f.r0;
+  const double bs = sqrt(dot(B, B));
+  uint32_t m = 0u;
+  for (int i = 0; i < s.n_spheres; ++i) {
+    const d3 c = ld3(s.sc[i]) - B;
+    const double R = s.sr[i];
+    const double tol = 1e-6 * (sqrt(dot(c, c)) + reach + R) + 1e-9 * bs;
+    if (!(fabs(dot(n, c)) > R + tol)) m |= 1u << i;
+  }
+  return m;
+}
+
 // The whole march of one pixel (blackhole_solution_test.cc:167-298) with the
 // escape extension.  Returns through px.
 __device__ inline void trace64(const DevScene& s, int x, int y, Pixel64& px) {
@@ -357,6 +390,7 @@ __device__ inline void trace64(const DevScene& s, int x, int y, Pixel64& px) {
   Hit64 h;
   bool hit = false;
   const int nstep = s.nstep;
+  const uint32_t smask = plane_spheres64(s, f);
 
   // One step at the current u (the caller advances u).
   auto step = [&]() -> bool {
@@ -367,7 +401,7 @@ __device__ inline void trace64(const DevScene& s, int x, int y, Pixel64& px) {
     d3 P = matrow(f.R, lv) + B;
     w_prev = w;
     ++steps;
-    if (find_collision64(s, P_prev, P, &h)) return true;
+    if (find_collision64(s, P_prev, P, &h, smask)) return true;
     P_prev2 = P_prev;
     P_prev = P;
     return false;
@@ -382,7 +416,7 @@ __device__ inline void trace64(const DevScene& s, int x, int y, Pixel64& px) {
     hit = step();
   }
   if (!hit && f.captured) {
-    if (find_collision64(s, P_prev, B, &h)) {
+    if (find_collision64(s, P_prev, B, &h, smask)) {
       hit = true;
     } else {
       px.rgb = 0;
