@@ -907,32 +907,36 @@ __device__ inline void setup_ray(const DevScene& s, int x, int y, bool force_bis
   // y = 3 M u and q = (b_c/b)^2, G(u) = 0 reads 3y^2 - 2y^3 = q, a cubic
   // symmetric under y -> 1 - y, q -> 1 - q.  Its root y* in [0, 1) starts
   // from s P(s), s = sqrt(q), for q <= 1/2 and from 1 - s P(s),
-  // s = sqrt(1 - q), above, where P is a least-squares degree-5 fit to y/s on
-  // [0, sqrt(1/2)] (relative error 3e-5): the start is good to 3e-5 of
-  // min(y*, 1 - y*), which cancels the Newton constant's growth as y* -> 1,
-  // and two Newton steps in FP64 bring u* within 1e-15 of the root outside
-  // the photon-sphere band.  The bisection cell containing it is then exact
-  // unless u* sits within 1e-5 cells of a cell edge, where (as in the band)
-  // the reference's bisection is rerun.
+  // s = sqrt(1 - q), above, where P is a least-squares degree-9 fit to y/s on
+  // [0, sqrt(1/2)] (relative error 9e-8): the start, evaluated in FP32 with
+  // 1 - s P(s) formed in FP64, is good to 3e-7 of min(y*, 1 - y*), which
+  // cancels the Newton constant's growth as y* -> 1, and one Newton step in
+  // FP64 brings u* within 1e-13 of the root outside the photon-sphere band
+  // (6e-12 at q = 1 - 1e-7, where two steps from a degree-5 start left
+  // 2e-11).  The bisection cell containing it is then exact unless u* sits
+  // within 1e-5 cells of a cell edge, where (as in the band) the reference's
+  // bisection is rerun.
   bool bisect = force_bisect || r.band || s.root == BH_ROOT_EXACT;
   if (!bisect) {
     const double q = s.bc2 * r.ib2;
     const float qf = (float)q, qc = (float)(1.0 - q);
     const float sq = __builtin_amdgcn_sqrtf(fmaxf(fminf(qf, qc), 0.f));
-    float p = 0.18513675f;
-    p = __builtin_fmaf(p, sq, -0.17374028f);
-    p = __builtin_fmaf(p, sq, 0.12028659f);
-    p = __builtin_fmaf(p, sq, 0.037219696f);
-    p = __builtin_fmaf(p, sq, 0.11216883f);
-    p = __builtin_fmaf(p, sq, 0.57733941f);
-    const float z = sq * p;
-    double u = s.Uc * (double)(qf <= qc ? z : 1.f - z);
-    const double M2 = s.M2, M6 = s.M6;
+    float p = 0.68455708f;
+    p = __builtin_fmaf(p, sq, -1.7155893f);
+    p = __builtin_fmaf(p, sq, 1.9122510f);
+    p = __builtin_fmaf(p, sq, -1.1384754f);
+    p = __builtin_fmaf(p, sq, 0.42832887f);
+    p = __builtin_fmaf(p, sq, -0.063991360f);
+    p = __builtin_fmaf(p, sq, 0.043269463f);
+    p = __builtin_fmaf(p, sq, 0.052839678f);
+    p = __builtin_fmaf(p, sq, 0.11112535f);
+    p = __builtin_fmaf(p, sq, 0.57735020f);
+    const double z = (double)(sq * p);
+    double u = s.Uc * (qf <= qc ? z : 1.0 - z);
     // (the FP32 1/G' leaves the quadratic convergence a 1e-7 linear term)
-#pragma unroll
-    for (int it = 0; it < 2; ++it) {
-      const double g = __builtin_fma(u * u, __builtin_fma(M2, u, -1.0), r.ib2);
-      const double gp = u * __builtin_fma(M6, u, -2.0);
+    {
+      const double g = __builtin_fma(u * u, __builtin_fma(s.M2, u, -1.0), r.ib2);
+      const double gp = u * __builtin_fma(s.M6, u, -2.0);
       const double ig = (double)__builtin_amdgcn_rcpf((float)gp);
       u = __builtin_fma(-g, ig, u);
     }

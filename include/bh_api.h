@@ -315,7 +315,7 @@ int bh_chord_sincos(bh_ctx* ctx, const float* x_host, float* sc_host, int64_t n)
  * (blackhole_solution.h:38-55), per pixel: the turning point l (the lower
  * end of the 20-step bisection cell of the root of G) of every pixel of the
  * frame, as the device's per-ray set-up forms it for the FP32 march (closed
- * form, two Newton steps and the cell, or the bisection itself near a cell
+ * form, one Newton step and the cell, or the bisection itself near a cell
  * edge and in the photon-sphere band); 0 for a captured or degenerate ray.
  * l_host receives width x height doubles, row-major. */
 int bh_turning_points(bh_ctx* ctx, const bh_scene* scene, const bh_options* opt,

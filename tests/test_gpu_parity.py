@@ -476,8 +476,8 @@ def test_error_paths(renderer):
 
 @pytest.mark.parametrize("case", ["disc_orbit", "plain_hole", "turned_away", "near_band"])
 def test_turning_points_match_the_reference_bisection(renderer, case):
-    # the device's per-ray turning point (closed-form start, two FP64 Newton
-    # steps, the 20-step bisection cell; bisection near a cell edge) lies in
+    # the device's per-ray turning point (closed-form start, one FP64 Newton
+    # step, the 20-step bisection cell; bisection near a cell edge) lies in
     # the bisection cell of the reference's SolveG (blackhole_solution.h:
     # 38-55, the oracle's or_solve_g) for the pixel's own impact parameter:
     # the device forms the cell's lower end as l0 + k cell, the reference by
