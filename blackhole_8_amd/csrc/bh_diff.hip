@@ -145,30 +145,35 @@ __device__ inline void diff_frame(const DiffScene& s, int px, int py, Frame<N>& 
   }
 }
 
-// Inbound turning point of G(u) = 2M u^3 - u^2 + 1/b^2 for b >= b_c: the
-// closed form u* = (1/(3M)) (1/2 + sin(asin(2q - 1)/3)), q = 27 M^2 / b^2,
-// evaluated in FP32 (in FP64 within 1e-3 of the photon sphere, where the
-// root is ill-conditioned) and polished by three FP64 Newton steps while
-// G' < 0.
+// Inbound turning point of G(u) = 2M u^3 - u^2 + 1/b^2 for b >= b_c.  With
+// y = 3 M u and q = 27 M^2 / b^2, G = 0 reads 3y^2 - 2y^3 = q; the root
+// starts from s P(s), s = sqrt(q), for q <= 1/2 and from 1 - s P(s),
+// s = sqrt(1 - q), above (P: the degree-9 fit of y/s on [0, sqrt(1/2)],
+// good to 3e-7 of min(y, 1 - y) in FP32) and is polished by two FP64 Newton
+// steps with an FP32 1/G' while G' < 0: within 1e-15 of the root outside
+// the photon-sphere band, and as close as the double root's conditioning
+// allows within it.
 __device__ inline double turning_point(double ib2, double M, double Uc) {
   const double q = fmin(1.0, 27.0 * M * M * ib2);
-  double u;
-  if (q > 0.999) {
-    double sn, cs;
-    sincos_small64(asin(2.0 * q - 1.0) * (1.0 / 3.0), &sn, &cs);
-    u = Uc * (0.5 + sn);
-  } else {
-    float sn, cs;
-    sincos_core(asinf((float)(2.0 * q - 1.0)) * (1.f / 3.f), &sn, &cs);
-    u = Uc * (0.5 + (double)sn);
-  }
+  const float qf = (float)q, qc = (float)(1.0 - q);
+  const float sq = __builtin_amdgcn_sqrtf(fmaxf(fminf(qf, qc), 0.f));
+  float p = 0.68455708f;
+  p = __builtin_fmaf(p, sq, -1.7155893f);
+  p = __builtin_fmaf(p, sq, 1.9122510f);
+  p = __builtin_fmaf(p, sq, -1.1384754f);
+  p = __builtin_fmaf(p, sq, 0.42832887f);
+  p = __builtin_fmaf(p, sq, -0.063991360f);
+  p = __builtin_fmaf(p, sq, 0.043269463f);
+  p = __builtin_fmaf(p, sq, 0.052839678f);
+  p = __builtin_fmaf(p, sq, 0.11112535f);
+  p = __builtin_fmaf(p, sq, 0.57735020f);
+  const double z = (double)(sq * p);
+  double u = Uc * (qf <= qc ? z : 1.0 - z);
 #pragma unroll
-  for (int it = 0; it < 3; ++it) {
+  for (int it = 0; it < 2; ++it) {
     const double g = __builtin_fma(u * u, __builtin_fma(2.0 * M, u, -1.0), ib2);
     const double gp = u * __builtin_fma(6.0 * M, u, -2.0);
-    double ig = __builtin_amdgcn_rcp(gp);
-    ig = ig * __builtin_fma(-gp, ig, 2.0);
-    ig = ig * __builtin_fma(-gp, ig, 2.0);
+    const double ig = (double)__builtin_amdgcn_rcpf((float)gp);
     if (gp < 0.0) u = __builtin_fma(-g, ig, u);
   }
   return u;
