@@ -1198,16 +1198,45 @@ __device__ inline bool refine64(const DevScene& s, Lane<double> L, const RaySetu
   if (n < 1 || (r.captured && n > nstep)) return false;
   if (cls != BH_HIT_DISC && !(ESCAPE && cls == BH_HIT_NONE)) return false;
   double w = 0.0, a = 0.0, a_prev = 0.0, d = 0.0, ka = 0.0, kb = 0.0;
-  for (int k = 1; k <= n; ++k) {
-    if (k == nstep) to_outbound<double>(L);
-    double G;
-    step_coeffs<double>(k, nstep, L, &G, &ka, &kb);
-    const double wn = rsq<double>(G);
+  // the steps in the order of the march, one phase (one step cubic) at a
+  // time; within a phase two steps' G and w are formed together, so that
+  // their dependent FP64 chains overlap (the sums keep their order)
+  auto add = [&](double wn, double ka_n, double kb_n) {
     d = (w + wn) * L.h;
     w = wn;
     a_prev = a;
     a += d;
+    ka = ka_n;
+    kb = kb_n;
+  };
+  auto run = [&](int k0, int k1, auto coeffs) {
+    int k = k0;
+    for (; k + 1 <= k1; k += 2) {
+      double G1, ka1, kb1, G2, ka2, kb2;
+      coeffs(k, &G1, &ka1, &kb1);
+      coeffs(k + 1, &G2, &ka2, &kb2);
+      const double w1 = rsq<double>(G1), w2 = rsq<double>(G2);
+      add(w1, ka1, kb1);
+      add(w2, ka2, kb2);
+    }
+    if (k <= k1) {
+      double G1, ka1, kb1;
+      coeffs(k, &G1, &ka1, &kb1);
+      add(rsq<double>(G1), ka1, kb1);
+    }
+  };
+  run(1, min(n, nstep - 1), [&](int k, double* G, double* ka_n, double* kb_n) {
+    coeffs_in<double>(k, nstep, L, G, ka_n, kb_n);
+  });
+  if (n >= nstep) {
+    to_outbound<double>(L);
+    double G1, ka1, kb1;
+    coeffs_final<double>(nstep, L, &G1, &ka1, &kb1);
+    add(rsq<double>(G1), ka1, kb1);
   }
+  run(nstep + 1, n, [&](int k, double* G, double* ka_n, double* kb_n) {
+    coeffs_out<double>(k, nstep, L, G, ka_n, kb_n);
+  });
   o.steps = n;
   o.a = a;
   o.d = d;
