@@ -128,6 +128,19 @@ void march_frames_kernel(const DevScene* __restrict__ sp, DevOut out, DevQueue q
 // sub-queues packed together (the front segments, indexed through their
 // prefix sum).  The last workgroup to finish clears the frame's counts for
 // the next frame; count[1] of sub-queue 0 counts the finished workgroups.
+// One round of vector loads over the scene record's 64-byte lines, a lane
+// each, before a wave's first ray: a wave of this pass runs nearly alone on
+// its CU after the march has filled L2 with framebuffer lines, so the scalar
+// loads of the record's constants would each wait on memory in turn.
+__device__ __forceinline__ void touch_scene(const DevScene& s, uint32_t lane) {
+  constexpr uint32_t kLines = (sizeof(DevScene) + 63) / 64;
+  static_assert(kLines <= 64, "one line per lane");
+  if (lane < kLines) {
+    const uint32_t v = reinterpret_cast<const uint32_t*>(&s)[lane * 16];
+    __asm__ volatile("" ::"v"(v) : "memory");
+  }
+}
+
 template <bool ESCAPE, bool SPH>
 __device__ __forceinline__ void band_rays(const DevScene& s, const DevOut& out, const DevQueue& q,
                                           uint32_t blk, uint32_t R, uint32_t nband) {
@@ -140,6 +153,7 @@ __device__ __forceinline__ void band_rays(const DevScene& s, const DevOut& out, 
     const uint64_t* items = q.items + (size_t)k * q.sub_cap;
     const uint32_t nb = min(count[0], q.sub_cap);
     const uint32_t ne = min(count[2], q.sub_cap - nb);
+    if (part * 64 < ne) touch_scene(s, lane);
 #pragma unroll 1
     for (uint32_t i = part * 64 + lane; i < ne; i += R * 64) {
       const uint64_t it = items[q.sub_cap - 1u - i];
@@ -157,6 +171,7 @@ __device__ __forceinline__ void band_rays(const DevScene& s, const DevOut& out, 
     }
     __syncthreads();
     const uint32_t total = pre[kQueueSubs];
+    if ((blk - kQueueSubs * R) * 64 < total) touch_scene(s, lane);
 #pragma unroll 1
     for (uint32_t j = (blk - kQueueSubs * R) * 64 + lane; j < total; j += nband * 64) {
       uint32_t lo = 0, hi = kQueueSubs;  // the k with pre[k] <= j < pre[k + 1]
