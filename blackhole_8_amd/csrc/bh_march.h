@@ -487,12 +487,13 @@ __device__ inline void march(const DevScene& s, Lane<T>& L, bool captured, doubl
     step(n, G, ka, kb, false);
   };
   // K steps n0..n0+K-1 (coeffs: coeffs_in or coeffs_out; len carries K,
-  // four or six): the increments first, then one check of the block.  The swept angle only grows, so a
-  // crossing threshold or sphere window is passed in the block iff the
-  // block's last angle passes it, and the horizon guard iff the largest
-  // increment exceeds it; only a lane whose block holds such an event
-  // replays the block's step tests in order, on angles that rise monotonely
-  // to the block's last one, so every step is tested as one at a time.
+  // four or six): the increments first, then one check of the block.  The
+  // swept angle only grows, so a crossing threshold or sphere window is
+  // passed in the block iff the block's last angle passes it, and the horizon
+  // guard iff the largest increment exceeds it; only a lane whose block holds
+  // such an event replays the block's step tests in order, on angles that
+  // rise monotonely to the block's last one, so every step is tested as one
+  // at a time.
   // The increments grow along the inbound steps and shrink along the
   // outbound ones (G falls as u rises below 1/(3M)), so the largest of a
   // block is its last inbound or its first outbound one (the 1e-4 margin of
